@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: the GRAPE-6 force path of kira on one MI355X.
+
+Workload (force-only benchmark): Plummer N = 65536, equal masses, N-body
+units, eps^2 = 1e-8, every i against every j, j predicted from times t_j
+spread uniformly over [t_i - 2^-10, t_i].  One step = one full sweep on the
+device: j predictor + force/jerk/pot/nearest-neighbour kernel + split
+reduction + FP64 nearest-neighbour refinement, inputs resident in HBM.
+
+Prints one JSON line: pair interactions per second (N*(N-1)*steps / time),
+the roofline of the force kernel against the FP32 vector peak (60 flop per
+pair interaction), and the CPU direct-sum baseline measured on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 65536
+EPS2 = 1e-8
+TSPREAD = 2.0 ** -10
+FLOP_PER_PAIR = 60
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (AMD published)
+SEED = 65536
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_input(g, n):
+    """Plummer sphere (scaled with a device pot sweep) plus predictor inputs
+    (a, jerk from a device sweep at t = 0; snap = 0)."""
+    from starlab_amd.ics import makeplummer
+    idx = np.arange(n, dtype=np.int32)
+    z = np.zeros((n, 3))
+
+    def dev_potential(m, x):
+        g.set_j_particles(idx, idx, np.zeros(n), np.ones(n), m, z, z, z, z, x)
+        assert g.self_sweep(n, 0, n, 0.0, 0.0) == 0
+        _, _, pot, _ = g.read_results(n)
+        return 0.5 * float((m * pot).sum())
+
+    m, x, v = makeplummer(n, SEED, potential_fn=dev_potential)
+    g.set_j_particles(idx, idx, np.zeros(n), np.ones(n), m, z, z, z, v, x)
+    assert g.self_sweep(n, 0, n, 0.0, EPS2) == 0
+    acc, jerk, _, _ = g.read_results(n)
+    rng = np.random.default_rng(SEED)
+    tj = -rng.uniform(0.0, TSPREAD, n)
+    g.set_j_particles(idx, idx, tj, np.full(n, TSPREAD), m, z, jerk / 6.0, acc / 2.0, v, x)
+    return dict(m=m, x=x, v=v, a2=acc / 2.0, j6=jerk / 6.0, tj=tj, idx=idx)
+
+
+def cpu_baseline(inp, target_s=12.0):
+    """The oracle's FP64 direct sum (OpenMP over i, all host threads) on a
+    bounded sample of the same workload: the first ni i-particles against
+    all N j (predicted on the host)."""
+    import oracle
+    n = len(inp["m"])
+    xp, vp = oracle.predict(inp["x"], inp["v"], inp["a2"], inp["j6"], np.zeros((n, 3)),
+                            inp["tj"], 0.0, 4)
+    idx = inp["idx"]
+    ni = 64
+    t0 = time.perf_counter()
+    oracle.force(xp[:ni], vp[:ni], idx[:ni], xp, vp, inp["m"], idx, EPS2)
+    dt = time.perf_counter() - t0
+    ni = int(min(n, max(64, ni * target_s / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    oracle.force(xp[:ni], vp[:ni], idx[:ni], xp, vp, inp["m"], idx, EPS2)
+    dt = time.perf_counter() - t0
+    pairs = ni * (n - 1)
+    return {"value": pairs / dt, "unit": "pair interactions/s", "cores": oracle.threads(),
+            "kind": "port", "seconds": round(dt, 3),
+            "sample": f"{ni} i-particles x {n} j (FP64 direct sum + nn, oracle/g6_oracle.c, "
+                      f"OpenMP over i)"}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "force_n65536_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "libg6 drives one GPU per process; this build measures "
+                                   "N=1 only (multi-GPU sharding is a design note)"}))
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+
+    from starlab_amd.g6 import GRAPE6
+    g = GRAPE6()
+    if g.open() != 0:
+        raise SystemExit(f"g6_open_ failed: {g.last_error()}")
+    n = a.n
+    log(f"bench: generating Plummer N={n}")
+    inp = make_input(g, n)
+
+    for _ in range(a.warmup):
+        assert g.self_sweep(n, 0, n, 0.0, EPS2) == 0
+    g.sync()
+    g.option("timing", 1)
+    g.reset_timing()
+    g.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        assert g.self_sweep(n, 0, n, 0.0, EPS2) == 0
+    g.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g.option("timing", 0)
+
+    pairs_per_step = n * (n - 1)
+    value = pairs_per_step * a.steps / elapsed
+    kcalls, kms = g.kernel_time("force")
+    force_ms = kms / max(kcalls, 1)
+    achieved = FLOP_PER_PAIR * pairs_per_step / (force_ms * 1e-3) / 1e12
+    breakdown = {}
+    for k in ("predict", "force", "reduce", "refine"):
+        c, ms = g.kernel_time(k)
+        breakdown[k] = round(ms / max(c, 1), 4)
+    wg, nsplit, jchunk = g.geometry()
+    acc, jerk, pot, nn = g.read_results(n)
+    refinements = g.refinements()
+
+    out = {
+        "metric": "pair interactions/s at N=64K vs FP32 roofline; ensemble p50/p99 latency",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "force-only: Plummer N=65536, equal masses, N-body units, "
+                               "eps2=1e-8, all i vs all j, t_j spread U[0,2^-10], "
+                               "double-single positions, FP32 pair math, FP64 accumulation",
+                   "n": n, "eps2": EPS2, "pairs_per_step": pairs_per_step},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": load_traffic(),
+                     "kernel": "k_force", "kernel_ms": force_ms,
+                     "flop_per_pair": FLOP_PER_PAIR},
+        "kernel_ms": breakdown,
+        "force_geometry": {"workgroups": wg, "nsplit": nsplit, "jchunk": jchunk},
+        "nn_refinements": refinements,
+    }
+    if not a.no_cpu:
+        log("bench: CPU baseline")
+        cb = cpu_baseline(inp)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,72 @@
+/*
+ * g6x.h -- libg6.so extensions beyond the GRAPE-6 ABI.
+ *
+ * None of these has a counterpart in the reference (kira calls only the
+ * grape6.h entry points).  They exist for callers that hold many particles
+ * at once -- benchmarks, test harnesses, ensemble drivers -- and for
+ * measurement.  All follow the ABI's conventions: C linkage, plain pointers,
+ * double[n][3] AoS vectors, 0 = success, negative = error (see
+ * g6x_last_error()).
+ */
+#ifndef LIBG6_MI355X_G6X_H
+#define LIBG6_MI355X_G6X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bulk form of g6_set_j_particle_: n particles, address[k] / index[k],
+ * tj[k], dtj[k], mass[k], and k18, j6, a2, v, x as double[n][3]. */
+int g6x_set_j_particles(int cluster_id, int n, const int *address,
+                        const int *index, const double *tj, const double *dtj,
+                        const double *mass, const double *k18, const double *j6,
+                        const double *a2, const double *v, const double *x);
+
+/* Device-resident sweep: the j-particles at addresses [i_first, i_first+ni)
+ * are themselves the i-particles, at their device-predicted positions and
+ * velocities for time ti (each excludes itself).  Runs predictor + force +
+ * reduction + nearest-neighbour refinement on the device and leaves the
+ * results there; nothing crosses PCIe.  Asynchronous: g6x_sync() waits. */
+int g6x_self_sweep(int cluster_id, int nj, int i_first, int ni, double ti,
+                   double eps2);
+
+/* Copy the results of the last g6x_self_sweep (ni entries) to the host. */
+int g6x_read_results(int cluster_id, int ni, double *acc, double *jerk,
+                     double *pot, int *nnbindex);
+
+/* Device-predicted j state at the last prediction (double[nj][3] each). */
+int g6x_read_predicted(int cluster_id, int nj, double *xp, double *vp);
+
+/* Wait for all queued device work. */
+int g6x_sync(int cluster_id);
+
+/* Options (also read from the environment at g6_open_):
+ *   "predict_order"  4 (default: a2, j6) or 5 (adds k18)     G6_PREDICT_ORDER
+ *   "npipes"         value returned by g6_npipes_            G6_NPIPES
+ *   "nb_cap"         neighbour entries stored per i          G6_NB_CAP
+ *   "waves_target"   waves the force grid aims for           G6_WAVES
+ *   "nsplit"         force j-split (0 = automatic)           G6_NSPLIT
+ *   "timing"         1 = time each kernel with HIP events    G6_TIMING
+ * Returns 0, or -1 for an unknown key. */
+int       g6x_set_option(const char *key, long long value);
+long long g6x_get_option(const char *key);
+
+/* Per-kernel event timing, accumulated since the last reset (needs
+ * "timing" = 1).  kernel: 0 predictor, 1 force, 2 reduce, 3 nn refine,
+ * 4 neighbour list, 5 j scatter.  Returns launches; *ms = total ms. */
+int  g6x_kernel_time(int kernel, double *ms);
+void g6x_reset_timing(void);
+
+/* Geometry of the last force launch: workgroups, j-split, j per split. */
+int  g6x_last_force_geometry(int *workgroups, int *nsplit, int *jchunk);
+
+/* Number of nearest-neighbour refinements (FP64 rescans) in the last call. */
+int  g6x_last_refinements(void);
+
+/* Text of the last error (empty if none). */
+const char *g6x_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""CPU oracle for the GRAPE-6 force path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package, and only as the checker.  The product
+(``starlab_amd`` / ``libg6.so``) never imports, links or calls it.
+
+``liboracle_g6.so`` is built from ``g6_oracle.c`` (FP64, no FMA contraction)
+by ``oracle/Makefile``.  ``oracle/_ref/libharp3sim.so`` is the reference's
+own IEEE GRAPE-4 simulator (``src/harpsims/harp3fastsim.c``) compiled from
+its source; it exists only where the reference tree was present at build time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(HERE, "liboracle_g6.so")
+_REF = os.path.join(HERE, "_ref", "libharp3sim.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle (and oracle/_ref when the reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.or_ran2_seed.argtypes = [C.c_int]
+        L.or_ran2_seed.restype = C.c_int
+        L.or_randinter.argtypes = [C.c_double, C.c_double]
+        L.or_randinter.restype = C.c_double
+        L.or_n_rand.restype = C.c_int
+        L.or_threads.restype = C.c_int
+        L.or_makeplummer.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.or_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 C.c_double, C.c_int, _dp, _dp]
+        L.or_force.argtypes = [C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp,
+                               _ip, C.c_double, _dp, _dp, _dp, _ip, _dp]
+        L.or_force_ld.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
+                                  _ip, C.c_double, _dp, _dp, _dp]
+        L.or_neighbours.argtypes = [C.c_int, _dp, _dp, _ip, C.c_int, C.c_int,
+                                    _dp, _ip, C.c_int, _ip, _ip]
+        L.or_energies.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _d(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# -- random numbers / initial conditions ------------------------------------
+
+def ran2_sequence(seed: int, n: int, a: float = 0.0, b: float = 1.0) -> np.ndarray:
+    L = lib()
+    L.or_ran2_seed(seed)
+    return np.array([L.or_randinter(a, b) for _ in range(n)])
+
+
+def makeplummer(n: int, seed: int, scale: bool = True):
+    m = np.zeros(n)
+    x = np.zeros((n, 3))
+    v = np.zeros((n, 3))
+    lib().or_makeplummer(n, seed, int(scale), m, x, v)
+    return m, x, v
+
+
+# -- predictor / force / neighbours / energies ------------------------------
+
+def predict(x, v, a2, j6, k18, tj, ti: float, order: int = 4):
+    x = _d(x); nj = x.shape[0]
+    xp = np.zeros((nj, 3)); vp = np.zeros((nj, 3))
+    lib().or_predict(nj, x, _d(v), _d(a2), _d(j6), _d(k18), _d(tj), ti, order, xp, vp)
+    return xp, vp
+
+
+def force(xi, vi, excl, xj, vj, mj, jindex, eps2: float):
+    xi = _d(xi); ni = xi.shape[0]; nj = _d(xj).shape[0]
+    acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)
+    nn = np.zeros(ni, np.int32); dnn2 = np.zeros(ni)
+    lib().or_force(ni, xi, _d(vi), _i(excl), nj, _d(xj), _d(vj), _d(mj),
+                   _i(jindex), eps2, acc, jerk, pot, nn, dnn2)
+    return acc, jerk, pot, nn, dnn2
+
+
+def force_ld(xi, vi, excl: int, xj, vj, mj, jindex, eps2: float):
+    acc = np.zeros(3); jerk = np.zeros(3); pot = np.zeros(1)
+    lib().or_force_ld(_d(xi), _d(vi), int(excl), _d(xj).shape[0], _d(xj), _d(vj),
+                      _d(mj), _i(jindex), eps2, acc, jerk, pot)
+    return acc, jerk, pot[0]
+
+
+def neighbours(xi, h2, excl, include_self: bool, xj, jindex, cap: int):
+    xi = _d(xi); ni = xi.shape[0]
+    counts = np.zeros(ni, np.int32)
+    lists = np.full(ni * max(cap, 1), -1, np.int32)
+    lib().or_neighbours(ni, xi, _d(h2), _i(excl), int(include_self), _d(xj).shape[0],
+                        _d(xj), _i(jindex), cap, counts, lists)
+    return counts, lists.reshape(ni, max(cap, 1))
+
+
+def threads() -> int:
+    return lib().or_threads()
+
+
+def energies(m, x, v, eps2: float = 0.0):
+    ep = C.c_double(); ek = C.c_double()
+    lib().or_energies(len(m), _d(m), _d(x), _d(v), eps2, C.byref(ep), C.byref(ek))
+    return ep.value, ek.value
+
+
+# -- the reference's own GRAPE-4 IEEE simulator (oracle/_ref) ----------------
+
+class Harp3Sim:
+    """ctypes view of ``harp3fastsim.c`` (GRAPE-4 ABI; 3 pipes, <= 1000 j).
+
+    Conventions of that simulator (harp3fastsim.c:223-322): eps2 and h2 are
+    doubled internally, the neighbour test is on r^2 + 2 eps2 < 2 h2, self is
+    skipped by r^2 > 0, and pot is not computed.
+    """
+
+    NMAX, IPMAX, JPMAX, NNBMAX = 1000, 3, 26, 1024
+
+    def __init__(self):
+        if not os.path.exists(_REF):
+            raise FileNotFoundError(_REF)
+        L = C.CDLL(_REF)
+        self.L = L
+        L.h3setti_.argtypes = [C.POINTER(C.c_double)]
+        L.h3jpdma_indirect_.argtypes = [C.POINTER(C.c_int), _ip, _dp, _dp, _dp,
+                                        _dp, _dp, _dp, C.POINTER(C.c_int)]
+        L.h3calc_firsthalf_.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        _dp, _dp, _dp, _dp]
+        L.h3calc_lasthalf_.argtypes = [C.POINTER(C.c_int), _dp, _dp, _dp]
+        L.h3nblist_.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), _ip]
+        L.h3nblist_.restype = C.c_int
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(_REF)
+
+    def set_j(self, x, v, a2, j6, m, tj):
+        """Load all j (hostindex is 1-based, at most JPMAX per DMA call)."""
+        n = len(m)
+        assert n <= self.NMAX
+        x, v, a2, j6, m, tj = (_d(t) for t in (x, v, a2, j6, m, tj))
+        mode = C.c_int(0)
+        for s in range(0, n, self.JPMAX):
+            cnt = min(self.JPMAX, n - s)
+            idx = _i(np.arange(s + 1, s + cnt + 1))
+            self.L.h3jpdma_indirect_(C.byref(C.c_int(cnt)), idx, x, v, a2, j6, m, tj,
+                                     C.byref(mode))
+
+    def predicted(self, nj):
+        xp = np.ctypeslib.as_array((C.c_double * (self.NMAX * 3)).in_dll(self.L, "xjpred"))
+        vp = np.ctypeslib.as_array((C.c_double * (self.NMAX * 3)).in_dll(self.L, "vjpred"))
+        return xp.reshape(self.NMAX, 3)[:nj].copy(), vp.reshape(self.NMAX, 3)[:nj].copy()
+
+    def calc(self, ti, nj, xi, vi, eps2, h2):
+        """Force on <= 3 i-particles; returns acc, jerk, neighbour lists."""
+        ni = len(xi)
+        assert ni <= self.IPMAX
+        t = C.c_double(ti)
+        self.L.h3setti_(C.byref(t))
+        e = _d(np.full(ni, eps2 / 2.0))
+        h = _d(np.asarray(h2, dtype=np.float64) / 2.0)
+        self.L.h3calc_firsthalf_(C.byref(C.c_int(nj)), C.byref(C.c_int(ni)),
+                                 _d(xi), _d(vi), e, h)
+        acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)
+        self.L.h3calc_lasthalf_(C.byref(C.c_int(ni)), acc, jerk, pot)
+        lists = []
+        for i in range(ni):
+            buf = np.zeros(self.NNBMAX, np.int32)
+            cnt = self.L.h3nblist_(C.byref(C.c_int(0)), C.byref(C.c_int(i)), buf)
+            lists.append(buf[:cnt].copy())
+        return acc, jerk, lists

@@ -1,0 +1,350 @@
+/*
+ * g6_oracle.c -- CPU oracle for the GRAPE-6 force path (TEST INFRASTRUCTURE).
+ *
+ * Only the test suite, the smoke check and the CPU-baseline leg of the
+ * benchmark load this code, and only as the checker.  The product library
+ * never links or calls it.
+ *
+ * Build: see oracle/Makefile (always -O2 -ffp-contract=off: every FP64
+ * expression rounds as written, in the order written).
+ *
+ * Parity pins (tests/test_oracle.py): two-body closed form, Newton's third
+ * law, Galilean/translation invariance, jerk = d(acc)/dt by finite
+ * differences, long-double brute force, the IEEE GRAPE-4 simulator built
+ * from the reference sources (oracle/_ref), Taylor-series identities of the
+ * predictor, brute-force neighbour sets, ran2 against the published
+ * Numerical Recipes test values' structure and Plummer invariants.
+ */
+#include "g6_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* ran2: Numerical Recipes (2nd ed.) long-period generator, as used by      */
+/* Starlab's randinter (src/std/randinter.C:257-324).  Constants are the    */
+/* published ones; EPS is Starlab's 1e-14 (randinter.C:255), not NR's.       */
+/* ------------------------------------------------------------------------ */
+
+#define R2_M1   2147483563
+#define R2_M2   2147483399
+#define R2_A1   40014
+#define R2_A2   40692
+#define R2_Q1   53668
+#define R2_Q2   52774
+#define R2_R1   12211
+#define R2_R2   3791
+#define R2_TAB  32
+#define R2_DIV  (1 + (R2_M1 - 1) / R2_TAB)
+#define R2_MAX  (1.0 - 1.e-14)
+
+static int r2_state = 0;          /* randx in randinter.C:49 */
+static int r2_state2 = 123456789; /* idum2 */
+static int r2_y = 0;
+static int r2_v[R2_TAB];
+static int r2_calls = 0;
+
+static int lcg_step(int s, int a, int q, int r, int m)
+{
+    /* Schrage's method: a*s mod m without overflow */
+    int k = s / q;
+    s = a * (s - k * q) - k * r;
+    if (s < 0) s += m;
+    return s;
+}
+
+static double ran2_next(void)
+{
+    if (r2_state <= 0) {                 /* (re)initialise the shuffle table */
+        int s = -r2_state;
+        if (s < 1) s = 1;
+        r2_state2 = s;
+        for (int t = R2_TAB + 7; t >= 0; t--) {
+            s = lcg_step(s, R2_A1, R2_Q1, R2_R1, R2_M1);
+            if (t < R2_TAB) r2_v[t] = s;
+        }
+        r2_y = r2_v[0];
+        r2_state = s;
+    }
+    r2_state  = lcg_step(r2_state,  R2_A1, R2_Q1, R2_R1, R2_M1);
+    r2_state2 = lcg_step(r2_state2, R2_A2, R2_Q2, R2_R2, R2_M2);
+    int slot = r2_y / R2_DIV;
+    r2_y = r2_v[slot] - r2_state2;
+    r2_v[slot] = r2_state;
+    if (r2_y < 1) r2_y += R2_M1 - 1;
+    double u = (1.0 / R2_M1) * r2_y;
+    return u > R2_MAX ? R2_MAX : u;
+}
+
+int or_ran2_seed(int seed)
+{
+    /* srandinter(seed, 0): randx = -|seed| forces re-initialisation */
+    int s = abs(seed);
+    r2_state = -s;
+    r2_calls = 0;
+    return s;
+}
+
+double or_randinter(double a, double b)
+{
+    r2_calls++;
+    return a + (b - a) * ran2_next();
+}
+
+int or_n_rand(void) { return r2_calls; }
+
+int or_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plummer model (makeplummer.C:55-214).                                     */
+/* ------------------------------------------------------------------------ */
+
+static void plummer_energies(int n, const double *m, const double *x,
+                             const double *v, double *epot, double *ekin)
+{
+    or_energies(n, m, x, v, 0.0, epot, ekin);
+}
+
+void or_makeplummer(int n, int seed, int scale, double *m, double *x, double *v)
+{
+    const double PI = 3.14159265358979323846;
+    const double TWO_PI = 2.0 * PI;
+    double mfrac = 0.999;                      /* MFRAC_DEFAULT, :50 */
+    double partmass = 1.0 / (double)n;
+
+    or_ran2_seed(seed);
+
+    for (int i = 0; i < n; i++) {
+        m[i] = partmass;
+        double m_min = (i * mfrac) / n;
+        double m_max = ((i + 1) * mfrac) / n;
+        double rrrr = or_randinter(m_min, m_max);
+        double radius = 1 / sqrt(pow(rrrr, -2.0 / 3.0) - 1);
+        double theta = acos(or_randinter(-1.0, 1.0));
+        double phi = or_randinter(0.0, TWO_PI);
+        x[3*i+0] = radius * sin(theta) * cos(phi);
+        x[3*i+1] = radius * sin(theta) * sin(phi);
+        x[3*i+2] = radius * cos(theta);
+
+        double xx = 0.0, yy = 0.1;             /* von Neumann rejection */
+        while (yy > xx * xx * pow(1.0 - xx * xx, 3.5)) {
+            xx = or_randinter(0.0, 1.0);
+            yy = or_randinter(0.0, 0.1);
+        }
+        double velocity = xx * sqrt(2.0) * pow(1.0 + radius * radius, -0.25);
+        theta = acos(or_randinter(-1.0, 1.0));
+        phi = or_randinter(0.0, TWO_PI);
+        v[3*i+0] = velocity * sin(theta) * cos(phi);
+        v[3*i+1] = velocity * sin(theta) * sin(phi);
+        v[3*i+2] = velocity * cos(theta);
+    }
+
+    double xfac = 1 / 1.695;                   /* :186-193 */
+    double vfac = 1 / sqrt(xfac);
+    for (int i = 0; i < 3 * n; i++) { x[i] *= xfac; v[i] *= vfac; }
+
+    /* to_com(): move to the centre-of-mass frame */
+    double mt = 0, cx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        mt += m[i];
+        for (int k = 0; k < 3; k++) {
+            cx[k] += m[i] * x[3*i+k];
+            cv[k] += m[i] * v[3*i+k];
+        }
+    }
+    for (int k = 0; k < 3; k++) { cx[k] /= mt; cv[k] /= mt; }
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) { x[3*i+k] -= cx[k]; v[3*i+k] -= cv[k]; }
+
+    if (scale && n > 1) {
+        /* scale_virial(-0.5) then scale_energy(-0.25), eps = 0 (scale.C:157-190) */
+        double pot, kin;
+        plummer_energies(n, m, x, v, &pot, &kin);
+        double q = -0.5;
+        double vscale = sqrt(q * pot / kin);
+        for (int i = 0; i < 3 * n; i++) v[i] *= vscale;
+        kin = q * pot;
+        double energy = kin + pot;
+        if (energy < 0) {
+            double xscale = energy / -0.25;
+            double vs2 = sqrt(1. / xscale);
+            for (int i = 0; i < 3 * n; i++) { x[i] *= xscale; v[i] *= vs2; }
+        }
+    }
+
+    /* reshuffle_all (:236-253): swap each slot with a random one */
+    for (int i = 0; i < n; i++) {
+        int j = (int)or_randinter(0, n);
+        if (j != i) {
+            double t = m[i]; m[i] = m[j]; m[j] = t;
+            for (int k = 0; k < 3; k++) {
+                t = x[3*i+k]; x[3*i+k] = x[3*j+k]; x[3*j+k] = t;
+                t = v[3*i+k]; v[3*i+k] = v[3*j+k]; v[3*j+k] = t;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* j predictor.  Fields follow g6_set_j_particle_: a2 = a/2, j6 = j/6,       */
+/* k18 = a''/18 (hdyn_grape6.C:246-249, hdyn_ev.C:1076).  With s = ti - tj,  */
+/* the 4th-order form is the reference GRAPE simulator's Horner scheme       */
+/* (src/harpsims/harp3fastsim.c:251-265):                                    */
+/*   xp = ((j6*s + a2)*s + v)*s + x                                          */
+/*   vp = (j6*(1.5 s) + a2)*(2 s) + v                                        */
+/* 5th order replaces j6 by (k18*(0.75 s) + j6) in xp and by (k18*s + j6)    */
+/* in vp, i.e. the Taylor terms (3/4) k18 s^4 and 3 k18 s^3; with k18 = 0    */
+/* both orders round identically.                                            */
+/* ------------------------------------------------------------------------ */
+
+void or_predict(int nj, const double *x, const double *v, const double *a2,
+                const double *j6, const double *k18, const double *tj,
+                double ti, int order, double *xp, double *vp)
+{
+    for (int j = 0; j < nj; j++) {
+        double s = ti - tj[j];
+        double s1 = 1.5 * s, s2 = 2.0 * s;
+        for (int k = 0; k < 3; k++) {
+            int q = 3 * j + k;
+            double cx = j6[q], cv = j6[q];
+            if (order >= 5) {
+                cx = k18[q] * (0.75 * s) + j6[q];
+                cv = k18[q] * s + j6[q];
+            }
+            xp[q] = ((cx * s + a2[q]) * s + v[q]) * s + x[q];
+            vp[q] = (cv * s1 + a2[q]) * s2 + v[q];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Direct sum: accumulate_acc_and_jerk (hdyn_ev.C:2019-2036), evaluated in   */
+/* the same order; d = x_j - x_i; dot products left to right                 */
+/* (starlab_vector.h:68-71).                                                 */
+/* ------------------------------------------------------------------------ */
+
+void or_force(int ni, const double *xi, const double *vi, const int *excl,
+              int nj, const double *xj, const double *vj, const double *mj,
+              const int *jindex, double eps2,
+              double *acc, double *jerk, double *pot, int *nn, double *dnn2)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ni; i++) {
+        double ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0, p = 0;
+        double best = INFINITY;
+        int besti = -1;
+        const double x0 = xi[3*i], y0 = xi[3*i+1], z0 = xi[3*i+2];
+        const double u0 = vi[3*i], v0 = vi[3*i+1], w0 = vi[3*i+2];
+        for (int j = 0; j < nj; j++) {
+            if (jindex[j] == excl[i]) continue;
+            double dx = xj[3*j] - x0, dy = xj[3*j+1] - y0, dz = xj[3*j+2] - z0;
+            double du = vj[3*j] - u0, dv = vj[3*j+1] - v0, dw = vj[3*j+2] - w0;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            double r2inv = 1.0 / (d2 + eps2);
+            double a3 = -3.0 * (dx * du + dy * dv + dz * dw) * r2inv;
+            double mrinv = mj[j] * sqrt(r2inv);
+            p -= mrinv;
+            double mr3inv = mrinv * r2inv;
+            ax += mr3inv * dx; ay += mr3inv * dy; az += mr3inv * dz;
+            jx += mr3inv * (du + a3 * dx);
+            jy += mr3inv * (dv + a3 * dy);
+            jz += mr3inv * (dw + a3 * dz);
+            if (d2 < best) { best = d2; besti = jindex[j]; }   /* strict <, hdyn_inline.C:44 */
+        }
+        acc[3*i] = ax; acc[3*i+1] = ay; acc[3*i+2] = az;
+        jerk[3*i] = jx; jerk[3*i+1] = jy; jerk[3*i+2] = jz;
+        pot[i] = p;
+        if (nn) nn[i] = besti;
+        if (dnn2) dnn2[i] = best;
+    }
+}
+
+void or_force_ld(const double *xi, const double *vi, int excl,
+                 int nj, const double *xj, const double *vj, const double *mj,
+                 const int *jindex, double eps2,
+                 double *acc, double *jerk, double *pot)
+{
+    long double a[3] = {0, 0, 0}, jk[3] = {0, 0, 0}, p = 0;
+    for (int j = 0; j < nj; j++) {
+        if (jindex[j] == excl) continue;
+        long double d[3], w[3];
+        for (int k = 0; k < 3; k++) {
+            d[k] = (long double)xj[3*j+k] - (long double)xi[k];
+            w[k] = (long double)vj[3*j+k] - (long double)vi[k];
+        }
+        long double r2 = d[0]*d[0] + d[1]*d[1] + d[2]*d[2] + (long double)eps2;
+        long double rv = d[0]*w[0] + d[1]*w[1] + d[2]*w[2];
+        long double rinv = 1.0L / sqrtl(r2);
+        long double mr = (long double)mj[j] * rinv;
+        long double mr3 = mr * rinv * rinv;
+        p -= mr;
+        for (int k = 0; k < 3; k++) {
+            a[k] += mr3 * d[k];
+            jk[k] += mr3 * (w[k] - 3.0L * rv / r2 * d[k]);
+        }
+    }
+    for (int k = 0; k < 3; k++) { acc[k] = (double)a[k]; jerk[k] = (double)jk[k]; }
+    *pot = (double)p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Neighbour spheres.                                                        */
+/* ------------------------------------------------------------------------ */
+
+void or_neighbours(int ni, const double *xi, const double *h2, const int *excl,
+                   int include_self, int nj, const double *xj, const int *jindex,
+                   int cap, int *counts, int *lists)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ni; i++) {
+        int c = 0;
+        if (h2[i] > 0) {
+            for (int j = 0; j < nj; j++) {
+                if (!include_self && jindex[j] == excl[i]) continue;
+                double dx = xj[3*j] - xi[3*i];
+                double dy = xj[3*j+1] - xi[3*i+1];
+                double dz = xj[3*j+2] - xi[3*i+2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 < h2[i]) {
+                    if (c < cap) lists[(long)i * cap + c] = jindex[j];
+                    c++;
+                }
+            }
+        }
+        counts[i] = c;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Energies (each pair once).                                                */
+/* ------------------------------------------------------------------------ */
+
+void or_energies(int n, const double *m, const double *x, const double *v,
+                 double eps2, double *epot, double *ekin)
+{
+    double pot = 0, kin = 0;
+#pragma omp parallel for reduction(+:pot) schedule(dynamic, 16)
+    for (int i = 0; i < n; i++) {
+        double pi = 0;
+        for (int j = i + 1; j < n; j++) {
+            double dx = x[3*j] - x[3*i], dy = x[3*j+1] - x[3*i+1], dz = x[3*j+2] - x[3*i+2];
+            pi -= m[j] / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+        }
+        pot += m[i] * pi;
+    }
+    for (int i = 0; i < n; i++)
+        kin += m[i] * (v[3*i] * v[3*i] + v[3*i+1] * v[3*i+1] + v[3*i+2] * v[3*i+2]);
+    *epot = pot;
+    *ekin = 0.5 * kin;
+}

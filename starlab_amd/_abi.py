@@ -1,0 +1,59 @@
+"""ctypes prototypes of every entry point libg6.so exports.
+
+``GRAPE6`` lists the entry points of include/grape6.h (the reference ABI,
+include/grape6.h:18-102 of the Starlab tree); ``G6X`` those of include/g6x.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+_i = C.c_int
+_pi = C.POINTER(C.c_int)
+_pd = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name: (restype, [argtypes])
+GRAPE6 = {
+    "g6_set_tunit_": (None, [_pi]),
+    "g6_set_xunit_": (None, [_pi]),
+    "g6_open_": (_i, [_pi]),
+    "g6_close_": (_i, [_pi]),
+    "g6_open_special": (_i, [_i, _pi]),
+    "g6_open_special2": (_i, [_i, _pi, _i]),
+    "g6_reset_": (_i, [_pi]),
+    "g6_reset_fofpga_": (_i, [_pi]),
+    "g6_npipes_": (_i, []),
+    "g6_set_ti_": (None, [_pi, _pd]),
+    "g6_set_j_particle_": (_i, [_pi, _pi, _pi, _pd, _pd, _pd, _vp, _vp, _vp, _vp, _vp]),
+    "g6calc_firsthalf_": (None, [_pi, _pi, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _pd, _vp]),
+    "g6calc_lasthalf_": (_i, [_pi, _pi, _pi, _vp, _vp, _vp, _pd, _vp, _vp, _vp, _vp]),
+    "g6calc_lasthalf2_": (_i, [_pi, _pi, _pi, _vp, _vp, _vp, _pd, _vp, _vp, _vp, _vp, _vp]),
+    "g6_read_neighbour_list_": (_i, [_pi]),
+    "g6_read_neighbour_list_old_": (_i, [_pi]),
+    "g6_get_neighbour_list_": (_i, [_pi, _pi, _pi, _pi, _vp]),
+    "g6_initialize_jp_buffer_": (_i, [_pi, _pi]),
+    "g6_flush_jp_buffer_": (_i, [_pi]),
+}
+
+G6X = {
+    "g6x_set_j_particles": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "g6x_self_sweep": (_i, [_i, _i, _i, _i, C.c_double, C.c_double]),
+    "g6x_read_results": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
+    "g6x_read_predicted": (_i, [_i, _i, _vp, _vp]),
+    "g6x_sync": (_i, [_i]),
+    "g6x_set_option": (_i, [C.c_char_p, C.c_longlong]),
+    "g6x_get_option": (C.c_longlong, [C.c_char_p]),
+    "g6x_kernel_time": (_i, [_i, _pd]),
+    "g6x_reset_timing": (None, []),
+    "g6x_last_force_geometry": (_i, [_pi, _pi, _pi]),
+    "g6x_last_refinements": (_i, []),
+    "g6x_last_error": (C.c_char_p, []),
+}
+
+
+def declare(lib: C.CDLL) -> None:
+    for table in (GRAPE6, G6X):
+        for name, (res, args) in table.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args

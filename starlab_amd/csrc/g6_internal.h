@@ -1,0 +1,96 @@
+// g6_internal.h -- device data layout and kernel launchers of libg6 (gfx950).
+//
+// HBM layout (one process = one device = one j-memory):
+//   JRaw   jraw[cap]        160 B/j  as stored by g6_set_j_particle_ (FP64)
+//   JPred  jpred[cap]        64 B/j  predicted: double-single position,
+//                                    FP32 velocity and mass (force kernel)
+//   double xp64[3][cap]      24 B/j  predicted FP64 position (nn refinement,
+//                                    neighbour lists; bit-exact distances)
+//   double vp64[3][cap]      24 B/j  predicted FP64 velocity (read-back only)
+//   int    jindex[cap]        4 B/j  the `index` field of each address
+// i-block (npipes slots, SoA so each lane reads its own slot coalesced):
+//   IPred  10 arrays of float/int, FP64 positions 3 arrays, h2
+// Partial sums of the j-split force: 7 FP64 arrays + 3 nn words, [split][i].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace g6 {
+
+// One j-particle as given to g6_set_j_particle_ (20 doubles, 160 B).
+struct JRaw {
+    double x[3], v[3], a2[3], j6[3], k18[3];
+    double tj, m, dtj, pad[2];
+};
+static_assert(sizeof(JRaw) == 160, "JRaw layout");
+
+// One predicted j-particle (16 floats, 64 B: one scalar-cache line).  The
+// ten fields the force loop reads are contiguous so they arrive in two
+// scalar loads.
+struct JPred {
+    float xh, yh, zh;          // position, high FP32 word
+    float xl, yl, zl;          // position, low FP32 word (x = xh + xl)
+    float vx, vy, vz;          // velocity
+    float m;                   // mass
+    float pad[6];
+};
+static_assert(sizeof(JPred) == 64, "JPred layout");
+
+// i-block, structure of arrays with stride `ld` (>= ni, multiple of 64).
+struct IBlock {
+    const float *xh, *yh, *zh, *xl, *yl, *zl, *vx, *vy, *vz;
+    const int *excl;                 // address excluded for this i (-1: none)
+    const double *x64, *y64, *z64;   // FP64 position (refinement, lists)
+    int ld;
+};
+
+// Partial results of one j-split, structure of arrays [split][ld].
+struct Partials {
+    double *c[7];          // ax ay az jx jy jz pot
+    float *best;           // smallest FP32 r^2 (+inf: none)
+    int *btile;            // first address of the 256-j tile holding it
+    float *second;         // runner-up FP32 r^2
+    int ld;                // stride between splits (= IBlock ld)
+};
+
+// Final results (caller layout: double[ni][3] acc, jerk; pot[ni]; nn[ni]).
+struct Results {
+    double *acc, *jerk, *pot;
+    int *nn;
+    int *refine_count;     // device counter of ambiguous nearest neighbours
+    int *refine_list;      // their i slots
+};
+
+constexpr int kTile = 256;           // j per FP32 partial-sum tile
+// Nearest-neighbour ordering by FP32 r^2 is trusted only when the runner-up
+// exceeds the best by more than this relative margin (16 FP32 ulps; the
+// FP32 r^2 of a double-single difference is within ~4 ulps of the exact
+// value); otherwise the i is rescanned in FP64.
+constexpr float kNNBand = 1.9073486328125e-06f;   // 2^-19
+
+// Kernel launchers (g6_kernels.hip).  All enqueue on `s` and return the
+// launch status.
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, const JRaw *staged,
+                          const int *addr, int n);
+hipError_t launch_predict(hipStream_t s, const JRaw *jraw, int nj, double ti,
+                          int order, JPred *jpred, double *xp64, double *vp64,
+                          int cap);
+hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
+                             const double *xp64, int cap, int i_first, int ni,
+                             float *ibuf, int ld);
+hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
+                        const IBlock &ib, int ni, float eps2, int nsplit,
+                        int jchunk, const Partials &p, bool xcd_remap);
+hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
+                         const JPred *jpred, int nj, const IBlock &ib, float eps2,
+                         const int *jindex, const Results &r);
+hipError_t launch_refine(hipStream_t s, const double *xp64, int cap,
+                         const int *jindex, int nj, const IBlock &ib, int ni,
+                         const Results &r);
+hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,
+                         const int *jindex, int nj, const IBlock &ib,
+                         const double *h2, const int *slots, int nslots,
+                         int include_self, int nb_cap, int *counts, int *lists);
+
+}  // namespace g6

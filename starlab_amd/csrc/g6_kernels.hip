@@ -1,0 +1,494 @@
+// g6_kernels.hip -- gfx950 kernels of libg6: j scatter, j predictor, direct-sum
+// force (acc, jerk, pot, nearest neighbour), split reduction, FP64
+// nearest-neighbour refinement and FP64 neighbour lists.
+//
+// Force pair law: accumulate_acc_and_jerk, src/node/dyn/hdyn/evolve/hdyn_ev.C:2019-2036
+//   d = x_j - x_i, w = v_j - v_i, r2 = d.d + eps2
+//   pot -= m/r;  acc += m d / r^3;  jerk += m/r^3 (w - 3 (d.w)/r2 d)
+// Nearest neighbour: update_nn_coll, hdyn_inline.C:34-53 (smallest d.d,
+// strict <, so the lowest address wins ties), excluding the j whose index
+// equals index[i] (hdyn_grape6.C:3462-3468).
+//
+// Precision: positions double-single (hi + lo FP32), pair arithmetic and
+// rsqrt FP32, 256-pair tile partial sums in FP32 folded into FP64
+// accumulators, FP64 across j-splits.  FP64 is also used by the predictor
+// and by the exact nearest-neighbour / neighbour-list checks.
+#include "g6_internal.h"
+
+#include <climits>
+
+namespace g6 {
+
+// ---------------------------------------------------------------------------
+// j-memory update: copy staged records to their addresses.
+// ---------------------------------------------------------------------------
+__global__ void k_scatter(JRaw *__restrict__ jraw, const JRaw *__restrict__ staged,
+                          const int *__restrict__ addr, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) jraw[addr[k]] = staged[k];
+}
+
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, const JRaw *staged,
+                          const int *addr, int n)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s,
+                       jraw, staged, addr, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// j predictor (FP64, no contraction: rounds exactly as the CPU oracle).
+// Horner form of the reference GRAPE simulator, harp3fastsim.c:251-265:
+//   xp = ((c*s + a2)*s + v)*s + x,   vp = (c'*s1 + a2)*s2 + v
+// with s = ti - tj, s1 = 1.5 s, s2 = 2 s, c = c' = j6 (4th order) or
+// c = k18*(0.75 s) + j6, c' = k18*s + j6 (5th order).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void split_ds(double x, float &hi, float &lo)
+{
+    hi = (float)x;
+    lo = (float)(x - (double)hi);
+}
+
+__global__ __launch_bounds__(256) void k_predict(
+    const JRaw *__restrict__ jraw, int nj, double ti, int order,
+    JPred *__restrict__ jpred, double *__restrict__ xp64,
+    double *__restrict__ vp64, int cap)
+{
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const JRaw &r = jraw[j];
+    const double s = ti - r.tj;
+    const double s1 = 1.5 * s, s2 = 2.0 * s;
+    double xp[3], vp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double cx = r.j6[k], cv = r.j6[k];
+        if (order >= 5) {
+            cx = r.k18[k] * (0.75 * s) + r.j6[k];
+            cv = r.k18[k] * s + r.j6[k];
+        }
+        xp[k] = ((cx * s + r.a2[k]) * s + r.v[k]) * s + r.x[k];
+        vp[k] = (cv * s1 + r.a2[k]) * s2 + r.v[k];
+    }
+    JPred p;
+    split_ds(xp[0], p.xh, p.xl);
+    split_ds(xp[1], p.yh, p.yl);
+    split_ds(xp[2], p.zh, p.zl);
+    p.m = (float)r.m;
+    p.vx = (float)vp[0];
+    p.vy = (float)vp[1];
+    p.vz = (float)vp[2];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) p.pad[k] = 0.f;
+    jpred[j] = p;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xp64[(size_t)k * cap + j] = xp[k];
+        vp64[(size_t)k * cap + j] = vp[k];
+    }
+}
+
+hipError_t launch_predict(hipStream_t s, const JRaw *jraw, int nj, double ti,
+                          int order, JPred *jpred, double *xp64, double *vp64,
+                          int cap)
+{
+    if (nj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_predict, dim3((nj + 255) / 256), dim3(256), 0, s,
+                       jraw, nj, ti, order, jpred, xp64, vp64, cap);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// i-block from j-memory (device-resident sweeps): slot k = address i_first+k.
+// ibuf holds 10 arrays of stride ld: xh yh zh xl yl zl vx vy vz excl.
+// ---------------------------------------------------------------------------
+__global__ void k_iprep_self(const JPred *__restrict__ jpred, int i_first, int ni,
+                             float *__restrict__ ibuf, int ld)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ni) return;
+    const JPred p = jpred[i_first + k];
+    ibuf[0 * ld + k] = p.xh;
+    ibuf[1 * ld + k] = p.yh;
+    ibuf[2 * ld + k] = p.zh;
+    ibuf[3 * ld + k] = p.xl;
+    ibuf[4 * ld + k] = p.yl;
+    ibuf[5 * ld + k] = p.zl;
+    ibuf[6 * ld + k] = p.vx;
+    ibuf[7 * ld + k] = p.vy;
+    ibuf[8 * ld + k] = p.vz;
+    reinterpret_cast<int *>(ibuf)[9 * ld + k] = i_first + k;
+}
+
+hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
+                             const double *, int, int i_first, int ni,
+                             float *ibuf, int ld)
+{
+    if (ni <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iprep_self, dim3((ni + 255) / 256), dim3(256), 0, s,
+                       jpred, i_first, ni, ibuf, ld);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Force kernel.  One wave = 64 i-particles (one per lane) against one j-split;
+// j is wave-uniform, so its record comes through scalar loads into SGPRs and
+// every VALU instruction does 64 pair interactions.  j is processed in tiles
+// of kTile: FP32 partial sums per tile, folded into FP64 at the tile end.
+//
+// Nearest neighbour: per lane, the smallest FP32 r^2 (v_min) and the
+// runner-up (v_med3 of best, runner-up, r^2) in each tile; at the tile end
+// the lane remembers which tile holds its best.  The reduction kernel finds
+// the address inside that tile by recomputing r^2 with the same FP32
+// operations.  Positive floats compare exactly, and ties resolve to the
+// lowest address as in the CPU path.
+// ---------------------------------------------------------------------------
+struct Acc32 {
+    float ax, ay, az, jx, jy, jz, kx, ky, kz, p;
+    float best, second;
+};
+
+// FP32 r^2 of a double-single difference.  The reduction kernel's rescan
+// calls the same function, so both round identically.
+__device__ __forceinline__ float ds_r2(float xjh, float yjh, float zjh, float xjl,
+                                       float yjl, float zjl, float xih, float yih,
+                                       float zih, float xil, float yil, float zil,
+                                       float eps2, float &dx, float &dy, float &dz)
+{
+    dx = (xjh - xih) + (xjl - xil);
+    dy = (yjh - yih) + (yjl - yil);
+    dz = (zjh - zih) + (zjl - zil);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+}
+
+template <bool MASK>
+__device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float xih,
+                                     float yih, float zih, float xil, float yil,
+                                     float zil, float vxi, float vyi, float vzi,
+                                     float eps2, Acc32 &a)
+{
+    float dx, dy, dz;
+    const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih, xil,
+                           yil, zil, eps2, dx, dy, dz);
+    const float dvx = J.vx - vxi;
+    const float dvy = J.vy - vyi;
+    const float dvz = J.vz - vzi;
+    const float rv = __builtin_fmaf(dx, dvx, __builtin_fmaf(dy, dvy, dz * dvz));
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float rnn = r2;
+    if (MASK) {
+        const bool self = (unsigned)jj == rel;
+        rinv = self ? 0.0f : rinv;
+        rnn = self ? __builtin_inff() : r2;
+    }
+    const float rinv2 = rinv * rinv;
+    const float mr = J.m * rinv;
+    const float mr3 = mr * rinv2;
+    const float q = mr3 * (rv * rinv2);
+    a.p -= mr;
+    a.ax = __builtin_fmaf(mr3, dx, a.ax);
+    a.ay = __builtin_fmaf(mr3, dy, a.ay);
+    a.az = __builtin_fmaf(mr3, dz, a.az);
+    a.jx = __builtin_fmaf(mr3, dvx, a.jx);
+    a.jy = __builtin_fmaf(mr3, dvy, a.jy);
+    a.jz = __builtin_fmaf(mr3, dvz, a.jz);
+    a.kx = __builtin_fmaf(q, dx, a.kx);
+    a.ky = __builtin_fmaf(q, dy, a.ky);
+    a.kz = __builtin_fmaf(q, dz, a.kz);
+    a.second = __builtin_amdgcn_fmed3f(a.best, a.second, rnn);
+    a.best = __builtin_fminf(a.best, rnn);
+}
+
+template <bool MASK>
+__device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
+                                           unsigned rel, float xih, float yih,
+                                           float zih, float xil, float yil,
+                                           float zil, float vxi, float vyi,
+                                           float vzi, float eps2, Acc32 &a)
+{
+    int jj = 0;
+    for (; jj + 4 <= tn; jj += 4) {
+        const JPred *g = jp + jj;
+        pair<MASK>(g[0], jj + 0, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK>(g[1], jj + 1, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK>(g[2], jj + 2, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK>(g[3], jj + 3, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+    }
+    for (; jj < tn; ++jj)
+        pair<MASK>(jp[jj], jj, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+}
+
+__global__ __launch_bounds__(64) void k_force(
+    const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
+    int jchunk, int n_igroups, Partials P, int remap)
+{
+    int b = blockIdx.x;
+    if (remap) {
+        // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+        // contiguous run of logical workgroups (same j-splits, so its L2
+        // holds only its share of j-memory).
+        const int per = gridDim.x >> 3;
+        b = (b & 7) * per + (b >> 3);
+    }
+    const int split = b / n_igroups;
+    const int ig = b - split * n_igroups;
+    const int lane = threadIdx.x;
+    const int i = ig * 64 + lane;
+    const int ic = i < ni ? i : ni - 1;
+
+    const float xih = ib.xh[ic], yih = ib.yh[ic], zih = ib.zh[ic];
+    const float xil = ib.xl[ic], yil = ib.yl[ic], zil = ib.zl[ic];
+    const float vxi = ib.vx[ic], vyi = ib.vy[ic], vzi = ib.vz[ic];
+    const int excl = i < ni ? ib.excl[ic] : -1;
+
+    const int jbeg = split * jchunk;
+    const int jend = min(jbeg + jchunk, nj);
+
+    double A0 = 0, A1 = 0, A2 = 0, A3 = 0, A4 = 0, A5 = 0, A6 = 0;
+    float rbest = __builtin_inff(), rsecond = __builtin_inff();
+    int rtile = -1;
+
+    for (int t = jbeg; t < jend; t += kTile) {
+        const int tn = min(kTile, jend - t);
+        Acc32 a;
+        a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.kx = a.ky = a.kz = a.p = 0.f;
+        a.best = a.second = __builtin_inff();
+        const unsigned rel = (unsigned)(excl - t);
+        const bool mine = rel < (unsigned)tn;
+        if (__ballot(mine) == 0)
+            tile_pairs<false>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
+                              vxi, vyi, vzi, eps2, a);
+        else
+            tile_pairs<true>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
+                             vxi, vyi, vzi, eps2, a);
+        A0 += (double)a.ax;
+        A1 += (double)a.ay;
+        A2 += (double)a.az;
+        A3 += (double)a.jx - 3.0 * (double)a.kx;
+        A4 += (double)a.jy - 3.0 * (double)a.ky;
+        A5 += (double)a.jz - 3.0 * (double)a.kz;
+        A6 += (double)a.p;
+        if (a.best < rbest) {
+            rsecond = __builtin_fminf(rbest, a.second);
+            rbest = a.best;
+            rtile = t;
+        } else {
+            rsecond = __builtin_fminf(rsecond, a.best);
+        }
+    }
+
+    if (i < ni) {
+        const size_t o = (size_t)split * P.ld + i;
+        P.c[0][o] = A0;
+        P.c[1][o] = A1;
+        P.c[2][o] = A2;
+        P.c[3][o] = A3;
+        P.c[4][o] = A4;
+        P.c[5][o] = A5;
+        P.c[6][o] = A6;
+        P.best[o] = rbest;
+        P.btile[o] = rtile;
+        P.second[o] = rsecond;
+    }
+}
+
+hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
+                        const IBlock &ib, int ni, float eps2, int nsplit,
+                        int jchunk, const Partials &p, bool xcd_remap)
+{
+    if (ni <= 0) return hipSuccess;
+    const int n_igroups = (ni + 63) / 64;
+    const int nwg = n_igroups * nsplit;
+    const int remap = (xcd_remap && nwg % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(k_force, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
+                       eps2, jchunk, n_igroups, p, remap);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reduction over j-splits (fixed split order: deterministic) into the
+// caller's layout.  The nearest neighbour is located inside its tile by
+// recomputing FP32 r^2 exactly as the force kernel did; when the runner-up is
+// within kNNBand of the best, the i is queued for the FP64 rescan.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce(Partials P, int nsplit, int ni,
+                                                const JPred *__restrict__ jpred,
+                                                int nj, IBlock ib, float eps2,
+                                                const int *__restrict__ jindex,
+                                                Results R)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    double A[7] = {0, 0, 0, 0, 0, 0, 0};
+    float best = __builtin_inff(), second = __builtin_inff();
+    int tile = -1;
+    for (int s = 0; s < nsplit; ++s) {
+        const size_t o = (size_t)s * P.ld + i;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) A[c] += P.c[c][o];
+        const float b = P.best[o];
+        if (b < best) {
+            second = __builtin_fminf(best, P.second[o]);
+            best = b;
+            tile = P.btile[o];
+        } else {
+            second = __builtin_fminf(second, b);
+        }
+    }
+    R.acc[3 * i + 0] = A[0];
+    R.acc[3 * i + 1] = A[1];
+    R.acc[3 * i + 2] = A[2];
+    R.jerk[3 * i + 0] = A[3];
+    R.jerk[3 * i + 1] = A[4];
+    R.jerk[3 * i + 2] = A[5];
+    R.pot[i] = A[6];
+
+    int nn = 0;
+    bool ambiguous = tile < 0;
+    if (tile >= 0) {
+        const float xih = ib.xh[i], yih = ib.yh[i], zih = ib.zh[i];
+        const float xil = ib.xl[i], yil = ib.yl[i], zil = ib.zl[i];
+        const int excl = ib.excl[i];
+        const int tend = min(tile + kTile, nj);
+        int found = -1;
+        for (int j = tile; j < tend; ++j) {
+            if (j == excl) continue;
+            const JPred &J = jpred[j];
+            float dx, dy, dz;
+            const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih,
+                                   xil, yil, zil, eps2, dx, dy, dz);
+            if (__float_as_uint(r2) == __float_as_uint(best)) { found = j; break; }
+        }
+        if (found >= 0) nn = jindex[found];
+        else ambiguous = true;
+        if (!(second > best * (1.0f + kNNBand))) ambiguous = true;
+    }
+    R.nn[i] = nn;
+    if (ambiguous && nj > 1) {
+        const int f = atomicAdd(R.refine_count, 1);
+        R.refine_list[f] = i;
+    }
+}
+
+hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
+                         const JPred *jpred, int nj, const IBlock &ib, float eps2,
+                         const int *jindex, const Results &r)
+{
+    if (ni <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce, dim3((ni + 255) / 256), dim3(256), 0, s, p,
+                       nsplit, ni, jpred, nj, ib, eps2, jindex, r);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 nearest-neighbour rescan for flagged i (one workgroup per i).
+// d^2 = dx*dx + dy*dy + dz*dz evaluated exactly as the FP64 CPU path.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_refine(const double *__restrict__ xp64,
+                                                int cap, const int *__restrict__ jindex,
+                                                int nj, IBlock ib, Results R)
+{
+#pragma clang fp contract(off)
+    __shared__ double sd[4];
+    __shared__ int sa[4];
+    const int cnt = *R.refine_count;
+    for (int f = blockIdx.x; f < cnt; f += gridDim.x) {
+        const int i = R.refine_list[f];
+        const double xi = ib.x64[i], yi = ib.y64[i], zi = ib.z64[i];
+        const int excl = ib.excl[i];
+        double best = __builtin_inf();
+        int ba = INT_MAX;
+        for (int j = threadIdx.x; j < nj; j += blockDim.x) {
+            if (j == excl) continue;
+            const double dx = xp64[j] - xi;
+            const double dy = xp64[(size_t)cap + j] - yi;
+            const double dz = xp64[2 * (size_t)cap + j] - zi;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < best) { best = d2; ba = j; }
+        }
+        // (d2, address) lexicographic minimum across the workgroup
+        for (int off = 32; off > 0; off >>= 1) {
+            const double od = __shfl_xor(best, off);
+            const int oa = __shfl_xor(ba, off);
+            if (od < best || (od == best && oa < ba)) { best = od; ba = oa; }
+        }
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { sd[w] = best; sa[w] = ba; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+                if (sd[k] < best || (sd[k] == best && sa[k] < ba)) { best = sd[k]; ba = sa[k]; }
+            if (ba != INT_MAX) R.nn[i] = jindex[ba];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_refine(hipStream_t s, const double *xp64, int cap,
+                         const int *jindex, int nj, const IBlock &ib, int ni,
+                         const Results &r)
+{
+    if (ni <= 0) return hipSuccess;
+    const int blocks = ni < 512 ? ni : 512;
+    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(256), 0, s, xp64, cap,
+                       jindex, nj, ib, r);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists (FP64, exact): one wave per listed i-slot; lanes sweep 64
+// consecutive j, ballot + prefix count keeps the list in ascending address.
+// counts[i] is the true count; at most nb_cap entries are stored.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_nblist(
+    const double *__restrict__ xp64, int cap, const int *__restrict__ jindex,
+    int nj, IBlock ib, const double *__restrict__ h2,
+    const int *__restrict__ slots, int nslots, int include_self, int nb_cap,
+    int *__restrict__ counts, int *__restrict__ lists)
+{
+#pragma clang fp contract(off)
+    const int w = blockIdx.x;
+    if (w >= nslots) return;
+    const int i = slots[w];
+    const int lane = threadIdx.x;
+    const double hh = h2[i];
+    const double xi = ib.x64[i], yi = ib.y64[i], zi = ib.z64[i];
+    const int excl = ib.excl[i];
+    int cnt = 0;
+    int *out = lists + (size_t)i * nb_cap;
+    for (int j0 = 0; j0 < nj; j0 += 64) {
+        const int j = j0 + lane;
+        bool in = false;
+        if (j < nj) {
+            const double dx = xp64[j] - xi;
+            const double dy = xp64[(size_t)cap + j] - yi;
+            const double dz = xp64[2 * (size_t)cap + j] - zi;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            in = d2 < hh && (include_self || j != excl);
+        }
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < nb_cap) out[pos] = jindex[j];
+        }
+        cnt += __popcll(m);
+    }
+    if (lane == 0) counts[i] = cnt;
+}
+
+hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,
+                         const int *jindex, int nj, const IBlock &ib,
+                         const double *h2, const int *slots, int nslots,
+                         int include_self, int nb_cap, int *counts, int *lists)
+{
+    if (nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nblist, dim3(nslots), dim3(64), 0, s, xp64, cap, jindex,
+                       nj, ib, h2, slots, nslots, include_self, nb_cap, counts,
+                       lists);
+    return hipGetLastError();
+}
+
+}  // namespace g6

@@ -1,0 +1,280 @@
+"""GPU parity: libg6.so (HIP, gfx950) through its C ABI against the FP64
+CPU oracle on identical inputs.
+
+Bars (see tests/gpu_common.py): predicted positions and velocities bitwise
+equal; pot relative error <= 1e-6; acc and jerk normwise error (|delta| over
+the sum of the magnitudes of the pair terms) <= 1e-6; nearest-neighbour
+index and neighbour-list membership and order exactly equal.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.gpu_common import (ACC_TOL, JERK_TOL, POT_TOL, compare, load_j,
+                              plummer_state)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g():
+    from starlab_amd.g6 import GRAPE6
+    dev = GRAPE6()
+    assert dev.open() == 0, dev.last_error()
+    yield dev
+    dev.close()
+
+
+def _predict(s, ti, order=4):
+    return oracle.predict(s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["tj"], ti, order)
+
+
+def _check(errs, label):
+    print(label, errs, flush=True)
+    assert errs["pot_rel"] <= POT_TOL, errs
+    assert errs["acc_norm"] <= ACC_TOL, errs
+    assert errs["jerk_norm"] <= JERK_TOL, errs
+    assert errs["nn_mismatch"] == 0, errs
+
+
+def _abi_force_vs_oracle(g, s, ti, eps2, isel, order=4, excl=None, xi=None, vi=None):
+    nj = len(s["m"])
+    g.set_ti(ti)
+    xp, vp = _predict(s, ti, order)
+    idx = s["idx"]
+    xi = xp[isel] if xi is None else xi
+    vi = vp[isel] if vi is None else vi
+    ex = idx[isel] if excl is None else np.asarray(excl, np.int32)
+    acc, jerk, pot, nn = g.force(nj, ex, xi, vi, eps2)
+    ref = oracle.force(xi, vi, ex, xp, vp, s["m"], idx, eps2)
+    ex_addr = np.array([e if 0 <= e < nj else -1 for e in ex])
+    return compare(acc, jerk, pot, nn, ref, xi, vi, s["m"], xp, vp, eps2, ex_addr)
+
+
+def test_predictor_bitwise(g):
+    s = plummer_state(1000, seed=3, k18=True)
+    load_j(g, s)
+    for order in (4, 5):
+        g.option("predict_order", order)
+        ti = 0.0
+        g.set_ti(ti)
+        idx = s["idx"][:4]
+        g.force(1000, idx, s["x"][:4], s["v"][:4], 1e-8)
+        xp, vp = g.read_predicted(1000)
+        rx, rv = _predict(s, ti, order)
+        np.testing.assert_array_equal(xp, rx)
+        np.testing.assert_array_equal(vp, rv)
+    g.option("predict_order", 4)
+
+
+def test_plummer_1024_per_particle_abi(g):
+    """Config-1 shape: N=1024 Plummer, eps = 0, loaded one particle at a time
+    through g6_set_j_particle_ exactly as kira does."""
+    s = plummer_state(1024, seed=42)
+    load_j(g, s, bulk=False)
+    errs = _abi_force_vs_oracle(g, s, 0.0, 0.0, np.arange(1024))
+    _check(errs, "plummer1024 eps=0")
+
+
+def test_chunking_and_ragged_sizes(g):
+    old = g.npipes()
+    s = plummer_state(1000, seed=5)      # nj not a multiple of the 256-j tile
+    load_j(g, s)
+    try:
+        for npipes, isel in ((100, np.arange(1000)), (7, np.arange(1, 20)), (2048, np.array([999])),
+                             (2048, np.arange(0, 1000, 3))):
+            g.option("npipes", npipes)
+            errs = _abi_force_vs_oracle(g, s, 0.0, 1e-6, isel)
+            _check(errs, f"npipes={npipes} ni={len(isel)}")
+    finally:
+        g.option("npipes", old)
+
+
+def test_tiny_systems(g):
+    for n in (1, 2, 3, 65):
+        s = plummer_state(max(n, 2), seed=11)
+        for k in s:
+            s[k] = s[k][:n]
+        load_j(g, s)
+        acc, jerk, pot, nn = g.force(n, s["idx"], s["x"], s["v"], 0.0)
+        xp, vp = _predict(s, 0.0)
+        racc, rjerk, rpot, rnn, _ = oracle.force(s["x"], s["v"], s["idx"], xp, vp, s["m"],
+                                                s["idx"], 0.0)
+        if n == 1:
+            assert np.all(acc == 0) and np.all(jerk == 0) and pot[0] == 0 and nn[0] == 0
+            continue
+        np.testing.assert_allclose(pot, rpot, rtol=1e-6)
+        np.testing.assert_allclose(acc, racc, rtol=1e-5, atol=1e-6 * np.abs(racc).max())
+        np.testing.assert_array_equal(nn, rnn)
+
+
+def test_predictor_time_spread_and_order5(g):
+    s = plummer_state(2000, seed=7, k18=True)
+    load_j(g, s)
+    ti = 0.0
+    for order in (4, 5):
+        g.option("predict_order", order)
+        errs = _abi_force_vs_oracle(g, s, ti, 1e-8, np.arange(0, 2000, 2), order=order)
+        _check(errs, f"order{order}")
+    g.option("predict_order", 4)
+
+
+def test_exclusion_by_parent_index(g):
+    """Perturbation call (hdyn_grape6.C:3415-3558): two components share the
+    parent's index; that j is excluded for both, positions are off-grid."""
+    s = plummer_state(600, seed=9)
+    load_j(g, s)
+    xp, vp = _predict(s, 0.0)
+    xi = xp[[10, 10, 20]] + np.array([[1e-4, 0, 0], [-1e-4, 0, 0], [0, 0, 0]])
+    vi = vp[[10, 10, 20]]
+    errs = _abi_force_vs_oracle(g, s, 0.0, 0.0, None, excl=[10, 10, 20], xi=xi, vi=vi)
+    _check(errs, "parent-index exclusion")
+    # an index that is not in j-memory excludes nothing
+    xi2 = xp[[5]] + 1e-3
+    errs = _abi_force_vs_oracle(g, s, 0.0, 0.0, None, excl=[123456], xi=xi2, vi=vp[[5]])
+    _check(errs, "no exclusion")
+
+
+def test_nn_ties_on_lattice(g):
+    gr = np.arange(6, dtype=float)
+    x = np.array([[a, b, c] for a in gr for b in gr for c in gr]) * 0.125
+    n = len(x)
+    s = dict(m=np.full(n, 1.0 / n), x=x, v=np.zeros_like(x), a2=np.zeros_like(x),
+             j6=np.zeros_like(x), k18=np.zeros_like(x), tj=np.zeros(n), dtj=np.ones(n),
+             idx=np.arange(n, dtype=np.int32))
+    load_j(g, s)
+    g.set_ti(0.0)
+    acc, jerk, pot, nn = g.force(n, s["idx"], x, s["v"], 0.0)
+    _, _, _, rnn, _ = oracle.force(x, s["v"], s["idx"], x, s["v"], s["m"], s["idx"], 0.0)
+    np.testing.assert_array_equal(nn, rnn)
+    assert g.refinements() > 0        # exact ties must go through the FP64 rescan
+
+
+def test_neighbour_lists_exact(g):
+    s = plummer_state(3000, seed=13)
+    load_j(g, s)
+    ti = 0.0
+    g.set_ti(ti)
+    xp, vp = _predict(s, ti)
+    isel = np.arange(0, 3000, 5)[:512]
+    rng = np.random.default_rng(1)
+    h2 = rng.uniform(0.0, 0.02, len(isel))
+    h2[::4] = 0.0
+    g.calc_firsthalf(3000, s["idx"][isel], xp[isel], vp[isel], 1e-8, h2)
+    rc, *_ = g.calc_lasthalf2(3000, len(isel))
+    assert rc == 0
+    assert g.read_neighbour_list() == 0
+    counts, lists = oracle.neighbours(xp[isel], h2, s["idx"][isel], 1, xp, s["idx"], 4096)
+    for k in range(len(isel)):
+        rc, lst = g.get_neighbour_list(k, 4096)
+        assert rc == 0
+        assert len(lst) == counts[k]
+        np.testing.assert_array_equal(lst, lists[k, :counts[k]])
+        if h2[k] > 0:
+            assert s["idx"][isel][k] in lst      # n_pipes > 48: self is listed
+
+
+def test_neighbour_list_overflow(g):
+    s = plummer_state(2000, seed=17)
+    load_j(g, s)
+    g.set_ti(0.0)
+    xp, vp = _predict(s, 0.0)
+    isel = np.array([0, 1, 2])
+    h2 = np.array([100.0, 0.01, 0.0])              # first one holds every j
+    g.calc_firsthalf(2000, s["idx"][isel], xp[isel], vp[isel], 0.0, h2)
+    assert g.calc_lasthalf2(2000, 3)[0] == 0
+    assert g.read_neighbour_list() == 0            # 2000 <= nb_cap
+    rc, lst = g.get_neighbour_list(0, 256)          # caller buffer too small
+    assert rc == 1 and len(lst) == 256
+    np.testing.assert_array_equal(lst, np.arange(256))
+    rc, lst = g.get_neighbour_list(2, 256)
+    assert rc == 0 and len(lst) == 0
+    old = g.option("nb_cap")
+    g.option("nb_cap", 100)                         # device storage too small
+    try:
+        g.calc_firsthalf(2000, s["idx"][isel], xp[isel], vp[isel], 0.0, h2)
+        assert g.calc_lasthalf2(2000, 3)[0] == 0
+        assert g.read_neighbour_list() > 0
+        rc, lst = g.get_neighbour_list(0, 4096)
+        assert rc == 1
+    finally:
+        g.option("nb_cap", old)
+
+
+def test_lazy_j_updates_and_reload(g):
+    s = plummer_state(1500, seed=19)
+    load_j(g, s)
+    isel = np.arange(0, 1500, 7)
+    errs = _abi_force_vs_oracle(g, s, 0.0, 1e-8, isel)
+    _check(errs, "before update")
+    rng = np.random.default_rng(2)
+    moved = rng.choice(1500, 40, replace=False)
+    for a in moved:
+        s["x"][a] += rng.normal(scale=0.01, size=3)
+        s["tj"][a] = -1e-4
+        g.set_j_particle(int(a), int(a), s["tj"][a], 1.0, s["m"][a], s["k18"][a], s["j6"][a],
+                         s["a2"][a], s["v"][a], s["x"][a])
+    errs = _abi_force_vs_oracle(g, s, 0.0, 1e-8, isel)
+    _check(errs, "after scatter update")
+    # close / reopen (GRAPE release and re-attach): kira reloads j
+    assert g.close() == 0
+    assert g.open() == 0
+    load_j(g, s)
+    errs = _abi_force_vs_oracle(g, s, 0.0, 1e-8, isel)
+    _check(errs, "after reopen")
+    assert g.reset() == 0
+    load_j(g, s)
+    errs = _abi_force_vs_oracle(g, s, 0.0, 1e-8, isel)
+    _check(errs, "after reset")
+
+
+def test_lasthalf_without_firsthalf_is_an_error(g):
+    rc, *_ = g.calc_lasthalf2(10, 1)
+    assert rc != 0
+
+
+def test_self_sweep_matches_abi_and_is_deterministic(g):
+    s = plummer_state(4096, seed=23)
+    load_j(g, s)
+    ti = 0.0
+    assert g.self_sweep(4096, 0, 4096, ti, 1e-8) == 0
+    a1, j1, p1, n1 = g.read_results(4096)
+    assert g.self_sweep(4096, 0, 4096, ti, 1e-8) == 0
+    a2, j2, p2, n2 = g.read_results(4096)
+    np.testing.assert_array_equal(a1, a2)
+    np.testing.assert_array_equal(j1, j2)
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_array_equal(n1, n2)
+    xp, vp = _predict(s, ti)
+    ref = oracle.force(xp, vp, s["idx"], xp, vp, s["m"], s["idx"], 1e-8)
+    errs = compare(a1, j1, p1, n1, ref, xp, vp, s["m"], xp, vp, 1e-8, s["idx"])
+    _check(errs, "self sweep 4096")
+
+
+def test_full_size_65536_sample(g):
+    """The force-only benchmark workload: Plummer N=65536, eps2=1e-8, t_j
+    spread over [0, 2^-10].  Checked on a 256-i sample against the oracle,
+    plus size-independent properties of the whole sweep."""
+    from starlab_amd.ics import makeplummer
+    n = 65536
+    m, x, v = makeplummer(n, 65536, potential_fn=lambda m, x: -0.25 * 2.0)
+    rng = np.random.default_rng(0)
+    s = dict(m=m, x=x, v=v, a2=rng.normal(size=(n, 3)) * 0.5, j6=rng.normal(size=(n, 3)),
+             k18=np.zeros((n, 3)), tj=-rng.uniform(0, 2.0 ** -10, n), dtj=np.ones(n),
+             idx=np.arange(n, dtype=np.int32))
+    load_j(g, s)
+    assert g.self_sweep(n, 0, n, 0.0, 1e-8) == 0
+    acc, jerk, pot, nn = g.read_results(n)
+    xp, vp = _predict(s, 0.0)
+    sel = rng.choice(n, 256, replace=False)
+    ref = oracle.force(xp[sel], vp[sel], s["idx"][sel], xp, vp, m, s["idx"], 1e-8)
+    errs = compare(acc[sel], jerk[sel], pot[sel], nn[sel], ref, xp[sel], vp[sel], m, xp, vp,
+                   1e-8, s["idx"][sel])
+    _check(errs, "65536 sample")
+    # momentum: sum m a = 0 up to the FP32 pair rounding
+    ma = (m[:, None] * acc).sum(0)
+    assert np.abs(ma).max() < 1e-6 * (m[:, None] * np.abs(acc)).sum(0).max()
+    # potential energy agrees with the oracle's pair sum
+    ep, _ = oracle.energies(m, xp, vp, 1e-8)
+    assert abs(0.5 * (m * pot).sum() - ep) < 1e-7 * abs(ep)
+    assert np.all(pot < 0) and np.all((nn >= 0) & (nn < n))
