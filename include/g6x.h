@@ -47,6 +47,9 @@ int g6x_sync(int cluster_id);
  *   "waves_target"   waves the force grid aims for           G6_WAVES
  *   "nsplit"         force j-split (0 = automatic)           G6_NSPLIT
  *   "timing"         1 = time each kernel with HIP events    G6_TIMING
+ *   "force_variant"  -1 auto, 0 scalar-load, 1 LDS-staged     G6_FORCE_VARIANT
+ *   "lds_min_ni"     auto picks the LDS kernel from this ni  G6_LDS_MIN_NI
+ *   "xcd_remap"      1 = keep each j-split on one XCD         G6_XCD_REMAP
  * Returns 0, or -1 for an unknown key. */
 int       g6x_set_option(const char *key, long long value);
 long long g6x_get_option(const char *key);
@@ -57,7 +60,8 @@ long long g6x_get_option(const char *key);
 int  g6x_kernel_time(int kernel, double *ms);
 void g6x_reset_timing(void);
 
-/* Geometry of the last force launch: workgroups, j-split, j per split. */
+/* Geometry of the last force launch: workgroups, j-split, j per split.
+ * Returns the kernel variant (0: scalar-load kernel, 1: LDS-staged). */
 int  g6x_last_force_geometry(int *workgroups, int *nsplit, int *jchunk);
 
 /* Number of nearest-neighbour refinements (FP64 rescans) in the last call. */

@@ -34,11 +34,15 @@ struct Options {
     int predict_order = 4;
     int npipes = 2048;
     int nb_cap = 8192;
-    int waves_target = 8192;
+    int waves_target = 16384;
     int nsplit = 0;
     int timing = 0;
     int device = 0;
     int xcd_remap = 1;
+    int force_variant = -1;   // -1 auto, 0 scalar-load kernel, 1 LDS kernel
+    int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
+                              // slower than the scalar-load kernel at every
+                              // size so far, so off unless asked for)
 };
 
 struct TimedPair { int kernel; hipEvent_t a, b; };
@@ -109,7 +113,7 @@ struct Engine {
     int last_refinements = 0;
     std::vector<double> last_h2;
     int last_include_self = 1;
-    int geom_wg = 0, geom_nsplit = 0, geom_jchunk = 0;
+    int geom_wg = 0, geom_nsplit = 0, geom_jchunk = 0, geom_variant = 0;
 
     // ---- timing -------------------------------------------------------------
     std::vector<hipEvent_t> ev_pool;
@@ -160,6 +164,8 @@ void read_env()
     o.timing = (int)env_ll("G6_TIMING", o.timing);
     o.device = (int)env_ll("G6_DEVICE", o.device);
     o.xcd_remap = (int)env_ll("G6_XCD_REMAP", o.xcd_remap);
+    o.force_variant = (int)env_ll("G6_FORCE_VARIANT", o.force_variant);
+    o.lds_min_ni = (int)env_ll("G6_LDS_MIN_NI", o.lds_min_ni);
     if (o.npipes < 1) o.npipes = 1;
     if (o.nb_cap < 1) o.nb_cap = 1;
 }
@@ -446,10 +452,17 @@ Partials partials_at(int ld, int nsplit)
 
 size_t partials_bytes(int ld, int nsplit) { return (size_t)ld * nsplit * (7 * 8 + 3 * 4); }
 
-// j-split: enough waves to fill the chip, whole tiles per split.
-void choose_split(int ni, int nj, int &nsplit, int &jchunk)
+int force_variant(int ni)
 {
-    const int n_iw = std::max(1, (ni + 63) / 64);
+    if (E.opt.force_variant == 0 || E.opt.force_variant == 1) return E.opt.force_variant;
+    return ni >= E.opt.lds_min_ni ? 1 : 0;
+}
+
+// j-split: enough waves to fill the chip, whole tiles per split.
+void choose_split(int ni, int nj, int variant, int &nsplit, int &jchunk)
+{
+    const int per_wave = force_igroup(variant) / (variant == 1 ? 4 : 1);
+    const int n_iw = std::max(1, (ni + per_wave - 1) / per_wave);
     const int ntiles = std::max(1, (nj + kTile - 1) / kTile);
     int s = E.opt.nsplit > 0 ? E.opt.nsplit
                               : std::max(1, (E.opt.waves_target + n_iw - 1) / n_iw);
@@ -464,18 +477,20 @@ void choose_split(int ni, int nj, int &nsplit, int &jchunk)
 int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r)
 {
     int nsplit, jchunk;
-    choose_split(ni, nj, nsplit, jchunk);
+    const int variant = force_variant(ni);
+    choose_split(ni, nj, variant, nsplit, jchunk);
     const int ld = ib.ld;
     if (ensure_partials(partials_bytes(ld, nsplit))) return -1;
     Partials p = partials_at(ld, nsplit);
     E.geom_nsplit = nsplit;
     E.geom_jchunk = jchunk;
-    E.geom_wg = ((ni + 63) / 64) * nsplit;
+    E.geom_wg = ((ni + force_igroup(variant) - 1) / force_igroup(variant)) * nsplit;
+    E.geom_variant = variant;
     HIPCHK(hipMemsetAsync(E.d_refine, 0, sizeof(int), E.stream));
     {
         KTimer t(K_FORCE);
         HIPCHK(launch_force(E.stream, E.d_jpred, nj, ib, ni, eps2, nsplit, jchunk, p,
-                            E.opt.xcd_remap != 0));
+                            E.opt.xcd_remap != 0, variant));
         t.stop();
     }
     {
@@ -947,6 +962,8 @@ int g6x_set_option(const char *key, long long value)
     else if (k == "nsplit") o.nsplit = std::max(0, (int)value);
     else if (k == "timing") o.timing = (int)value;
     else if (k == "xcd_remap") o.xcd_remap = (int)value;
+    else if (k == "force_variant") o.force_variant = (int)value;
+    else if (k == "lds_min_ni") o.lds_min_ni = (int)value;
     else return -1;
     E.pred_valid = false;
     return 0;
@@ -964,6 +981,8 @@ long long g6x_get_option(const char *key)
     if (k == "nsplit") return o.nsplit;
     if (k == "timing") return o.timing;
     if (k == "xcd_remap") return o.xcd_remap;
+    if (k == "force_variant") return o.force_variant;
+    if (k == "lds_min_ni") return o.lds_min_ni;
     return -1;
 }
 
@@ -984,7 +1003,7 @@ int g6x_last_force_geometry(int *workgroups, int *nsplit, int *jchunk)
     if (workgroups) *workgroups = E.geom_wg;
     if (nsplit) *nsplit = E.geom_nsplit;
     if (jchunk) *jchunk = E.geom_jchunk;
-    return 0;
+    return E.geom_variant;
 }
 
 int g6x_last_refinements(void) { return E.last_refinements; }

@@ -79,9 +79,12 @@ hipError_t launch_predict(hipStream_t s, const JRaw *jraw, int nj, double ti,
 hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
                              const double *xp64, int cap, int i_first, int ni,
                              float *ibuf, int ld);
+// variant 0: scalar-load kernel (64 i per workgroup); 1: LDS-staged kernel
+// (512 i per workgroup).  force_igroup() gives the i per workgroup.
+int force_igroup(int variant);
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
-                        int jchunk, const Partials &p, bool xcd_remap);
+                        int jchunk, const Partials &p, bool xcd_remap, int variant);
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          const JPred *jpred, int nj, const IBlock &ib, float eps2,
                          const int *jindex, const Results &r);

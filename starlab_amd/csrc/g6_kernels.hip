@@ -151,17 +151,23 @@ struct Acc32 {
     float best, second;
 };
 
-// FP32 r^2 of a double-single difference.  The reduction kernel's rescan
-// calls the same function, so both round identically.
+// FP32 r^2 of a double-single difference: ((dz*dz + dy*dy) + dx*dx) + eps2
+// with the two inner sums fused.  Every force variant and the reduction
+// kernel's rescan use this exact sequence, so all round identically (adding
+// eps2 = 0 is exact, which lets a kernel skip that add).
 __device__ __forceinline__ float ds_r2(float xjh, float yjh, float zjh, float xjl,
                                        float yjl, float zjl, float xih, float yih,
                                        float zih, float xil, float yil, float zil,
                                        float eps2, float &dx, float &dy, float &dz)
 {
+#pragma clang fp contract(off)
     dx = (xjh - xih) + (xjl - xil);
     dy = (yjh - yih) + (yjl - yil);
     dz = (zjh - zih) + (zjl - zil);
-    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+    float r2 = dz * dz;
+    r2 = __builtin_fmaf(dy, dy, r2);
+    r2 = __builtin_fmaf(dx, dx, r2);
+    return r2 + eps2;
 }
 
 template <bool MASK>
@@ -170,6 +176,7 @@ __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float
                                      float zil, float vxi, float vyi, float vzi,
                                      float eps2, Acc32 &a)
 {
+#pragma clang fp contract(off)
     float dx, dy, dz;
     const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih, xil,
                            yil, zil, eps2, dx, dy, dz);
@@ -188,7 +195,7 @@ __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float
     const float mr = J.m * rinv;
     const float mr3 = mr * rinv2;
     const float q = mr3 * (rv * rinv2);
-    a.p -= mr;
+    a.p = a.p - mr;
     a.ax = __builtin_fmaf(mr3, dx, a.ax);
     a.ay = __builtin_fmaf(mr3, dy, a.ay);
     a.az = __builtin_fmaf(mr3, dz, a.az);
@@ -295,16 +302,227 @@ __global__ __launch_bounds__(64) void k_force(
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged force kernel.  A workgroup of 256 lanes stages 256-j tiles of
+// predicted j-particles in LDS; every lane reads each j by broadcast into
+// VGPRs and accumulates it onto two i-particles (i and i + 256 of the
+// workgroup's 512).  With j in VGPRs the pair arithmetic is VOP2 FP32
+// add/sub/mul/fmac with vector operands only, which gfx950 issues at twice
+// the rate of forms that read an SGPR.  Two i per lane halve the LDS bytes per
+// pair interaction (40 B per j per lane, shared by two pairs).
+// Per-tile FP32 sums, nearest-neighbour tracking and partial output are the
+// same as k_force.
+// ---------------------------------------------------------------------------
+struct IPos {
+    float xh, yh, zh, xl, yl, zl, vx, vy, vz;
+};
+
+template <bool MASK, bool EPS>
+__device__ __forceinline__ void pair_v(float jxh, float jyh, float jzh, float jxl,
+                                       float jyl, float jzl, float jvx, float jvy,
+                                       float jvz, float jm, int jj, unsigned rel,
+                                       const IPos &I, float eps2v, Acc32 &a)
+{
+#pragma clang fp contract(off)
+    float dx, dy, dz;
+    float r2 = ds_r2(jxh, jyh, jzh, jxl, jyl, jzl, I.xh, I.yh, I.zh, I.xl, I.yl, I.zl,
+                     0.0f, dx, dy, dz);
+    if (EPS) r2 = r2 + eps2v;
+    const float dvx = jvx - I.vx;
+    const float dvy = jvy - I.vy;
+    const float dvz = jvz - I.vz;
+    float rv = dz * dvz;
+    rv = __builtin_fmaf(dy, dvy, rv);
+    rv = __builtin_fmaf(dx, dvx, rv);
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float rnn = r2;
+    if (MASK) {
+        const bool self = (unsigned)jj == rel;
+        rinv = self ? 0.0f : rinv;
+        rnn = self ? __builtin_inff() : r2;
+    }
+    const float rinv2 = rinv * rinv;
+    const float mr = jm * rinv;
+    const float mr3 = mr * rinv2;
+    const float q = mr3 * (rv * rinv2);
+    a.p = a.p - mr;
+    a.ax = __builtin_fmaf(mr3, dx, a.ax);
+    a.ay = __builtin_fmaf(mr3, dy, a.ay);
+    a.az = __builtin_fmaf(mr3, dz, a.az);
+    a.jx = __builtin_fmaf(mr3, dvx, a.jx);
+    a.jy = __builtin_fmaf(mr3, dvy, a.jy);
+    a.jz = __builtin_fmaf(mr3, dvz, a.jz);
+    a.kx = __builtin_fmaf(q, dx, a.kx);
+    a.ky = __builtin_fmaf(q, dy, a.ky);
+    a.kz = __builtin_fmaf(q, dz, a.kz);
+    a.second = __builtin_amdgcn_fmed3f(a.best, a.second, rnn);
+    a.best = __builtin_fminf(a.best, rnn);
+}
+
+constexpr int kLdsWG = 256;   // lanes per workgroup of the LDS kernel
+constexpr int kLdsI = 2;      // i-particles per lane
+
+template <bool M0, bool M1, bool EPS>
+__device__ __forceinline__ void tile_lds(const float4 *__restrict__ sj, int tn,
+                                         unsigned rel0, unsigned rel1, const IPos &I0,
+                                         const IPos &I1, float eps2v, Acc32 &a0,
+                                         Acc32 &a1)
+{
+#pragma unroll 2
+    for (int jj = 0; jj < tn; ++jj) {
+        const float4 A = sj[3 * jj + 0];   // xh yh zh xl
+        const float4 B = sj[3 * jj + 1];   // yl zl vx vy
+        const float4 C = sj[3 * jj + 2];   // vz m - -
+        pair_v<M0, EPS>(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y, jj, rel0, I0,
+                        eps2v, a0);
+        pair_v<M1, EPS>(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y, jj, rel1, I1,
+                        eps2v, a1);
+    }
+}
+
+__device__ __forceinline__ void acc_reset(Acc32 &a)
+{
+    a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.kx = a.ky = a.kz = a.p = 0.f;
+    a.best = a.second = __builtin_inff();
+}
+
+struct Acc64 {
+    double c[7];
+    float best, second;
+    int tile;
+};
+
+__device__ __forceinline__ void acc_fold(Acc64 &A, const Acc32 &a, int t)
+{
+    A.c[0] += (double)a.ax;
+    A.c[1] += (double)a.ay;
+    A.c[2] += (double)a.az;
+    A.c[3] += (double)a.jx - 3.0 * (double)a.kx;
+    A.c[4] += (double)a.jy - 3.0 * (double)a.ky;
+    A.c[5] += (double)a.jz - 3.0 * (double)a.kz;
+    A.c[6] += (double)a.p;
+    if (a.best < A.best) {
+        A.second = __builtin_fminf(A.best, a.second);
+        A.best = a.best;
+        A.tile = t;
+    } else {
+        A.second = __builtin_fminf(A.second, a.best);
+    }
+}
+
+__device__ __forceinline__ IPos load_i(const IBlock &ib, int ic)
+{
+    IPos I;
+    I.xh = ib.xh[ic]; I.yh = ib.yh[ic]; I.zh = ib.zh[ic];
+    I.xl = ib.xl[ic]; I.yl = ib.yl[ic]; I.zl = ib.zl[ic];
+    I.vx = ib.vx[ic]; I.vy = ib.vy[ic]; I.vz = ib.vz[ic];
+    return I;
+}
+
+template <bool EPS>
+__global__ __launch_bounds__(256) void k_force_lds(
+    const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
+    int jchunk, int n_igroups, Partials P, int remap)
+{
+    __shared__ float4 sj[3 * kTile];
+    int b = blockIdx.x;
+    if (remap) {
+        const int per = gridDim.x >> 3;
+        b = (b & 7) * per + (b >> 3);
+    }
+    const int split = b / n_igroups;
+    const int ig = b - split * n_igroups;
+    const int tid = threadIdx.x;
+    const int i0 = ig * (kLdsWG * kLdsI) + tid;
+    const int i1 = i0 + kLdsWG;
+    const IPos I0 = load_i(ib, i0 < ni ? i0 : ni - 1);
+    const IPos I1 = load_i(ib, i1 < ni ? i1 : ni - 1);
+    const int ex0 = i0 < ni ? ib.excl[i0] : -1;
+    const int ex1 = i1 < ni ? ib.excl[i1] : -1;
+    // eps2 in a VGPR: VOP2 arithmetic with an SGPR operand runs at half rate
+    float eps2v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(eps2v) : "s"(eps2));
+
+    const int jbeg = split * jchunk;
+    const int jend = min(jbeg + jchunk, nj);
+
+    Acc64 A0, A1;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) A0.c[c] = A1.c[c] = 0.0;
+    A0.best = A0.second = A1.best = A1.second = __builtin_inff();
+    A0.tile = A1.tile = -1;
+
+    const float4 *g4 = reinterpret_cast<const float4 *>(jpred);
+    for (int t = jbeg; t < jend; t += kTile) {
+        const int tn = min(kTile, jend - t);
+        __syncthreads();                       // previous tile fully read
+        if (tid < tn) {
+            const float4 p0 = g4[4 * (size_t)(t + tid) + 0];   // xh yh zh xl
+            const float4 p1 = g4[4 * (size_t)(t + tid) + 1];   // yl zl vx vy
+            const float4 p2 = g4[4 * (size_t)(t + tid) + 2];   // vz m pad pad
+            sj[3 * tid + 0] = p0;
+            sj[3 * tid + 1] = p1;
+            sj[3 * tid + 2] = p2;
+        }
+        __syncthreads();
+        Acc32 a0, a1;
+        acc_reset(a0);
+        acc_reset(a1);
+        const unsigned rel0 = (unsigned)(ex0 - t);
+        const unsigned rel1 = (unsigned)(ex1 - t);
+        const bool m0 = __ballot(rel0 < (unsigned)tn) != 0;
+        const bool m1 = __ballot(rel1 < (unsigned)tn) != 0;
+        if (!m0 && !m1)
+            tile_lds<false, false, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+        else if (m0 && !m1)
+            tile_lds<true, false, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+        else if (!m0 && m1)
+            tile_lds<false, true, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+        else
+            tile_lds<true, true, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+        acc_fold(A0, a0, t);
+        acc_fold(A1, a1, t);
+    }
+
+    const size_t o0 = (size_t)split * P.ld + i0;
+    if (i0 < ni) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) P.c[c][o0] = A0.c[c];
+        P.best[o0] = A0.best;
+        P.btile[o0] = A0.tile;
+        P.second[o0] = A0.second;
+    }
+    if (i1 < ni) {
+        const size_t o1 = o0 + kLdsWG;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) P.c[c][o1] = A1.c[c];
+        P.best[o1] = A1.best;
+        P.btile[o1] = A1.tile;
+        P.second[o1] = A1.second;
+    }
+}
+
+int force_igroup(int variant) { return variant == 1 ? kLdsWG * kLdsI : 64; }
+
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
-                        int jchunk, const Partials &p, bool xcd_remap)
+                        int jchunk, const Partials &p, bool xcd_remap, int variant)
 {
     if (ni <= 0) return hipSuccess;
-    const int n_igroups = (ni + 63) / 64;
+    const int n_igroups = (ni + force_igroup(variant) - 1) / force_igroup(variant);
     const int nwg = n_igroups * nsplit;
     const int remap = (xcd_remap && nwg % 8 == 0) ? 1 : 0;
-    hipLaunchKernelGGL(k_force, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
-                       eps2, jchunk, n_igroups, p, remap);
+    if (variant == 1) {
+        if (eps2 != 0.0f)
+            hipLaunchKernelGGL(k_force_lds<true>, dim3(nwg), dim3(kLdsWG), 0, s, jpred, nj,
+                               ib, ni, eps2, jchunk, n_igroups, p, remap);
+        else
+            hipLaunchKernelGGL(k_force_lds<false>, dim3(nwg), dim3(kLdsWG), 0, s, jpred, nj,
+                               ib, ni, eps2, jchunk, n_igroups, p, remap);
+    } else {
+        hipLaunchKernelGGL(k_force, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
+                           eps2, jchunk, n_igroups, p, remap);
+    }
     return hipGetLastError();
 }
 
@@ -353,14 +571,24 @@ __global__ __launch_bounds__(256) void k_reduce(Partials P, int nsplit, int ni,
         const float xil = ib.xl[i], yil = ib.yl[i], zil = ib.zl[i];
         const int excl = ib.excl[i];
         const int tend = min(tile + kTile, nj);
+        const uint32_t bb = __float_as_uint(best);
         int found = -1;
-        for (int j = tile; j < tend; ++j) {
-            if (j == excl) continue;
-            const JPred &J = jpred[j];
-            float dx, dy, dz;
-            const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih,
-                                   xil, yil, zil, eps2, dx, dy, dz);
-            if (__float_as_uint(r2) == __float_as_uint(best)) { found = j; break; }
+        // batches of 8 independent record loads keep several in flight
+        for (int j0 = tile; j0 < tend && found < 0; j0 += 8) {
+            uint32_t r2b[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = min(j0 + u, tend - 1);
+                const JPred &J = jpred[j];
+                float dx, dy, dz;
+                r2b[u] = __float_as_uint(ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih,
+                                               zih, xil, yil, zil, eps2, dx, dy, dz));
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = j0 + u;
+                if (found < 0 && j < tend && j != excl && r2b[u] == bb) found = j;
+            }
         }
         if (found >= 0) nn = jindex[found];
         else ambiguous = true;

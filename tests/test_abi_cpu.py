@@ -116,3 +116,22 @@ def test_missing_library_raises(tmp_path, monkeypatch):
     monkeypatch.setattr(starlab_amd, "_lib", None)
     with pytest.raises(starlab_amd.LibraryMissing):
         starlab_amd.load()
+
+
+def test_plain_c_caller_links_and_runs(tmp_path):
+    """A C program written against include/grape6.h links -lg6 with no other
+    library named, exactly as kira's configure probe does (AC_CHECK_LIB(g6,
+    g6_open_), configure.ac:204-212 of the Starlab tree)."""
+    src = tmp_path / "caller.c"
+    src.write_text(
+        '#include "grape6.h"\n#include <stdio.h>\n'
+        "int main(void){int c=0; printf(\"%d\\n\", g6_npipes_());"
+        " int rc = g6_open_(&c); if (rc == 0) g6_close_(&c); printf(\"%d\\n\", rc); return 0;}\n")
+    exe = tmp_path / "caller"
+    libdir = os.path.join(ROOT, "starlab_amd")
+    subprocess.run(["gcc", f"-I{os.path.join(ROOT, 'include')}", str(src), f"-L{libdir}",
+                    f"-Wl,-rpath,{libdir}", "-lg6", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    lines = r.stdout.split()
+    assert lines[0] == "2048"

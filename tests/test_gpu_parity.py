@@ -25,6 +25,14 @@ def g():
     dev.close()
 
 
+@pytest.fixture(params=[0, 1], ids=["smem", "lds"])
+def variant(g, request):
+    """Run a test on the scalar-load kernel and on the LDS-staged kernel."""
+    g.option("force_variant", request.param)
+    yield request.param
+    g.option("force_variant", -1)
+
+
 def _predict(s, ti, order=4):
     return oracle.predict(s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["tj"], ti, order)
 
@@ -67,7 +75,7 @@ def test_predictor_bitwise(g):
     g.option("predict_order", 4)
 
 
-def test_plummer_1024_per_particle_abi(g):
+def test_plummer_1024_per_particle_abi(g, variant):
     """Config-1 shape: N=1024 Plummer, eps = 0, loaded one particle at a time
     through g6_set_j_particle_ exactly as kira does."""
     s = plummer_state(1024, seed=42)
@@ -76,7 +84,7 @@ def test_plummer_1024_per_particle_abi(g):
     _check(errs, "plummer1024 eps=0")
 
 
-def test_chunking_and_ragged_sizes(g):
+def test_chunking_and_ragged_sizes(g, variant):
     old = g.npipes()
     s = plummer_state(1000, seed=5)      # nj not a multiple of the 256-j tile
     load_j(g, s)
@@ -90,7 +98,7 @@ def test_chunking_and_ragged_sizes(g):
         g.option("npipes", old)
 
 
-def test_tiny_systems(g):
+def test_tiny_systems(g, variant):
     for n in (1, 2, 3, 65):
         s = plummer_state(max(n, 2), seed=11)
         for k in s:
@@ -108,7 +116,7 @@ def test_tiny_systems(g):
         np.testing.assert_array_equal(nn, rnn)
 
 
-def test_predictor_time_spread_and_order5(g):
+def test_predictor_time_spread_and_order5(g, variant):
     s = plummer_state(2000, seed=7, k18=True)
     load_j(g, s)
     ti = 0.0
@@ -119,7 +127,7 @@ def test_predictor_time_spread_and_order5(g):
     g.option("predict_order", 4)
 
 
-def test_exclusion_by_parent_index(g):
+def test_exclusion_by_parent_index(g, variant):
     """Perturbation call (hdyn_grape6.C:3415-3558): two components share the
     parent's index; that j is excluded for both, positions are off-grid."""
     s = plummer_state(600, seed=9)
@@ -135,7 +143,7 @@ def test_exclusion_by_parent_index(g):
     _check(errs, "no exclusion")
 
 
-def test_nn_ties_on_lattice(g):
+def test_nn_ties_on_lattice(g, variant):
     gr = np.arange(6, dtype=float)
     x = np.array([[a, b, c] for a in gr for b in gr for c in gr]) * 0.125
     n = len(x)
@@ -233,7 +241,7 @@ def test_lasthalf_without_firsthalf_is_an_error(g):
     assert rc != 0
 
 
-def test_self_sweep_matches_abi_and_is_deterministic(g):
+def test_self_sweep_matches_abi_and_is_deterministic(g, variant):
     s = plummer_state(4096, seed=23)
     load_j(g, s)
     ti = 0.0
