@@ -1,0 +1,79 @@
+/*
+ * g6ens.h -- batched ensemble engine of libg6.so: many independent N-body
+ * systems of uneven N integrated together on one MI355X.
+ *
+ * No counterpart in the reference's C ABI.  The integrator restates kira's
+ * Hermite scheme for each system with one shared, power-of-two time step:
+ *   predictor        include/_dyn_.h:290-295 (predict_loworder)
+ *   force            hdyn_ev.C:2019-2036 (same device pair law as libg6's
+ *                    force kernel: double-single positions, FP32 pairs, FP64
+ *                    sums)
+ *   corrector        hdyn_ev.C:1235-1250 (get_derivs), :1717-1718
+ *   step criterion   hdyn_ev.C:627-655 (Aarseth), :855-887 (halve at most
+ *                    once, double only when commensurate)
+ *   first step       hdyn_ev.C:377-410 (set_first_timestep, eta/16)
+ * The system step is the minimum over its particles of kira's per-particle
+ * step.  Each engine step advances every resident system by its own step;
+ * systems finish independently at their t_end.  Results of a system are
+ * bitwise identical whatever else is resident (its arithmetic depends only on
+ * its own particles and N).
+ *
+ * Continuous batching: g6e_submit queues jobs on the host; g6e_run admits
+ * queued jobs into free slots while the particle pool (sized from an HBM
+ * budget at g6e_open) has room, runs engine steps as HIP-graph replays, and
+ * retires finished systems.  Jobs admitted between g6e_run calls overlap on
+ * the device with those already running.
+ *
+ * Conventions: C linkage, double[n][3] AoS vectors, 0 = success, negative =
+ * error (text in g6e_last_error()).  One engine per process.
+ */
+#ifndef LIBG6_MI355X_G6ENS_H
+#define LIBG6_MI355X_G6ENS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reserve device memory for at most `hbm_budget_bytes` of particle state and
+ * `max_systems` resident systems.  Opens the device if needed. */
+int g6e_open(long long hbm_budget_bytes, int max_systems);
+int g6e_close(void);
+
+/* Queue one system (masses m[n], positions x[n][3], velocities v[n][3]) to
+ * be integrated from t = 0 to t_end with accuracy parameter eta and
+ * softening eps2.  Returns a job id >= 0, or < 0 on error. */
+long long g6e_submit(int n, const double *m, const double *x, const double *v,
+                     double t_end, double eta, double eps2);
+
+/* Admit queued jobs, run up to `steps` engine steps (in graph replays of
+ * the configured length), retire finished systems.  Returns the number of
+ * systems resident or queued after the call (0 = idle), < 0 on error. */
+int g6e_run(int steps);
+
+/* Job ids finished since the last call (at most max); returns the count. */
+int g6e_poll(int max, long long *job_ids);
+
+/* Final state of a finished job: positions, velocities (double[n][3]), and
+ * diagnostics: final time, steps taken, total energy at t = 0 and at the end
+ * (kinetic + potential with the job's softening).  Frees the result. */
+int g6e_result(long long job_id, int n, double *x, double *v, double *t_final,
+               long long *steps, double *e0, double *e1);
+
+/* Engine statistics since g6e_open: engine steps run, pair interactions
+ * evaluated, systems finished, device seconds spent in engine steps. */
+int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
+              double *device_seconds);
+
+/* Options: "steps_per_graph" (default 8), "jsplit_target" (j per split
+ * within a system, default 512; 0 = no split), "max_split" (default 16),
+ * "lookahead" (queued jobs examined per admission pass, default 64).
+ * Returns 0, or -1 for an unknown key. */
+int       g6e_set_option(const char *key, long long value);
+long long g6e_get_option(const char *key);
+
+const char *g6e_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

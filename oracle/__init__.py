@@ -55,6 +55,11 @@ def lib():
                                     _dp, _ip, C.c_int, _ip, _ip]
         L.or_energies.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_hermite_shared.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_longlong,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_hermite_shared.restype = C.c_int
         _lib = L
     return _lib
 
@@ -116,6 +121,17 @@ def neighbours(xi, h2, excl, include_self: bool, xj, jindex, cap: int):
     lib().or_neighbours(ni, xi, _d(h2), _i(excl), int(include_self), _d(xj).shape[0],
                         _d(xj), _i(jindex), cap, counts, lists)
     return counts, lists.reshape(ni, max(cap, 1))
+
+
+def hermite_shared(m, x, v, t_end, eta, eps2, dt_max=0.125, max_steps=10**8):
+    """Integrate one system with the shared-step Hermite rules; returns
+    (x, v, t_final, steps, e0, e1, rc)."""
+    x = np.array(x, dtype=np.float64, order="C")
+    v = np.array(v, dtype=np.float64, order="C")
+    t = C.c_double(); ns = C.c_longlong(); e0 = C.c_double(); e1 = C.c_double()
+    rc = lib().or_hermite_shared(len(m), _d(m), x, v, t_end, eta, eps2, dt_max, max_steps,
+                                 C.byref(t), C.byref(ns), C.byref(e0), C.byref(e1))
+    return x, v, t.value, ns.value, e0.value, e1.value, rc
 
 
 def threads() -> int:

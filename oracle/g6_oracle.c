@@ -348,3 +348,128 @@ void or_energies(int n, const double *m, const double *x, const double *v,
     *epot = pot;
     *ekin = 0.5 * kin;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Shared-step Hermite integrator (checker of the ensemble engine).          */
+/* ------------------------------------------------------------------------ */
+
+static void hs_force(int n, const double *m, const double *x, const double *v, double eps2,
+                     double *a, double *j, double *pot)
+{
+    for (int i = 0; i < n; i++) {
+        double ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0, p = 0;
+        for (int k = 0; k < n; k++) {
+            if (k == i) continue;
+            double dx = x[3*k] - x[3*i], dy = x[3*k+1] - x[3*i+1], dz = x[3*k+2] - x[3*i+2];
+            double du = v[3*k] - v[3*i], dv = v[3*k+1] - v[3*i+1], dw = v[3*k+2] - v[3*i+2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            double r2inv = 1.0 / (d2 + eps2);
+            double a3 = -3.0 * (dx * du + dy * dv + dz * dw) * r2inv;
+            double mrinv = m[k] * sqrt(r2inv);
+            p -= mrinv;
+            double mr3inv = mrinv * r2inv;
+            ax += mr3inv * dx; ay += mr3inv * dy; az += mr3inv * dz;
+            jx += mr3inv * (du + a3 * dx);
+            jy += mr3inv * (dv + a3 * dy);
+            jz += mr3inv * (dw + a3 * dz);
+        }
+        a[3*i] = ax; a[3*i+1] = ay; a[3*i+2] = az;
+        j[3*i] = jx; j[3*i+1] = jy; j[3*i+2] = jz;
+        pot[i] = p;
+    }
+}
+
+static double hs_energy(int n, const double *m, const double *v, const double *pot)
+{
+    double kin = 0, pe = 0;
+    for (int i = 0; i < n; i++) {
+        kin += m[i] * (v[3*i] * v[3*i] + v[3*i+1] * v[3*i+1] + v[3*i+2] * v[3*i+2]);
+        pe += m[i] * pot[i];
+    }
+    return 0.5 * kin + 0.5 * pe;
+}
+
+static double hs_pow2_below(double x)
+{
+    int e;
+    frexp(x, &e);
+    return ldexp(1.0, e - 1);
+}
+
+int or_hermite_shared(int n, const double *m, double *x, double *v, double t_end,
+                      double eta, double eps2, double dt_max, long long max_steps,
+                      double *t_final, long long *steps, double *e0, double *e1)
+{
+    const double ONE3 = 1.0 / 3.0, ONE12 = 1.0 / 12.0;
+    double *a = calloc(3 * (size_t)n, sizeof(double)), *jk = calloc(3 * (size_t)n, sizeof(double));
+    double *a1 = calloc(3 * (size_t)n, sizeof(double)), *j1 = calloc(3 * (size_t)n, sizeof(double));
+    double *xp = calloc(3 * (size_t)n, sizeof(double)), *vp = calloc(3 * (size_t)n, sizeof(double));
+    double *pot = calloc((size_t)n, sizeof(double));
+    int rc = 0;
+    hs_force(n, m, x, v, eps2, a, jk, pot);
+    *e0 = hs_energy(n, m, v, pot);
+    /* first step (set_first_timestep), minimum over particles */
+    double nat = INFINITY;
+    for (int i = 0; i < n; i++) {
+        double eta_init = eta / 16;
+        double asq = a[3*i]*a[3*i] + a[3*i+1]*a[3*i+1] + a[3*i+2]*a[3*i+2];
+        double jsq = jk[3*i]*jk[3*i] + jk[3*i+1]*jk[3*i+1] + jk[3*i+2]*jk[3*i+2];
+        double dt_adot = 1e300, dt_ff = 1e300;
+        if (jsq > 0) dt_adot = asq / jsq;
+        if (pot[i] < 0 && asq > 0) dt_ff = -pot[i] / asq;
+        double c = eta_init * sqrt(fmin(dt_adot, dt_ff));
+        c = fmin(c, eta_init);
+        if (c < nat) nat = c;
+    }
+    double dt_cap = fmin(dt_max, t_end);
+    double t = 0, dt = hs_pow2_below(fmin(nat, dt_cap));
+    if (t + dt > t_end) dt = t_end - t;
+    long long ns = 0;
+    while (t < t_end) {
+        for (int q = 0; q < 3 * n; q++) {       /* predict_loworder */
+            double dt3 = dt * ONE3, dt2 = dt * 0.5;
+            xp[q] = ((jk[q] * dt3 + a[q]) * dt2 + v[q]) * dt + x[q];
+            vp[q] = (jk[q] * dt2 + a[q]) * dt + v[q];
+        }
+        hs_force(n, m, xp, vp, eps2, a1, j1, pot);
+        nat = INFINITY;
+        for (int i = 0; i < n; i++) {
+            double bt2[3], at3[3];
+            for (int k = 0; k < 3; k++) {
+                int q = 3 * i + k;
+                bt2[k] = -3 * (a[q] - a1[q]) - dt * (2 * jk[q] + j1[q]);
+                at3[k] = 2 * (a[q] - a1[q]) + dt * (jk[q] + j1[q]);
+            }
+            for (int k = 0; k < 3; k++) {
+                int q = 3 * i + k;
+                x[q] = xp[q] + (0.05 * at3[k] + ONE12 * bt2[k]) * dt * dt;
+                v[q] = vp[q] + (0.25 * at3[k] + ONE3 * bt2[k]) * dt;
+                a[q] = a1[q];
+                jk[q] = j1[q];
+            }
+            double dt2 = dt * dt, dt4 = dt2 * dt2, dt6 = dt4 * dt2;
+            double l2 = 36 * (at3[0]*at3[0] + at3[1]*at3[1] + at3[2]*at3[2]) / dt6;
+            double k2 = 4 * (bt2[0]*bt2[0] + bt2[1]*bt2[1] + bt2[2]*bt2[2]) / dt4;
+            double j2 = j1[3*i]*j1[3*i] + j1[3*i+1]*j1[3*i+1] + j1[3*i+2]*j1[3*i+2];
+            double asq = a1[3*i]*a1[3*i] + a1[3*i+1]*a1[3*i+1] + a1[3*i+2]*a1[3*i+2];
+            double c = eta * sqrt((sqrt(asq * k2) + j2) / (sqrt(j2 * l2) + k2));
+            if (!(c > 0)) c = INFINITY;
+            if (c < nat) nat = c;
+        }
+        t += dt;
+        ns++;
+        if (t >= t_end) break;
+        double T = dt, nd = T;
+        if (nat < T) nd = 0.5 * T;
+        else if (nat >= 2 * T && fmod(t, 2 * T) == 0.0 && 2 * T <= dt_cap) nd = 2 * T;
+        if (t + nd > t_end) nd = t_end - t;
+        dt = nd;
+        if (!(dt > 1e-30) || ns > max_steps) { rc = 1; break; }
+    }
+    hs_force(n, m, x, v, eps2, a1, j1, pot);
+    *e1 = hs_energy(n, m, v, pot);
+    *t_final = t;
+    *steps = ns;
+    free(a); free(jk); free(a1); free(j1); free(xp); free(vp); free(pot);
+    return rc;
+}

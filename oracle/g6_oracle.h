@@ -78,6 +78,17 @@ void or_neighbours(int ni, const double *xi, const double *h2, const int *excl,
 void or_energies(int n, const double *m, const double *x, const double *v,
                  double eps2, double *epot, double *ekin);
 
+/* --- shared-step Hermite integrator (ensemble engine checker) ------------ */
+/* One system from t = 0 to t_end with kira's rules: predictor _dyn_.h:290-295,
+ * FP64 pair law hdyn_ev.C:2019-2036 (softened), corrector hdyn_ev.C:1235-1250
+ * and :1717-1718, Aarseth step :627-655 reduced by min over particles,
+ * power-of-two step rule :866-887 (dt <= dt_max), first step :377-410.
+ * x, v are updated in place.  Returns 0, or 1 if the step underflowed or
+ * max_steps was exceeded.  e0/e1: total energy at start/end. */
+int or_hermite_shared(int n, const double *m, double *x, double *v, double t_end,
+                      double eta, double eps2, double dt_max, long long max_steps,
+                      double *t_final, long long *steps, double *e0, double *e1);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,7 +1,8 @@
 """ctypes prototypes of every entry point libg6.so exports.
 
 ``GRAPE6`` lists the entry points of include/grape6.h (the reference ABI,
-include/grape6.h:18-102 of the Starlab tree); ``G6X`` those of include/g6x.h.
+include/grape6.h:18-102 of the Starlab tree); ``G6X`` those of include/g6x.h;
+``G6E`` those of include/g6ens.h (the ensemble engine).
 """
 from __future__ import annotations
 
@@ -51,8 +52,22 @@ G6X = {
 }
 
 
+G6E = {
+    "g6e_open": (_i, [C.c_longlong, _i]),
+    "g6e_close": (_i, []),
+    "g6e_submit": (C.c_longlong, [_i, _vp, _vp, _vp, C.c_double, C.c_double, C.c_double]),
+    "g6e_run": (_i, [_i]),
+    "g6e_poll": (_i, [_i, _vp]),
+    "g6e_result": (_i, [C.c_longlong, _i, _vp, _vp, _pd, C.POINTER(C.c_longlong), _pd, _pd]),
+    "g6e_stats": (_i, [C.POINTER(C.c_longlong), _pd, C.POINTER(C.c_longlong), _pd]),
+    "g6e_set_option": (_i, [C.c_char_p, C.c_longlong]),
+    "g6e_get_option": (C.c_longlong, [C.c_char_p]),
+    "g6e_last_error": (C.c_char_p, []),
+}
+
+
 def declare(lib: C.CDLL) -> None:
-    for table in (GRAPE6, G6X):
+    for table in (GRAPE6, G6X, G6E):
         for name, (res, args) in table.items():
             f = getattr(lib, name)
             f.restype = res

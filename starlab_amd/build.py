@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libg6.so")
 
-SOURCES = ["g6_kernels.hip", "g6_engine.hip"]
+SOURCES = ["g6_kernels.hip", "g6_engine.hip", "g6_ens.hip"]
 ARCH = "gfx950"
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

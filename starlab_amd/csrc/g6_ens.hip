@@ -1,0 +1,1023 @@
+// g6_ens.hip -- batched ensemble engine (include/g6ens.h): many independent
+// N-body systems integrated together, one shared power-of-two step per
+// system, kira's Hermite scheme.
+//
+// Device state lives in one particle pool (capacity from the HBM budget);
+// each resident system owns a contiguous range [off, off + n) and a header
+// slot.  One engine step is four kernels, captured in a HIP graph:
+//   k_ens_predict   thread per particle: kira's predictor to t + dt
+//                   (include/_dyn_.h:290-295), double-single copy for pairs
+//   k_ens_force     one wave per (system, 64-i chunk, j-split): the libg6 pair
+//                   law (FP32 pairs on double-single positions, 256-j FP32
+//                   tiles folded into FP64) against the system's own j
+//   k_ens_correct   thread per particle: sum j-splits in order, Hermite
+//                   corrector (hdyn_ev.C:1235-1250, 1717-1718), Aarseth step
+//                   (hdyn_ev.C:627-655) -> atomic min into the system header
+//   k_ens_finalize  thread per slot: t += dt and kira's power-of-two step rule
+//                   (hdyn_ev.C:855-887); first step as set_first_timestep
+//                   (hdyn_ev.C:377-410)
+// Atomic min of a positive double through its bit pattern is order-free, and
+// every sum runs in a fixed order, so a system's trajectory does not depend
+// on what else is resident.
+#include "g6_internal.h"
+#include "../../include/g6ens.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace g6 {
+namespace ens {
+
+enum Phase : int { FREE = 0, INIT = 1, STEP = 2, FINAL = 3, DONE = 4, FAILED = 5 };
+
+struct SysHdr {
+    double t, dt, t_end, eta, eps2, dt_max;
+    unsigned long long pmin;      // bits of the smallest candidate step (positive double)
+    long long steps;
+    int off, n, phase, nsplit, jchunk, pad;
+};
+
+struct Pool {
+    int cap = 0;
+    double *x = nullptr, *v = nullptr, *a = nullptr, *j = nullptr;   // [3][cap]
+    double *xp = nullptr, *vp = nullptr;                             // [3][cap]
+    double *m = nullptr, *pot = nullptr, *pot0 = nullptr;            // [cap]
+    JPred *jp = nullptr;                                             // [cap]
+    double *part = nullptr;                                          // [max_split][7][cap]
+    int max_split = 0;
+};
+
+struct Chunk { int slot, i0; };
+struct FItem { int slot, i0, split, pad; };
+
+constexpr double kOneThird = 1.0 / 3.0;
+constexpr double kOne12 = 1.0 / 12.0;
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_ens_predict(Pool P, const SysHdr *__restrict__ H,
+                                                    const Chunk *__restrict__ chunks,
+                                                    const int *__restrict__ nchunks)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x;
+    if (c >= *nchunks) return;
+    const Chunk ch = chunks[c];
+    const SysHdr &h = H[ch.slot];
+    if (h.phase < INIT || h.phase > FINAL) return;
+    const int i = ch.i0 + threadIdx.x;
+    if (i >= h.n) return;
+    const size_t p = (size_t)h.off + i;
+    const size_t cap = P.cap;
+    const double dt = h.phase == STEP ? h.dt : 0.0;
+    const double dt3 = dt * kOneThird, dt2 = dt * 0.5;
+    double xp[3], vp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double x = P.x[k * cap + p], v = P.v[k * cap + p];
+        const double a = P.a[k * cap + p], jk = P.j[k * cap + p];
+        // predict_loworder, include/_dyn_.h:290-295
+        xp[k] = ((jk * dt3 + a) * dt2 + v) * dt + x;
+        vp[k] = (jk * dt2 + a) * dt + v;
+        P.xp[k * cap + p] = xp[k];
+        P.vp[k * cap + p] = vp[k];
+    }
+    JPred q;
+    q.xh = (float)xp[0]; q.xl = (float)(xp[0] - (double)q.xh);
+    q.yh = (float)xp[1]; q.yl = (float)(xp[1] - (double)q.yh);
+    q.zh = (float)xp[2]; q.zl = (float)(xp[2] - (double)q.zh);
+    q.vx = (float)vp[0]; q.vy = (float)vp[1]; q.vz = (float)vp[2];
+    q.m = (float)P.m[p];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q.pad[k] = 0.f;
+    P.jp[p] = q;
+}
+
+// ---------------------------------------------------------------------------
+struct EAcc {
+    float ax, ay, az, jx, jy, jz, kx, ky, kz, p;
+};
+
+template <bool MASK>
+__device__ __forceinline__ void ens_pair(const JPred &J, int jj, unsigned rel, const JPred &I,
+                                         float eps2, EAcc &a)
+{
+#pragma clang fp contract(off)
+    const float dx = (J.xh - I.xh) + (J.xl - I.xl);
+    const float dy = (J.yh - I.yh) + (J.yl - I.yl);
+    const float dz = (J.zh - I.zh) + (J.zl - I.zl);
+    float r2 = dz * dz;
+    r2 = __builtin_fmaf(dy, dy, r2);
+    r2 = __builtin_fmaf(dx, dx, r2);
+    r2 = r2 + eps2;
+    const float dvx = J.vx - I.vx, dvy = J.vy - I.vy, dvz = J.vz - I.vz;
+    float rv = dz * dvz;
+    rv = __builtin_fmaf(dy, dvy, rv);
+    rv = __builtin_fmaf(dx, dvx, rv);
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    if (MASK) rinv = ((unsigned)jj == rel) ? 0.0f : rinv;
+    const float rinv2 = rinv * rinv;
+    const float mr = J.m * rinv;
+    const float mr3 = mr * rinv2;
+    const float q = mr3 * (rv * rinv2);
+    a.p = a.p - mr;
+    a.ax = __builtin_fmaf(mr3, dx, a.ax);
+    a.ay = __builtin_fmaf(mr3, dy, a.ay);
+    a.az = __builtin_fmaf(mr3, dz, a.az);
+    a.jx = __builtin_fmaf(mr3, dvx, a.jx);
+    a.jy = __builtin_fmaf(mr3, dvy, a.jy);
+    a.jz = __builtin_fmaf(mr3, dvz, a.jz);
+    a.kx = __builtin_fmaf(q, dx, a.kx);
+    a.ky = __builtin_fmaf(q, dy, a.ky);
+    a.kz = __builtin_fmaf(q, dz, a.kz);
+}
+
+template <bool MASK>
+__device__ __forceinline__ void ens_tile(const JPred *__restrict__ jp, int tn, unsigned rel,
+                                         const JPred &I, float eps2, EAcc &a)
+{
+    int jj = 0;
+    for (; jj + 4 <= tn; jj += 4) {
+        const JPred *g = jp + jj;
+        ens_pair<MASK>(g[0], jj + 0, rel, I, eps2, a);
+        ens_pair<MASK>(g[1], jj + 1, rel, I, eps2, a);
+        ens_pair<MASK>(g[2], jj + 2, rel, I, eps2, a);
+        ens_pair<MASK>(g[3], jj + 3, rel, I, eps2, a);
+    }
+    for (; jj < tn; ++jj) ens_pair<MASK>(jp[jj], jj, rel, I, eps2, a);
+}
+
+__global__ __launch_bounds__(64) void k_ens_force(Pool P, const SysHdr *__restrict__ H,
+                                                  const FItem *__restrict__ items,
+                                                  const int *__restrict__ nitems)
+{
+    const int w = blockIdx.x;
+    if (w >= *nitems) return;
+    const FItem it = items[w];
+    const SysHdr &h = H[it.slot];
+    if (h.phase < INIT || h.phase > FINAL) return;
+    const int n = h.n;
+    const int lane = threadIdx.x;
+    const int i = it.i0 + lane;
+    const int ic = i < n ? i : n - 1;
+    const JPred *__restrict__ base = P.jp + h.off;
+    const JPred I = base[ic];
+    const int self = i < n ? i : -1;
+    const float eps2 = (float)h.eps2;
+    const int jbeg = it.split * h.jchunk;
+    const int jend = min(jbeg + h.jchunk, n);
+    double A[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int t = jbeg; t < jend; t += kTile) {
+        const int tn = min(kTile, jend - t);
+        EAcc a;
+        a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.kx = a.ky = a.kz = a.p = 0.f;
+        const unsigned rel = (unsigned)(self - t);
+        if (__ballot(rel < (unsigned)tn) == 0)
+            ens_tile<false>(base + t, tn, rel, I, eps2, a);
+        else
+            ens_tile<true>(base + t, tn, rel, I, eps2, a);
+        A[0] += (double)a.ax;
+        A[1] += (double)a.ay;
+        A[2] += (double)a.az;
+        A[3] += (double)a.jx - 3.0 * (double)a.kx;
+        A[4] += (double)a.jy - 3.0 * (double)a.ky;
+        A[5] += (double)a.jz - 3.0 * (double)a.kz;
+        A[6] += (double)a.p;
+    }
+    if (i < n) {
+        const size_t cap = P.cap;
+        const size_t p = (size_t)h.off + i;
+        double *out = P.part + (size_t)it.split * 7 * cap;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) out[c * cap + p] = A[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dot3(const double *u, const double *v)
+{
+    return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+
+__global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__ H,
+                                                    const Chunk *__restrict__ chunks,
+                                                    const int *__restrict__ nchunks)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x;
+    if (c >= *nchunks) return;
+    const Chunk ch = chunks[c];
+    SysHdr &h = H[ch.slot];
+    const int phase = h.phase;
+    if (phase < INIT || phase > FINAL) return;
+    const int i = ch.i0 + threadIdx.x;
+    double cand = __builtin_inf();
+    if (i < h.n) {
+        const size_t cap = P.cap;
+        const size_t p = (size_t)h.off + i;
+        double a1[3] = {0, 0, 0}, j1[3] = {0, 0, 0}, pot = 0;
+        for (int s = 0; s < h.nsplit; ++s) {
+            const double *in = P.part + (size_t)s * 7 * cap;
+            a1[0] += in[0 * cap + p]; a1[1] += in[1 * cap + p]; a1[2] += in[2 * cap + p];
+            j1[0] += in[3 * cap + p]; j1[1] += in[4 * cap + p]; j1[2] += in[5 * cap + p];
+            pot += in[6 * cap + p];
+        }
+        if (phase == INIT) {
+            // set_first_timestep, hdyn_ev.C:377-410: eta/16 * sqrt(min(a^2/j^2, -pot/a^2))
+            for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = a1[k]; P.j[k * cap + p] = j1[k]; }
+            P.pot[p] = pot;
+            P.pot0[p] = pot;
+            const double eta_init = h.eta / 16;
+            const double asq = dot3(a1, a1), jsq = dot3(j1, j1);
+            double dt_adot = 1e300, dt_ff = 1e300;
+            if (jsq > 0) dt_adot = asq / jsq;
+            if (pot < 0 && asq > 0) dt_ff = -pot / asq;
+            cand = eta_init * sqrt(fmin(dt_adot, dt_ff));
+            cand = fmin(cand, eta_init);
+        } else if (phase == STEP) {
+            const double dt = h.dt;
+            double bt2[3], at3[3], a0[3], j0[3];
+            for (int k = 0; k < 3; ++k) {
+                a0[k] = P.a[k * cap + p];
+                j0[k] = P.j[k * cap + p];
+                // get_derivs, hdyn_ev.C:1235-1250
+                bt2[k] = -3 * (a0[k] - a1[k]) - dt * (2 * j0[k] + j1[k]);
+                at3[k] = 2 * (a0[k] - a1[k]) + dt * (j0[k] + j1[k]);
+            }
+            for (int k = 0; k < 3; ++k) {
+                // hdyn_ev.C:1717-1718
+                P.x[k * cap + p] = P.xp[k * cap + p] + (0.05 * at3[k] + kOne12 * bt2[k]) * dt * dt;
+                P.v[k * cap + p] = P.vp[k * cap + p] + (0.25 * at3[k] + kOneThird * bt2[k]) * dt;
+                P.a[k * cap + p] = a1[k];
+                P.j[k * cap + p] = j1[k];
+            }
+            P.pot[p] = pot;
+            // Aarseth criterion, hdyn_ev.C:627-655
+            const double dt2 = dt * dt, dt4 = dt2 * dt2, dt6 = dt4 * dt2;
+            const double l2 = 36 * dot3(at3, at3) / dt6;
+            const double k2 = 4 * dot3(bt2, bt2) / dt4;
+            const double j2 = dot3(j1, j1);
+            const double asq = dot3(a1, a1);
+            cand = h.eta * sqrt((sqrt(asq * k2) + j2) / (sqrt(j2 * l2) + k2));
+            if (!(cand > 0)) cand = __builtin_inf();
+        } else {   // FINAL: potential at the exact final positions
+            P.pot[p] = pot;
+        }
+    }
+    // wave minimum, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) cand = fmin(cand, __shfl_xor(cand, off));
+    if (threadIdx.x == 0 && phase != FINAL && cand < __builtin_inf())
+        atomicMin(&h.pmin, (unsigned long long)__double_as_longlong(cand));
+}
+
+__device__ __forceinline__ double pow2_below(double x)
+{
+    int e;
+    const double f = frexp(x, &e);   // x = f * 2^e, f in [0.5, 1)
+    (void)f;
+    return ldexp(1.0, e - 1);
+}
+
+__global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restrict__ phase_out,
+                               long long max_steps, double dt_min)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    SysHdr &h = H[s];
+    const double nat = __longlong_as_double((long long)h.pmin);
+    if (h.phase == INIT) {
+        double dt = fmin(nat, h.dt_max);
+        dt = pow2_below(dt);
+        while (h.t != 0 && fmod(h.t, dt) != 0) dt *= 0.5;
+        h.dt = dt;
+        h.phase = (dt > dt_min) ? STEP : FAILED;
+        if (h.t + h.dt > h.t_end) h.dt = h.t_end - h.t;
+    } else if (h.phase == STEP) {
+        h.t += h.dt;
+        h.steps += 1;
+        if (h.t >= h.t_end) {
+            h.phase = FINAL;
+        } else {
+            // hdyn_ev.C:866-887: halve at most once; double only when the
+            // time is commensurate with the doubled step
+            const double T = h.dt;
+            double nd = T;
+            if (nat < T) nd = 0.5 * T;
+            else if (nat >= 2 * T && fmod(h.t, 2 * T) == 0.0 && 2 * T <= h.dt_max) nd = 2 * T;
+            if (h.t + nd > h.t_end) nd = h.t_end - h.t;
+            h.dt = nd;
+            if (!(nd > dt_min) || h.steps > max_steps) h.phase = FAILED;
+        }
+    } else if (h.phase == FINAL) {
+        h.phase = DONE;
+    }
+    h.pmin = 0x7FF0000000000000ull;   // +inf
+    phase_out[s] = h.phase;
+}
+
+// ---------------------------------------------------------------------------
+// admission and retirement
+// staged particle record: m, x[3], v[3], dest index (as double)
+__global__ void k_ens_unpack(Pool P, const double *__restrict__ rec, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double *r = rec + 8 * (size_t)k;
+    const size_t p = (size_t)r[7];
+    const size_t cap = P.cap;
+    P.m[p] = r[0];
+    for (int c = 0; c < 3; ++c) {
+        P.x[c * cap + p] = r[1 + c];
+        P.v[c * cap + p] = r[4 + c];
+        P.a[c * cap + p] = 0.0;
+        P.j[c * cap + p] = 0.0;
+    }
+    P.pot[p] = 0.0;
+    P.pot0[p] = 0.0;
+}
+
+__global__ void k_ens_set_headers(SysHdr *__restrict__ H, const SysHdr *__restrict__ staged,
+                                  const int *__restrict__ slots, int n, int *__restrict__ phase_out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    H[slots[k]] = staged[k];
+    phase_out[slots[k]] = staged[k].phase;
+}
+
+// output record per particle: x[3], v[3], pot, pot0 (8 doubles)
+__global__ void k_ens_pack(Pool P, const int *__restrict__ src, double *__restrict__ out, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t p = src[k];
+    const size_t cap = P.cap;
+    double *o = out + 8 * (size_t)k;
+    for (int c = 0; c < 3; ++c) {
+        o[c] = P.x[c * cap + p];
+        o[3 + c] = P.v[c * cap + p];
+    }
+    o[6] = P.pot[p];
+    o[7] = P.pot0[p];
+}
+
+// ===========================================================================
+// host engine
+// ===========================================================================
+namespace {
+
+struct Job {
+    long long id;
+    int n;
+    std::vector<double> m, x, v;
+    double t_end, eta, eps2;
+};
+
+struct SlotInfo {
+    bool busy = false;
+    long long job = -1;
+    int off = 0, n = 0, nsplit = 1;
+    std::vector<double> m, x0, v0;
+    double eps2 = 0, t_end = 0;
+};
+
+struct Result {
+    int n;
+    std::vector<double> x, v;
+    double t, e0, e1;
+    long long steps;
+    bool failed;
+};
+
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    int chunk_grid = 0, item_grid = 0, steps = 0;
+};
+
+struct Engine {
+    bool open = false;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Pool P;
+    int max_slots = 0;
+    SysHdr *d_hdr = nullptr;
+    int *d_phase = nullptr, *h_phase = nullptr;
+    Chunk *d_chunks = nullptr;
+    FItem *d_items = nullptr;
+    int *d_counts = nullptr;          // [0] chunks, [1] items
+    int *h_counts = nullptr;
+    Chunk *h_chunks = nullptr;
+    FItem *h_items = nullptr;
+    int chunk_cap = 0, item_cap = 0;
+    int n_chunks = 0, n_items = 0;
+    bool items_dirty = true;
+    // staging
+    double *h_stage = nullptr, *d_stage = nullptr;
+    size_t stage_cap = 0;             // doubles
+    SysHdr *h_hstage = nullptr, *d_hstage = nullptr;
+    int *h_hslots = nullptr, *d_hslots = nullptr;
+    int *h_src = nullptr, *d_src = nullptr;
+    double *h_out = nullptr, *d_out = nullptr;
+    size_t out_cap = 0;               // particles
+    std::vector<SlotInfo> slots;
+    std::map<int, int> free_ranges;   // off -> len
+    std::deque<Job> queue;
+    long long next_id = 0;
+    std::unordered_map<long long, Result> results;
+    std::vector<long long> finished;
+    std::vector<GraphExec> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // options
+    int steps_per_graph = 8;
+    int jsplit_target = 512;
+    int max_split = 16;
+    int lookahead = 64;
+    double dt_max = 0.125;
+    long long max_steps = 50000000;
+    // stats
+    long long engine_steps = 0, n_finished = 0;
+    double pairs = 0, dev_seconds = 0;
+};
+
+Engine X;
+
+int efail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    X.err = buf;
+    fprintf(stderr, "libg6 ensemble: %s\n", buf);
+    return code;
+}
+
+#define ECHK(call)                                                             \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return efail(-1, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+template <class T> void dfree(T *&p) { if (p) { (void)hipFree(p); p = nullptr; } }
+template <class T> void hfree(T *&p) { if (p) { (void)hipHostFree(p); p = nullptr; } }
+
+size_t bytes_per_particle(int max_split)
+{
+    return sizeof(double) * (3 * 6 + 3) + sizeof(JPred) + sizeof(double) * 7 * (size_t)max_split;
+}
+
+int nsplit_for(int n)
+{
+    if (X.jsplit_target <= 0) return 1;
+    int s = (n + X.jsplit_target - 1) / X.jsplit_target;
+    return std::max(1, std::min(s, X.max_split));
+}
+
+int ensure_stage(size_t doubles)
+{
+    if (doubles <= X.stage_cap) return 0;
+    size_t cap = std::max(doubles, X.stage_cap * 2);
+    hfree(X.h_stage); dfree(X.d_stage);
+    X.stage_cap = 0;
+    ECHK(hipHostMalloc((void **)&X.h_stage, sizeof(double) * cap, hipHostMallocDefault));
+    ECHK(hipMalloc((void **)&X.d_stage, sizeof(double) * cap));
+    X.stage_cap = cap;
+    return 0;
+}
+
+int ensure_out(size_t particles)
+{
+    if (particles <= X.out_cap) return 0;
+    size_t cap = std::max(particles, X.out_cap * 2);
+    hfree(X.h_out); dfree(X.d_out); hfree(X.h_src); dfree(X.d_src);
+    X.out_cap = 0;
+    ECHK(hipHostMalloc((void **)&X.h_out, sizeof(double) * 8 * cap, hipHostMallocDefault));
+    ECHK(hipMalloc((void **)&X.d_out, sizeof(double) * 8 * cap));
+    ECHK(hipHostMalloc((void **)&X.h_src, sizeof(int) * cap, hipHostMallocDefault));
+    ECHK(hipMalloc((void **)&X.d_src, sizeof(int) * cap));
+    X.out_cap = cap;
+    return 0;
+}
+
+int ensure_items(int chunks, int items)
+{
+    if (chunks > X.chunk_cap) {
+        int cap = std::max(chunks, X.chunk_cap * 2);
+        hfree(X.h_chunks); dfree(X.d_chunks);
+        ECHK(hipHostMalloc((void **)&X.h_chunks, sizeof(Chunk) * cap, hipHostMallocDefault));
+        ECHK(hipMalloc((void **)&X.d_chunks, sizeof(Chunk) * cap));
+        X.chunk_cap = cap;
+        for (auto &g : X.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        X.graphs.clear();
+    }
+    if (items > X.item_cap) {
+        int cap = std::max(items, X.item_cap * 2);
+        hfree(X.h_items); dfree(X.d_items);
+        ECHK(hipHostMalloc((void **)&X.h_items, sizeof(FItem) * cap, hipHostMallocDefault));
+        ECHK(hipMalloc((void **)&X.d_items, sizeof(FItem) * cap));
+        X.item_cap = cap;
+        for (auto &g : X.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        X.graphs.clear();
+    }
+    return 0;
+}
+
+int pool_alloc(int n)
+{
+    for (auto it = X.free_ranges.begin(); it != X.free_ranges.end(); ++it) {
+        if (it->second >= n) {
+            const int off = it->first, len = it->second;
+            X.free_ranges.erase(it);
+            if (len > n) X.free_ranges[off + n] = len - n;
+            return off;
+        }
+    }
+    return -1;
+}
+
+void pool_free(int off, int n)
+{
+    auto it = X.free_ranges.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != X.free_ranges.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        X.free_ranges.erase(nx);
+    }
+    if (it != X.free_ranges.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            X.free_ranges.erase(it);
+        }
+    }
+}
+
+// Retire DONE / FAILED slots: gather their state on the device, copy once.
+int retire()
+{
+    std::vector<int> done;
+    for (int s = 0; s < X.max_slots; ++s)
+        if (X.slots[s].busy && (X.h_phase[s] == DONE || X.h_phase[s] == FAILED)) done.push_back(s);
+    if (done.empty()) return 0;
+    size_t total = 0;
+    for (int s : done) total += X.slots[s].n;
+    if (ensure_out(total)) return -1;
+    size_t k = 0;
+    for (int s : done)
+        for (int i = 0; i < X.slots[s].n; ++i) X.h_src[k++] = X.slots[s].off + i;
+    ECHK(hipMemcpyAsync(X.d_src, X.h_src, sizeof(int) * total, hipMemcpyHostToDevice, X.stream));
+    hipLaunchKernelGGL(k_ens_pack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, X.stream,
+                       X.P, X.d_src, X.d_out, (int)total);
+    ECHK(hipGetLastError());
+    ECHK(hipMemcpyAsync(X.h_out, X.d_out, sizeof(double) * 8 * total, hipMemcpyDeviceToHost,
+                        X.stream));
+    // final headers of the retired slots
+    std::vector<SysHdr> hdr(done.size());
+    for (size_t q = 0; q < done.size(); ++q)
+        ECHK(hipMemcpyAsync(&hdr[q], X.d_hdr + done[q], sizeof(SysHdr), hipMemcpyDeviceToHost,
+                            X.stream));
+    ECHK(hipStreamSynchronize(X.stream));
+    k = 0;
+    for (size_t q = 0; q < done.size(); ++q) {
+        SlotInfo &si = X.slots[done[q]];
+        Result r;
+        r.n = si.n;
+        r.x.resize(3 * (size_t)si.n);
+        r.v.resize(3 * (size_t)si.n);
+        std::vector<double> pot(si.n), pot0(si.n);
+        for (int i = 0; i < si.n; ++i) {
+            const double *o = X.h_out + 8 * (k + i);
+            for (int c = 0; c < 3; ++c) { r.x[3 * i + c] = o[c]; r.v[3 * i + c] = o[3 + c]; }
+            pot[i] = o[6];
+            pot0[i] = o[7];
+        }
+        k += si.n;
+        r.t = hdr[q].t;
+        r.steps = hdr[q].steps;
+        r.failed = X.h_phase[done[q]] == FAILED;
+        // total energy, kinetic + potential (softened as integrated)
+        {
+            double kin = 0, pe = 0, kin0 = 0, pe0 = 0;
+            for (int i = 0; i < si.n; ++i) {
+                const double *vv = &r.v[3 * i];
+                kin += si.m[i] * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+                pe += si.m[i] * pot[i];
+                const double *v0 = &si.v0[3 * i];
+                kin0 += si.m[i] * (v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
+                pe0 += si.m[i] * pot0[i];
+            }
+            r.e1 = 0.5 * kin + 0.5 * pe;
+            r.e0 = 0.5 * kin0 + 0.5 * pe0;
+        }
+        X.pairs += (double)(r.steps + 2) * si.n * (double)(si.n - 1);
+        X.results[si.job] = std::move(r);
+        X.finished.push_back(si.job);
+        X.n_finished++;
+        pool_free(si.off, si.n);
+        si = SlotInfo();
+        X.h_phase[done[q]] = FREE;
+    }
+    // mark retired slots FREE on the device
+    std::vector<SysHdr> zero(done.size());
+    for (auto &z : zero) { memset(&z, 0, sizeof z); z.phase = FREE; z.pmin = 0x7FF0000000000000ull; }
+    for (size_t q = 0; q < done.size(); ++q) X.h_hslots[q] = done[q];
+    memcpy(X.h_hstage, zero.data(), sizeof(SysHdr) * done.size());
+    ECHK(hipMemcpyAsync(X.d_hstage, X.h_hstage, sizeof(SysHdr) * done.size(), hipMemcpyHostToDevice,
+                        X.stream));
+    ECHK(hipMemcpyAsync(X.d_hslots, X.h_hslots, sizeof(int) * done.size(), hipMemcpyHostToDevice,
+                        X.stream));
+    hipLaunchKernelGGL(k_ens_set_headers, dim3((unsigned)((done.size() + 255) / 256)), dim3(256),
+                       0, X.stream, X.d_hdr, X.d_hstage, X.d_hslots, (int)done.size(), X.d_phase);
+    ECHK(hipGetLastError());
+    ECHK(hipStreamSynchronize(X.stream));
+    X.items_dirty = true;
+    return 0;
+}
+
+// Admit queued jobs into free slots while the pool has room.
+int admit()
+{
+    if (X.queue.empty()) return 0;
+    std::vector<int> new_slots;
+    std::vector<SysHdr> new_hdr;
+    size_t staged = 0;
+    std::vector<std::pair<int, Job>> taken;
+    int examined = 0;
+    for (auto it = X.queue.begin(); it != X.queue.end() && examined < X.lookahead;) {
+        ++examined;
+        int slot = -1;
+        for (int s = 0; s < X.max_slots; ++s)
+            if (!X.slots[s].busy) { slot = s; break; }
+        if (slot < 0) break;
+        const int off = pool_alloc(it->n);
+        if (off < 0) { ++it; continue; }
+        SlotInfo &si = X.slots[slot];
+        si.busy = true;
+        si.job = it->id;
+        si.off = off;
+        si.n = it->n;
+        si.nsplit = nsplit_for(it->n);
+        si.m = it->m;
+        si.x0 = it->x;
+        si.v0 = it->v;
+        si.eps2 = it->eps2;
+        si.t_end = it->t_end;
+        SysHdr h;
+        memset(&h, 0, sizeof h);
+        h.t = 0; h.dt = 0; h.t_end = it->t_end; h.eta = it->eta; h.eps2 = it->eps2;
+        h.dt_max = std::min(X.dt_max, it->t_end);
+        h.pmin = 0x7FF0000000000000ull;
+        h.steps = 0;
+        h.off = off; h.n = it->n; h.phase = INIT; h.nsplit = si.nsplit;
+        h.jchunk = ((it->n + si.nsplit - 1) / si.nsplit + kTile - 1) / kTile * kTile;
+        if (si.nsplit > 1) {
+            // split boundaries on whole tiles; recompute the split count
+            h.nsplit = (it->n + h.jchunk - 1) / h.jchunk;
+            si.nsplit = h.nsplit;
+        }
+        new_slots.push_back(slot);
+        new_hdr.push_back(h);
+        staged += it->n;
+        taken.emplace_back(slot, std::move(*it));
+        it = X.queue.erase(it);
+    }
+    if (new_slots.empty()) return 0;
+    if (ensure_stage(8 * staged)) return -1;
+    size_t k = 0;
+    for (auto &tj : taken) {
+        const Job &j = tj.second;
+        const int off = X.slots[tj.first].off;
+        for (int i = 0; i < j.n; ++i, ++k) {
+            double *r = X.h_stage + 8 * k;
+            r[0] = j.m[i];
+            for (int c = 0; c < 3; ++c) { r[1 + c] = j.x[3 * i + c]; r[4 + c] = j.v[3 * i + c]; }
+            r[7] = (double)(off + i);
+        }
+    }
+    ECHK(hipMemcpyAsync(X.d_stage, X.h_stage, sizeof(double) * 8 * staged, hipMemcpyHostToDevice,
+                        X.stream));
+    hipLaunchKernelGGL(k_ens_unpack, dim3((unsigned)((staged + 255) / 256)), dim3(256), 0, X.stream,
+                       X.P, X.d_stage, (int)staged);
+    ECHK(hipGetLastError());
+    for (size_t q = 0; q < new_slots.size(); ++q) {
+        X.h_hslots[q] = new_slots[q];
+        X.h_hstage[q] = new_hdr[q];
+    }
+    ECHK(hipMemcpyAsync(X.d_hstage, X.h_hstage, sizeof(SysHdr) * new_slots.size(),
+                        hipMemcpyHostToDevice, X.stream));
+    ECHK(hipMemcpyAsync(X.d_hslots, X.h_hslots, sizeof(int) * new_slots.size(),
+                        hipMemcpyHostToDevice, X.stream));
+    hipLaunchKernelGGL(k_ens_set_headers, dim3((unsigned)((new_slots.size() + 255) / 256)),
+                       dim3(256), 0, X.stream, X.d_hdr, X.d_hstage, X.d_hslots,
+                       (int)new_slots.size(), X.d_phase);
+    ECHK(hipGetLastError());
+    for (size_t q = 0; q < new_slots.size(); ++q) X.h_phase[new_slots[q]] = INIT;
+    ECHK(hipStreamSynchronize(X.stream));
+    X.items_dirty = true;
+    return 0;
+}
+
+int rebuild_items()
+{
+    int nc = 0, ni = 0;
+    for (int s = 0; s < X.max_slots; ++s)
+        if (X.slots[s].busy) {
+            const int ch = (X.slots[s].n + 63) / 64;
+            nc += ch;
+            ni += ch * X.slots[s].nsplit;
+        }
+    if (ensure_items(std::max(nc, 1), std::max(ni, 1))) return -1;
+    int c = 0, w = 0;
+    for (int s = 0; s < X.max_slots; ++s) {
+        const SlotInfo &si = X.slots[s];
+        if (!si.busy) continue;
+        for (int i0 = 0; i0 < si.n; i0 += 64) {
+            X.h_chunks[c++] = Chunk{s, i0};
+            for (int sp = 0; sp < si.nsplit; ++sp) X.h_items[w++] = FItem{s, i0, sp, 0};
+        }
+    }
+    X.n_chunks = nc;
+    X.n_items = ni;
+    X.h_counts[0] = nc;
+    X.h_counts[1] = ni;
+    if (nc) ECHK(hipMemcpyAsync(X.d_chunks, X.h_chunks, sizeof(Chunk) * nc, hipMemcpyHostToDevice,
+                                X.stream));
+    if (ni) ECHK(hipMemcpyAsync(X.d_items, X.h_items, sizeof(FItem) * ni, hipMemcpyHostToDevice,
+                                X.stream));
+    ECHK(hipMemcpyAsync(X.d_counts, X.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, X.stream));
+    ECHK(hipStreamSynchronize(X.stream));
+    X.items_dirty = false;
+    return 0;
+}
+
+int bucket(int n)
+{
+    int b = 64;
+    while (b < n) b *= 2;
+    return b;
+}
+
+void enqueue_step(int cgrid, int igrid)
+{
+    hipLaunchKernelGGL(k_ens_predict, dim3(cgrid), dim3(64), 0, X.stream, X.P, X.d_hdr, X.d_chunks,
+                       X.d_counts);
+    hipLaunchKernelGGL(k_ens_force, dim3(igrid), dim3(64), 0, X.stream, X.P, X.d_hdr, X.d_items,
+                       X.d_counts + 1);
+    hipLaunchKernelGGL(k_ens_correct, dim3(cgrid), dim3(64), 0, X.stream, X.P, X.d_hdr, X.d_chunks,
+                       X.d_counts);
+    hipLaunchKernelGGL(k_ens_finalize, dim3((X.max_slots + 255) / 256), dim3(256), 0, X.stream,
+                       X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30);
+}
+
+// A graph of steps_per_graph engine steps for the current grid buckets.
+int get_graph(hipGraphExec_t &exec)
+{
+    const int cg = bucket(X.n_chunks), ig = bucket(X.n_items);
+    for (auto &g : X.graphs)
+        if (g.chunk_grid == cg && g.item_grid == ig && g.steps == X.steps_per_graph) {
+            exec = g.exec;
+            return 0;
+        }
+    hipGraph_t graph;
+    ECHK(hipStreamBeginCapture(X.stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < X.steps_per_graph; ++s) enqueue_step(cg, ig);
+    ECHK(hipStreamEndCapture(X.stream, &graph));
+    GraphExec ge;
+    ge.chunk_grid = cg;
+    ge.item_grid = ig;
+    ge.steps = X.steps_per_graph;
+    ECHK(hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    X.graphs.push_back(ge);
+    exec = ge.exec;
+    return 0;
+}
+
+int busy_count()
+{
+    int b = 0;
+    for (auto &s : X.slots) b += s.busy;
+    return b;
+}
+
+}  // namespace
+}  // namespace ens
+}  // namespace g6
+
+using namespace g6;
+using namespace g6::ens;
+
+extern "C" {
+
+int g6e_open(long long hbm_budget_bytes, int max_systems)
+{
+    if (X.open) return efail(-1, "g6e_open: already open");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return efail(-1, "g6e_open: no HIP device available");
+    if (max_systems < 1 || hbm_budget_bytes <= 0)
+        return efail(-1, "g6e_open: bad budget %lld / systems %d", hbm_budget_bytes, max_systems);
+    ECHK(hipStreamCreate(&X.stream));
+    ECHK(hipEventCreate(&X.ev0));
+    ECHK(hipEventCreate(&X.ev1));
+    X.P.max_split = X.max_split;
+    const size_t bpp = bytes_per_particle(X.max_split);
+    const long long cap = std::max(64LL, hbm_budget_bytes / (long long)bpp);
+    if (cap > (1LL << 30)) return efail(-1, "g6e_open: budget too large");
+    X.P.cap = (int)cap;
+    const size_t c = (size_t)cap;
+    ECHK(hipMalloc((void **)&X.P.x, sizeof(double) * 3 * c));
+    ECHK(hipMalloc((void **)&X.P.v, sizeof(double) * 3 * c));
+    ECHK(hipMalloc((void **)&X.P.a, sizeof(double) * 3 * c));
+    ECHK(hipMalloc((void **)&X.P.j, sizeof(double) * 3 * c));
+    ECHK(hipMalloc((void **)&X.P.xp, sizeof(double) * 3 * c));
+    ECHK(hipMalloc((void **)&X.P.vp, sizeof(double) * 3 * c));
+    ECHK(hipMalloc((void **)&X.P.m, sizeof(double) * c));
+    ECHK(hipMalloc((void **)&X.P.pot, sizeof(double) * c));
+    ECHK(hipMalloc((void **)&X.P.pot0, sizeof(double) * c));
+    ECHK(hipMalloc((void **)&X.P.jp, sizeof(JPred) * c));
+    ECHK(hipMalloc((void **)&X.P.part, sizeof(double) * 7 * c * (size_t)X.max_split));
+    X.max_slots = max_systems;
+    ECHK(hipMalloc((void **)&X.d_hdr, sizeof(SysHdr) * max_systems));
+    ECHK(hipMalloc((void **)&X.d_phase, sizeof(int) * max_systems));
+    ECHK(hipHostMalloc((void **)&X.h_phase, sizeof(int) * max_systems, hipHostMallocDefault));
+    ECHK(hipHostMalloc((void **)&X.h_hstage, sizeof(SysHdr) * max_systems, hipHostMallocDefault));
+    ECHK(hipMalloc((void **)&X.d_hstage, sizeof(SysHdr) * max_systems));
+    ECHK(hipHostMalloc((void **)&X.h_hslots, sizeof(int) * max_systems, hipHostMallocDefault));
+    ECHK(hipMalloc((void **)&X.d_hslots, sizeof(int) * max_systems));
+    ECHK(hipMalloc((void **)&X.d_counts, sizeof(int) * 2));
+    ECHK(hipHostMalloc((void **)&X.h_counts, sizeof(int) * 2, hipHostMallocDefault));
+    // all slots FREE
+    for (int s = 0; s < max_systems; ++s) {
+        memset(&X.h_hstage[s], 0, sizeof(SysHdr));
+        X.h_hstage[s].phase = FREE;
+        X.h_hstage[s].pmin = 0x7FF0000000000000ull;
+        X.h_hslots[s] = s;
+        X.h_phase[s] = FREE;
+    }
+    ECHK(hipMemcpy(X.d_hdr, X.h_hstage, sizeof(SysHdr) * max_systems, hipMemcpyHostToDevice));
+    ECHK(hipMemcpy(X.d_phase, X.h_phase, sizeof(int) * max_systems, hipMemcpyHostToDevice));
+    X.slots.assign(max_systems, SlotInfo());
+    X.free_ranges.clear();
+    X.free_ranges[0] = (int)cap;
+    X.queue.clear();
+    X.results.clear();
+    X.finished.clear();
+    X.items_dirty = true;
+    X.engine_steps = 0;
+    X.n_finished = 0;
+    X.pairs = 0;
+    X.dev_seconds = 0;
+    X.open = true;
+    if (rebuild_items()) return -1;
+    return 0;
+}
+
+int g6e_close(void)
+{
+    if (!X.open) return 0;
+    (void)hipStreamSynchronize(X.stream);
+    for (auto &g : X.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    X.graphs.clear();
+    dfree(X.P.x); dfree(X.P.v); dfree(X.P.a); dfree(X.P.j); dfree(X.P.xp); dfree(X.P.vp);
+    dfree(X.P.m); dfree(X.P.pot); dfree(X.P.pot0); dfree(X.P.jp); dfree(X.P.part);
+    dfree(X.d_hdr); dfree(X.d_phase); hfree(X.h_phase);
+    hfree(X.h_hstage); dfree(X.d_hstage); hfree(X.h_hslots); dfree(X.d_hslots);
+    dfree(X.d_counts); hfree(X.h_counts);
+    hfree(X.h_chunks); dfree(X.d_chunks); hfree(X.h_items); dfree(X.d_items);
+    X.chunk_cap = X.item_cap = 0;
+    hfree(X.h_stage); dfree(X.d_stage); X.stage_cap = 0;
+    hfree(X.h_out); dfree(X.d_out); hfree(X.h_src); dfree(X.d_src); X.out_cap = 0;
+    if (X.ev0) (void)hipEventDestroy(X.ev0);
+    if (X.ev1) (void)hipEventDestroy(X.ev1);
+    X.ev0 = X.ev1 = nullptr;
+    (void)hipStreamDestroy(X.stream);
+    X.stream = nullptr;
+    X.slots.clear();
+    X.open = false;
+    return 0;
+}
+
+long long g6e_submit(int n, const double *m, const double *x, const double *v, double t_end,
+                     double eta, double eps2)
+{
+    if (!X.open) return efail(-1, "g6e_submit: engine not open");
+    if (n < 1 || n > X.P.cap) return efail(-1, "g6e_submit: n = %d outside [1, %d]", n, X.P.cap);
+    if (!(t_end > 0) || !(eta > 0) || !(eps2 >= 0))
+        return efail(-1, "g6e_submit: bad t_end/eta/eps2");
+    Job j;
+    j.id = X.next_id++;
+    j.n = n;
+    j.m.assign(m, m + n);
+    j.x.assign(x, x + 3 * (size_t)n);
+    j.v.assign(v, v + 3 * (size_t)n);
+    j.t_end = t_end;
+    j.eta = eta;
+    j.eps2 = eps2;
+    X.queue.push_back(std::move(j));
+    return X.next_id - 1;
+}
+
+int g6e_run(int steps)
+{
+    if (!X.open) return efail(-1, "g6e_run: engine not open");
+    int done = 0;
+    while (true) {
+        if (retire()) return -1;
+        if (admit()) return -1;
+        if (X.items_dirty && rebuild_items()) return -1;
+        if (busy_count() == 0 || done >= steps) break;
+        hipGraphExec_t exec;
+        if (get_graph(exec)) return -1;
+        ECHK(hipEventRecord(X.ev0, X.stream));
+        ECHK(hipGraphLaunch(exec, X.stream));
+        ECHK(hipEventRecord(X.ev1, X.stream));
+        ECHK(hipMemcpyAsync(X.h_phase, X.d_phase, sizeof(int) * X.max_slots, hipMemcpyDeviceToHost,
+                            X.stream));
+        ECHK(hipStreamSynchronize(X.stream));
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, X.ev0, X.ev1) == hipSuccess) X.dev_seconds += ms * 1e-3;
+        X.engine_steps += X.steps_per_graph;
+        done += X.steps_per_graph;
+    }
+    return busy_count() + (int)X.queue.size();
+}
+
+int g6e_poll(int max, long long *job_ids)
+{
+    int k = 0;
+    while (k < max && !X.finished.empty()) {
+        job_ids[k++] = X.finished.front();
+        X.finished.erase(X.finished.begin());
+    }
+    return k;
+}
+
+int g6e_result(long long job_id, int n, double *x, double *v, double *t_final, long long *steps,
+               double *e0, double *e1)
+{
+    auto it = X.results.find(job_id);
+    if (it == X.results.end()) return efail(-1, "g6e_result: job %lld has no result", job_id);
+    Result &r = it->second;
+    if (n != r.n) return efail(-1, "g6e_result: n = %d, job has %d", n, r.n);
+    if (x) memcpy(x, r.x.data(), sizeof(double) * 3 * (size_t)n);
+    if (v) memcpy(v, r.v.data(), sizeof(double) * 3 * (size_t)n);
+    if (t_final) *t_final = r.t;
+    if (steps) *steps = r.steps;
+    if (e0) *e0 = r.e0;
+    if (e1) *e1 = r.e1;
+    const int rc = r.failed ? 1 : 0;
+    X.results.erase(it);
+    return rc;
+}
+
+int g6e_stats(long long *engine_steps, double *pairs, long long *finished, double *device_seconds)
+{
+    if (engine_steps) *engine_steps = X.engine_steps;
+    if (pairs) *pairs = X.pairs;
+    if (finished) *finished = X.n_finished;
+    if (device_seconds) *device_seconds = X.dev_seconds;
+    return 0;
+}
+
+int g6e_set_option(const char *key, long long value)
+{
+    std::string k(key ? key : "");
+    if (k == "steps_per_graph") X.steps_per_graph = (int)std::max(1LL, value);
+    else if (k == "jsplit_target") X.jsplit_target = (int)std::max(0LL, value);
+    else if (k == "max_split") {
+        if (X.open) return efail(-1, "max_split is fixed while open");
+        X.max_split = (int)std::max(1LL, std::min(64LL, value));
+    } else if (k == "lookahead") X.lookahead = (int)std::max(1LL, value);
+    else if (k == "dt_max_bits") X.dt_max = std::ldexp(1.0, (int)value);
+    else if (k == "max_steps") X.max_steps = std::max(1LL, value);
+    else return -1;
+    return 0;
+}
+
+long long g6e_get_option(const char *key)
+{
+    std::string k(key ? key : "");
+    if (k == "steps_per_graph") return X.steps_per_graph;
+    if (k == "jsplit_target") return X.jsplit_target;
+    if (k == "max_split") return X.max_split;
+    if (k == "lookahead") return X.lookahead;
+    if (k == "max_steps") return X.max_steps;
+    if (k == "pool_capacity") return X.P.cap;
+    if (k == "resident") return busy_count();
+    if (k == "queued") return (long long)X.queue.size();
+    return -1;
+}
+
+const char *g6e_last_error(void) { return X.err.c_str(); }
+
+}  // extern "C"

@@ -47,7 +47,7 @@ def test_grape6_header_lists_all_19_reference_entry_points():
 
 def test_every_declared_symbol_is_exported(lib):
     exported = _exported()
-    for h in ("grape6.h", "g6x.h"):
+    for h in ("grape6.h", "g6x.h", "g6ens.h"):
         missing = [n for n in _declared(h) if n not in exported]
         assert not missing, (h, missing)
 
@@ -56,6 +56,7 @@ def test_ctypes_table_covers_headers(lib):
     from starlab_amd import _abi
     assert sorted(_abi.GRAPE6) == _declared("grape6.h")
     assert sorted(_abi.G6X) == _declared("g6x.h")
+    assert sorted(_abi.G6E) == _declared("g6ens.h")
 
 
 def _run(code, env=None):

@@ -1,0 +1,122 @@
+"""GPU: the batched ensemble engine (include/g6ens.h).
+
+* each system against the FP64 shared-step oracle (oracle.hermite_shared,
+  same kira rules, FP64 pairs): over a short span the step sequence agrees
+  and positions agree to 1e-5 of the system size; energy conservation over
+  a longer span is no worse than 10x the oracle's + 1e-7;
+* a system's result is bitwise identical alone and in a mix admitted at
+  other times into other slots (continuous batching does not perturb it);
+* admission by HBM budget: a pool too small for the whole mix still
+  finishes every job, with jobs admitted as earlier ones retire.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from starlab_amd.ens import Ensemble, binary_single, plummer_cluster
+
+pytestmark = pytest.mark.gpu
+
+GB = 1 << 30
+
+
+def _jobs(seed, sizes, t_end, eta=0.05):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in sizes:
+        if n == 3:
+            m, x, v = binary_single(rng)
+            eps2 = 0.0
+        else:
+            m, x, v = plummer_cluster(n, rng)
+            eps2 = (4.0 / n) ** 2
+        out.append(dict(m=m, x=x, v=v, t_end=t_end, eta=eta, eps2=eps2))
+    return out
+
+
+def _run(jobs, budget=2 * GB, max_systems=4096, **opt):
+    e = Ensemble(budget, max_systems, **opt)
+    try:
+        ids = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in jobs]
+        res = e.run_until_idle()
+        stats = e.stats()
+    finally:
+        e.close()
+    return [res[i] for i in ids], stats
+
+
+def test_against_oracle_short_span():
+    jobs = _jobs(1, [3, 3, 16, 64, 200, 700, 1300], t_end=1.0 / 32)
+    res, _ = _run(jobs)
+    same_steps = 0
+    for j, r in zip(jobs, res):
+        xf, vf, t, steps, e0, e1, rc = oracle.hermite_shared(j["m"], j["x"], j["v"], j["t_end"],
+                                                            j["eta"], j["eps2"])
+        assert not r["failed"] and rc == 0
+        assert r["t"] == t
+        size = np.abs(j["x"]).max()
+        print(len(j["m"]), "steps gpu/cpu", r["steps"], steps, "dx",
+              np.abs(r["x"] - xf).max() / size, flush=True)
+        if r["steps"] == steps:
+            same_steps += 1
+            assert np.abs(r["x"] - xf).max() <= 1e-5 * size
+        assert abs(r["e0"] - e0) <= 1e-6 * abs(e0)
+    assert same_steps >= len(jobs) - 1
+
+
+def test_energy_conservation_vs_oracle():
+    jobs = _jobs(2, [3, 32, 256, 1024], t_end=0.5)
+    res, _ = _run(jobs)
+    for j, r in zip(jobs, res):
+        _, _, t, steps, e0, e1, rc = oracle.hermite_shared(j["m"], j["x"], j["v"], j["t_end"],
+                                                          j["eta"], j["eps2"])
+        de_gpu = abs((r["e1"] - r["e0"]) / r["e0"])
+        de_cpu = abs((e1 - e0) / e0)
+        print(len(j["m"]), "dE/E gpu", de_gpu, "cpu", de_cpu, "steps", r["steps"], steps,
+              flush=True)
+        assert de_gpu <= 10 * de_cpu + 1e-7
+
+
+def test_alone_equals_mixed_bitwise():
+    target = _jobs(3, [500], t_end=1.0 / 8)[0]
+    target3 = _jobs(4, [3], t_end=2.0)[0]
+    alone, _ = _run([target, target3])
+    mix = _jobs(5, [3, 64, 2048, 3, 700, 3, 128, 1500, 3, 3, 40], t_end=1.0 / 8)
+    e = Ensemble(2 * GB, 64)
+    try:
+        others = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in mix[:6]]
+        e.run(16)                                   # the mix is already running
+        a = e.submit(target["m"], target["x"], target["v"], target["t_end"], target["eta"],
+                     target["eps2"])
+        more = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in mix[6:]]
+        b = e.submit(target3["m"], target3["x"], target3["v"], target3["t_end"], target3["eta"],
+                     target3["eps2"])
+        res = e.run_until_idle()
+    finally:
+        e.close()
+    for ref, got in ((alone[0], res[a]), (alone[1], res[b])):
+        np.testing.assert_array_equal(got["x"], ref["x"])
+        np.testing.assert_array_equal(got["v"], ref["v"])
+        assert got["steps"] == ref["steps"] and got["e1"] == ref["e1"]
+    assert len(res) == len(mix) + 2
+
+
+def test_admission_by_hbm_budget():
+    jobs = _jobs(6, [1000] * 6 + [3] * 10, t_end=1.0 / 64)
+    # room for ~2 clusters of 1000 at a time
+    budget = 2500 * (21 * 8 + 64 + 7 * 8 * 16)
+    res, stats = _run(jobs, budget=budget, max_systems=8)
+    assert stats["finished"] == len(jobs)
+    assert all(not r["failed"] and r["t"] == 1.0 / 64 for r in res)
+
+
+def test_step_limit_marks_failure():
+    job = _jobs(7, [64], t_end=1.0)[0]
+    e = Ensemble(GB, 4, max_steps=5)
+    try:
+        jid = e.submit(job["m"], job["x"], job["v"], job["t_end"], job["eta"], job["eps2"])
+        res = e.run_until_idle()
+    finally:
+        e.close()
+        e.option("max_steps", 50_000_000)      # engine options outlive the engine
+    assert res[jid]["failed"] and res[jid]["t"] < 1.0
