@@ -5,9 +5,8 @@
  * No counterpart in the reference's C ABI.  The integrator restates kira's
  * Hermite scheme for each system with one shared, power-of-two time step:
  *   predictor        include/_dyn_.h:290-295 (predict_loworder)
- *   force            hdyn_ev.C:2019-2036 (same device pair law as libg6's
- *                    force kernel: double-single positions, FP32 pairs, FP64
- *                    sums)
+ *   force            hdyn_ev.C:2019-2036, in FP64 (default) or with libg6's
+ *                    FP32 pair law (option "precision")
  *   corrector        hdyn_ev.C:1235-1250 (get_derivs), :1717-1718
  *   step criterion   hdyn_ev.C:627-655 (Aarseth), :855-887 (halve at most
  *                    once, double only when commensurate)
@@ -66,7 +65,11 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
 
 /* Options: "steps_per_graph" (default 8), "jsplit_target" (j per split
  * within a system, default 512; 0 = no split), "max_split" (default 16),
- * "lookahead" (queued jobs examined per admission pass, default 64).
+ * "lookahead" (queued jobs examined per admission pass, default 64),
+ * "precision" (pair arithmetic of jobs submitted afterwards: 64 = FP64 in the
+ * CPU oracle's operation order, the default; 32 = FP32 pairs on
+ * double-single positions with a noise-floored step criterion),
+ * "max_steps" (per system; beyond it the job fails).
  * Returns 0, or -1 for an unknown key. */
 int       g6e_set_option(const char *key, long long value);
 long long g6e_get_option(const char *key);

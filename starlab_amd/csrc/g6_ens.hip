@@ -7,9 +7,14 @@
 // slot.  One engine step is four kernels, captured in a HIP graph:
 //   k_ens_predict   thread per particle: kira's predictor to t + dt
 //                   (include/_dyn_.h:290-295), double-single copy for pairs
-//   k_ens_force     one wave per (system, 64-i chunk, j-split): the libg6 pair
-//                   law (FP32 pairs on double-single positions, 256-j FP32
-//                   tiles folded into FP64) against the system's own j
+//   k_ens_force     one wave per (system, 64-i chunk, j-split), against the
+//                   system's own j.  Precision per system:
+//                   64 (default): FP64 pair law written in the CPU oracle's
+//                   exact operation order (hdyn_ev.C:2019-2036; IEEE division
+//                   and square root, no contraction), so with one j-split a
+//                   trajectory is bitwise the oracle's;
+//                   32: the libg6 pair law (FP32 pairs on double-single
+//                   positions, 256-j FP32 tiles folded into FP64)
 //   k_ens_correct   thread per particle: sum j-splits in order, Hermite
 //                   corrector (hdyn_ev.C:1235-1250, 1717-1718), Aarseth step
 //                   (hdyn_ev.C:627-655) -> atomic min into the system header
@@ -19,6 +24,14 @@
 // Atomic min of a positive double through its bit pattern is order-free, and
 // every sum runs in a fixed order, so a system's trajectory does not depend
 // on what else is resident.
+//
+// FP32 pairs carry ~1e-7 relative noise per force evaluation.  The Aarseth
+// criterion differences successive accelerations (at3 = 2(a0 - a1) + ...,
+// the third derivative times dt^3/6), so at small steps that noise looks
+// like a large third derivative and the step collapses.  Precision 32
+// therefore subtracts the noise floor (kNoiseFP32 |a| per evaluation) from
+// |bt2| and |at3| before applying the criterion; precision 64 uses kira's
+// criterion unchanged.
 #include "g6_internal.h"
 #include "../../include/g6ens.h"
 
@@ -42,8 +55,15 @@ struct SysHdr {
     double t, dt, t_end, eta, eps2, dt_max;
     unsigned long long pmin;      // bits of the smallest candidate step (positive double)
     long long steps;
-    int off, n, phase, nsplit, jchunk, pad;
+    int off, n, phase, nsplit, jchunk, prec;   // prec: 32 or 64
 };
+
+// FP64 predicted state for the precision-64 pair law (one scalar-cache line).
+struct JPred64 {
+    double x, y, z, m, vx, vy, vz, pad;
+};
+
+constexpr double kNoiseFP32 = 4.0e-7;   // relative force noise of the FP32 pair law
 
 struct Pool {
     int cap = 0;
@@ -51,6 +71,7 @@ struct Pool {
     double *xp = nullptr, *vp = nullptr;                             // [3][cap]
     double *m = nullptr, *pot = nullptr, *pot0 = nullptr;            // [cap]
     JPred *jp = nullptr;                                             // [cap]
+    JPred64 *jp64 = nullptr;                                         // [cap]
     double *part = nullptr;                                          // [max_split][7][cap]
     int max_split = 0;
 };
@@ -98,6 +119,10 @@ __global__ __launch_bounds__(64) void k_ens_predict(Pool P, const SysHdr *__rest
 #pragma unroll
     for (int k = 0; k < 6; ++k) q.pad[k] = 0.f;
     P.jp[p] = q;
+    JPred64 d;
+    d.x = xp[0]; d.y = xp[1]; d.z = xp[2]; d.m = P.m[p];
+    d.vx = vp[0]; d.vy = vp[1]; d.vz = vp[2]; d.pad = 0.0;
+    P.jp64[p] = d;
 }
 
 // ---------------------------------------------------------------------------
@@ -154,6 +179,70 @@ __device__ __forceinline__ void ens_tile(const JPred *__restrict__ jp, int tn, u
     for (; jj < tn; ++jj) ens_pair<MASK>(jp[jj], jj, rel, I, eps2, a);
 }
 
+// FP64 pair law in the CPU oracle's operation order (hdyn_ev.C:2019-2036 as
+// restated in oracle/g6_oracle.c), accumulating j in ascending order.
+struct DAcc {
+    double ax, ay, az, jx, jy, jz, p;
+};
+
+template <bool MASK>
+__device__ __forceinline__ void ens_pair64(const JPred64 &J, int jj, int self, const JPred64 &I,
+                                           double eps2, DAcc &a)
+{
+#pragma clang fp contract(off)
+    if (MASK && jj == self) return;
+    const double dx = J.x - I.x, dy = J.y - I.y, dz = J.z - I.z;
+    const double du = J.vx - I.vx, dv = J.vy - I.vy, dw = J.vz - I.vz;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double r2inv = 1.0 / (d2 + eps2);
+    const double a3 = -3.0 * (dx * du + dy * dv + dz * dw) * r2inv;
+    const double mrinv = J.m * sqrt(r2inv);
+    a.p -= mrinv;
+    const double mr3inv = mrinv * r2inv;
+    a.ax += mr3inv * dx;
+    a.ay += mr3inv * dy;
+    a.az += mr3inv * dz;
+    a.jx += mr3inv * (du + a3 * dx);
+    a.jy += mr3inv * (dv + a3 * dy);
+    a.jz += mr3inv * (dw + a3 * dz);
+}
+
+__device__ void ens_force64(const Pool &P, const SysHdr &h, const FItem &it)
+{
+    const int n = h.n;
+    const int i = it.i0 + threadIdx.x;
+    const int ic = i < n ? i : n - 1;
+    const JPred64 *__restrict__ base = P.jp64 + h.off;
+    const JPred64 I = base[ic];
+    const int self = i < n ? i : -1;
+    const int jbeg = it.split * h.jchunk;
+    const int jend = min(jbeg + h.jchunk, n);
+    const double eps2 = h.eps2;
+    DAcc a;
+    a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.p = 0.0;
+    for (int t = jbeg; t < jend; t += kTile) {
+        const int tn = min(kTile, jend - t);
+        const unsigned rel = (unsigned)(self - t);
+        if (__ballot(rel < (unsigned)tn) == 0) {
+            for (int jj = 0; jj < tn; ++jj) ens_pair64<false>(base[t + jj], t + jj, self, I, eps2, a);
+        } else {
+            for (int jj = 0; jj < tn; ++jj) ens_pair64<true>(base[t + jj], t + jj, self, I, eps2, a);
+        }
+    }
+    if (i < n) {
+        const size_t cap = P.cap;
+        const size_t p = (size_t)h.off + i;
+        double *out = P.part + (size_t)it.split * 7 * cap;
+        out[0 * cap + p] = a.ax;
+        out[1 * cap + p] = a.ay;
+        out[2 * cap + p] = a.az;
+        out[3 * cap + p] = a.jx;
+        out[4 * cap + p] = a.jy;
+        out[5 * cap + p] = a.jz;
+        out[6 * cap + p] = a.p;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_ens_force(Pool P, const SysHdr *__restrict__ H,
                                                   const FItem *__restrict__ items,
                                                   const int *__restrict__ nitems)
@@ -163,6 +252,10 @@ __global__ __launch_bounds__(64) void k_ens_force(Pool P, const SysHdr *__restri
     const FItem it = items[w];
     const SysHdr &h = H[it.slot];
     if (h.phase < INIT || h.phase > FINAL) return;
+    if (h.prec == 64) {
+        ens_force64(P, h, it);
+        return;
+    }
     const int n = h.n;
     const int lane = threadIdx.x;
     const int i = it.i0 + lane;
@@ -222,8 +315,15 @@ __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__
     if (i < h.n) {
         const size_t cap = P.cap;
         const size_t p = (size_t)h.off + i;
-        double a1[3] = {0, 0, 0}, j1[3] = {0, 0, 0}, pot = 0;
-        for (int s = 0; s < h.nsplit; ++s) {
+        // split 0 is taken as is (no "0 +" rounding), later splits added in order
+        double a1[3], j1[3], pot;
+        {
+            const double *in = P.part;
+            a1[0] = in[0 * cap + p]; a1[1] = in[1 * cap + p]; a1[2] = in[2 * cap + p];
+            j1[0] = in[3 * cap + p]; j1[1] = in[4 * cap + p]; j1[2] = in[5 * cap + p];
+            pot = in[6 * cap + p];
+        }
+        for (int s = 1; s < h.nsplit; ++s) {
             const double *in = P.part + (size_t)s * 7 * cap;
             a1[0] += in[0 * cap + p]; a1[1] += in[1 * cap + p]; a1[2] += in[2 * cap + p];
             j1[0] += in[3 * cap + p]; j1[1] += in[4 * cap + p]; j1[2] += in[5 * cap + p];
@@ -261,8 +361,16 @@ __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__
             P.pot[p] = pot;
             // Aarseth criterion, hdyn_ev.C:627-655
             const double dt2 = dt * dt, dt4 = dt2 * dt2, dt6 = dt4 * dt2;
-            const double l2 = 36 * dot3(at3, at3) / dt6;
-            const double k2 = 4 * dot3(bt2, bt2) / dt4;
+            double l2 = 36 * dot3(at3, at3) / dt6;
+            double k2 = 4 * dot3(bt2, bt2) / dt4;
+            if (h.prec != 64) {
+                // remove the FP32 force noise: a0 - a1 carries ~2 kNoiseFP32 |a|
+                const double na = kNoiseFP32 * sqrt(dot3(a1, a1));
+                const double at3m = fmax(sqrt(dot3(at3, at3)) - 4 * na, 0.0);
+                const double bt2m = fmax(sqrt(dot3(bt2, bt2)) - 6 * na, 0.0);
+                l2 = 36 * at3m * at3m / dt6;
+                k2 = 4 * bt2m * bt2m / dt4;
+            }
             const double j2 = dot3(j1, j1);
             const double asq = dot3(a1, a1);
             cand = h.eta * sqrt((sqrt(asq * k2) + j2) / (sqrt(j2 * l2) + k2));
@@ -378,6 +486,7 @@ struct Job {
     int n;
     std::vector<double> m, x, v;
     double t_end, eta, eps2;
+    int prec;
 };
 
 struct SlotInfo {
@@ -439,6 +548,7 @@ struct Engine {
     int jsplit_target = 512;
     int max_split = 16;
     int lookahead = 64;
+    int precision = 64;
     double dt_max = 0.125;
     long long max_steps = 50000000;
     // stats
@@ -472,7 +582,8 @@ template <class T> void hfree(T *&p) { if (p) { (void)hipHostFree(p); p = nullpt
 
 size_t bytes_per_particle(int max_split)
 {
-    return sizeof(double) * (3 * 6 + 3) + sizeof(JPred) + sizeof(double) * 7 * (size_t)max_split;
+    return sizeof(double) * (3 * 6 + 3) + sizeof(JPred) + sizeof(JPred64) +
+           sizeof(double) * 7 * (size_t)max_split;
 }
 
 int nsplit_for(int n)
@@ -677,7 +788,7 @@ int admit()
         h.dt_max = std::min(X.dt_max, it->t_end);
         h.pmin = 0x7FF0000000000000ull;
         h.steps = 0;
-        h.off = off; h.n = it->n; h.phase = INIT; h.nsplit = si.nsplit;
+        h.off = off; h.n = it->n; h.phase = INIT; h.nsplit = si.nsplit; h.prec = it->prec;
         h.jchunk = ((it->n + si.nsplit - 1) / si.nsplit + kTile - 1) / kTile * kTile;
         if (si.nsplit > 1) {
             // split boundaries on whole tiles; recompute the split count
@@ -845,6 +956,7 @@ int g6e_open(long long hbm_budget_bytes, int max_systems)
     ECHK(hipMalloc((void **)&X.P.pot, sizeof(double) * c));
     ECHK(hipMalloc((void **)&X.P.pot0, sizeof(double) * c));
     ECHK(hipMalloc((void **)&X.P.jp, sizeof(JPred) * c));
+    ECHK(hipMalloc((void **)&X.P.jp64, sizeof(JPred64) * c));
     ECHK(hipMalloc((void **)&X.P.part, sizeof(double) * 7 * c * (size_t)X.max_split));
     X.max_slots = max_systems;
     ECHK(hipMalloc((void **)&X.d_hdr, sizeof(SysHdr) * max_systems));
@@ -889,7 +1001,7 @@ int g6e_close(void)
     for (auto &g : X.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
     X.graphs.clear();
     dfree(X.P.x); dfree(X.P.v); dfree(X.P.a); dfree(X.P.j); dfree(X.P.xp); dfree(X.P.vp);
-    dfree(X.P.m); dfree(X.P.pot); dfree(X.P.pot0); dfree(X.P.jp); dfree(X.P.part);
+    dfree(X.P.m); dfree(X.P.pot); dfree(X.P.pot0); dfree(X.P.jp); dfree(X.P.jp64); dfree(X.P.part);
     dfree(X.d_hdr); dfree(X.d_phase); hfree(X.h_phase);
     hfree(X.h_hstage); dfree(X.d_hstage); hfree(X.h_hslots); dfree(X.d_hslots);
     dfree(X.d_counts); hfree(X.h_counts);
@@ -923,6 +1035,7 @@ long long g6e_submit(int n, const double *m, const double *x, const double *v, d
     j.t_end = t_end;
     j.eta = eta;
     j.eps2 = eps2;
+    j.prec = X.precision;
     X.queue.push_back(std::move(j));
     return X.next_id - 1;
 }
@@ -1000,7 +1113,10 @@ int g6e_set_option(const char *key, long long value)
     } else if (k == "lookahead") X.lookahead = (int)std::max(1LL, value);
     else if (k == "dt_max_bits") X.dt_max = std::ldexp(1.0, (int)value);
     else if (k == "max_steps") X.max_steps = std::max(1LL, value);
-    else return -1;
+    else if (k == "precision") {
+        if (value != 32 && value != 64) return efail(-1, "precision must be 32 or 64");
+        X.precision = (int)value;
+    } else return -1;
     return 0;
 }
 
@@ -1012,6 +1128,7 @@ long long g6e_get_option(const char *key)
     if (k == "max_split") return X.max_split;
     if (k == "lookahead") return X.lookahead;
     if (k == "max_steps") return X.max_steps;
+    if (k == "precision") return X.precision;
     if (k == "pool_capacity") return X.P.cap;
     if (k == "resident") return busy_count();
     if (k == "queued") return (long long)X.queue.size();
