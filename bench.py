@@ -9,7 +9,9 @@ reduction + FP64 nearest-neighbour refinement, inputs resident in HBM.
 
 Prints one JSON line: pair interactions per second (N*(N-1)*steps / time),
 the roofline of the force kernel against the FP32 vector peak (60 flop per
-pair interaction), and the CPU direct-sum baseline measured on this host.
+pair interaction), the ensemble legs (batched clusters; a served Poisson
+stream with p50/p99 latency) and the CPU direct-sum baseline measured on
+this host.
 """
 from __future__ import annotations
 
@@ -85,6 +87,24 @@ def cpu_baseline(inp, target_s=10.0):
                       f"neighbour, oracle/g6_oracle.c, OpenMP over i)"}
 
 
+def ensemble_legs(seconds=2.0):
+    """Batched ensemble (4096 Plummer clusters, N ~ U[32, 2048], integrated
+    to t = 1/32) and the served Poisson stream (2000 req/s, 70 % binary-single
+    scatterings to t = 32, 30 % clusters N ~ U[256, 4096] to t = 1/32)."""
+    from starlab_amd import serve
+    out = {}
+    for prec in (32, 64):
+        batch = serve.cluster_batch(4096, 32, 2048, 4, 1.0 / 32, prec=prec)
+        r = serve.run_batch(batch)
+        out[f"batch_fp{prec}"] = {k: r[k] for k in ("jobs", "wall_s", "systems_per_s",
+                                                    "pairs_per_s", "p50_s", "p99_s",
+                                                    "de_median", "de_max", "failed")}
+    jobs = serve.serving_mix(int(2000 * seconds), 5)
+    r = serve.run_poisson(jobs, 2000.0)
+    out["serving"] = r
+    return out
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "force_n65536_traffic.json")
     try:
@@ -101,6 +121,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-ens", action="store_true", help="skip the ensemble / serving legs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "libg6 drives one GPU per process; this build measures "
@@ -174,12 +195,18 @@ def main():
         "force_geometry": {"workgroups": wg, "nsplit": nsplit, "jchunk": jchunk},
         "nn_refinements": refinements,
     }
+    g.close()
+    if not a.no_ens:
+        log("bench: ensemble and serving legs")
+        ens = ensemble_legs()
+        out["ensemble"] = ens
+        out["ensemble_p50_s"] = ens["serving"]["p50_s"]
+        out["ensemble_p99_s"] = ens["serving"]["p99_s"]
     if not a.no_cpu:
         log("bench: CPU baseline")
         cb = cpu_baseline(inp)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
-    g.close()
 
 
 if __name__ == "__main__":

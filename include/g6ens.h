@@ -69,7 +69,10 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * "precision" (pair arithmetic of jobs submitted afterwards: 64 = FP64 in the
  * CPU oracle's operation order, the default; 32 = FP32 pairs on
  * double-single positions with a noise-floored step criterion),
- * "max_steps" (per system; beyond it the job fails).
+ * "max_steps" (per system; beyond it the job fails),
+ * "light_max" (systems with n <= light_max run in the light lane, which
+ * keeps stepping on its own stream while a heavy-lane graph runs; default
+ * 64).
  * Returns 0, or -1 for an unknown key. */
 int       g6e_set_option(const char *key, long long value);
 long long g6e_get_option(const char *key);

@@ -1,0 +1,183 @@
+"""A kira-style block-timestep Hermite driver for point-mass systems.
+
+This is the caller side of the GRAPE-6 ABI, restated for top-level point
+masses (no binary tree, no stellar evolution): it reproduces the order in
+which kira's GRAPE driver calls the library (src/node/dyn/hdyn/evolve/
+hdyn_grape6.C:3067-3338) and kira's integration rules:
+
+  block selection   all particles with the smallest t + dt
+                    (fast_get_nodes_to_move, hdyn_schedule.C:641-745)
+  j update          the previous block's particles re-sent with tj, a/2,
+                    j/6, k/18 (send_j_node_to_grape, hdyn_grape6.C:197-288,
+                    called for previous_nodes at :3112-3113)
+  i prediction      on the host, predict_loworder (include/_dyn_.h:290-295)
+  force             g6_set_ti_, g6calc_firsthalf_/g6calc_lasthalf2_ in
+                    chunks of g6_npipes_() (hdyn_grape6.C:474-910, 3219-3338)
+  corrector         get_derivs + corrector (hdyn_ev.C:1235-1250, 1717-1718),
+                    k/18 = bt2 / (9 dt^2) (hdyn_ev.C:1076)
+  time step         Aarseth criterion and the power-of-two rule
+                    (hdyn_ev.C:521-887); first step set_first_timestep
+                    (hdyn_ev.C:377-410)
+
+The force backend is pluggable.  ``G6Backend`` drives libg6.so through the
+C ABI; tests supply an FP64 host backend to obtain the all-CPU reference run.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+ONE3 = 1.0 / 3.0
+ONE12 = 1.0 / 12.0
+
+
+class G6Backend:
+    """libg6 through the GRAPE-6 ABI, one particle per g6_set_j_particle_."""
+
+    def __init__(self, g):
+        self.g = g
+
+    def open(self, n):
+        assert self.g.open() == 0, self.g.last_error()
+        self.npipes = self.g.npipes()
+
+    def send_j(self, idx, tj, dtj, m, x, v, a, j, k18):
+        for k, i in enumerate(idx):
+            rc = self.g.set_j_particle(int(i), int(i), float(tj[k]), float(dtj[k]), float(m[k]),
+                                       k18[k], j[k] / 6.0, a[k] / 2.0, v[k], x[k])
+            if rc:
+                raise RuntimeError(self.g.last_error())
+
+    def force(self, nj, t, idx, xi, vi, eps2):
+        self.g.set_ti(t)
+        ni = len(idx)
+        acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)
+        nn = np.zeros(ni, np.int32)
+        h2 = np.zeros(ni)
+        for s in range(0, ni, self.npipes):
+            e = min(ni, s + self.npipes)
+            self.g.calc_firsthalf(nj, idx[s:e], xi[s:e], vi[s:e], eps2, h2[s:e])
+            rc, a_, j_, p_, n_ = self.g.calc_lasthalf2(nj, e - s)
+            if rc:
+                raise RuntimeError(f"g6calc_lasthalf2_ returned {rc}: {self.g.last_error()}")
+            acc[s:e], jerk[s:e], pot[s:e], nn[s:e] = a_, j_, p_, n_
+        return acc, jerk, pot, nn
+
+    def close(self):
+        self.g.close()
+
+
+def first_step(eta, a, j, pot, dt_max):
+    """set_first_timestep (hdyn_ev.C:377-410), vectorized."""
+    eta_init = eta / 16.0
+    a2 = (a * a).sum(1)
+    j2 = (j * j).sum(1)
+    dt_adot = np.where(j2 > 0, a2 / np.where(j2 > 0, j2, 1.0), 1e300)
+    dt_ff = np.where((pot < 0) & (a2 > 0), -pot / np.where(a2 > 0, a2, 1.0), 1e300)
+    dt = eta_init * np.sqrt(np.minimum(dt_adot, dt_ff))
+    dt = np.minimum(np.minimum(dt, eta_init), dt_max)
+    return 2.0 ** np.floor(np.log2(dt))
+
+
+class BlockHermite:
+    def __init__(self, m, x, v, eta=0.1, eps2=0.0, dt_max=0.125):
+        self.n = len(m)
+        self.m = np.asarray(m, dtype=np.float64).copy()
+        self.x = np.asarray(x, dtype=np.float64).copy()
+        self.v = np.asarray(v, dtype=np.float64).copy()
+        self.a = np.zeros_like(self.x)
+        self.j = np.zeros_like(self.x)
+        self.k18 = np.zeros_like(self.x)
+        self.pot = np.zeros(self.n)
+        self.t = np.zeros(self.n)
+        self.dt = np.zeros(self.n)
+        self.eta = eta
+        self.eps2 = eps2
+        self.dt_max = dt_max
+        self.blocks = 0
+        self.particle_steps = 0
+
+    def predict(self, idx, t):
+        """predict_loworder (include/_dyn_.h:290-295) for particles idx."""
+        dt = (t - self.t[idx])[:, None]
+        dt3 = dt * ONE3
+        dt2 = dt * 0.5
+        xp = ((self.j[idx] * dt3 + self.a[idx]) * dt2 + self.v[idx]) * dt + self.x[idx]
+        vp = (self.j[idx] * dt2 + self.a[idx]) * dt + self.v[idx]
+        return xp, vp
+
+    def run(self, backend, t_end, on_block=None):
+        n = self.n
+        allp = np.arange(n, dtype=np.int32)
+        backend.open(n)
+        try:
+            # initial forces (all particles at t = 0), first steps
+            backend.send_j(allp, self.t, np.ones(n), self.m, self.x, self.v, self.a, self.j,
+                           self.k18)
+            a, j, pot, nn = backend.force(n, 0.0, allp, self.x, self.v, self.eps2)
+            self.a[:], self.j[:], self.pot[:] = a, j, pot
+            self.dt[:] = first_step(self.eta, a, j, pot, min(self.dt_max, t_end))
+            prev = allp
+            while True:
+                tn = self.t + self.dt
+                t_next = tn.min()
+                if t_next > t_end:
+                    break
+                blk = np.flatnonzero(tn == t_next).astype(np.int32)
+                # previous block's particles go to j-memory (hdyn_grape6.C:3112-3113)
+                backend.send_j(prev, self.t[prev], self.dt[prev], self.m[prev], self.x[prev],
+                               self.v[prev], self.a[prev], self.j[prev], self.k18[prev])
+                xi, vi = self.predict(blk, t_next)
+                a1, j1, pot, nn = backend.force(n, t_next, blk, xi, vi, self.eps2)
+                if on_block is not None:
+                    on_block(self, t_next, blk, xi, vi, a1, j1, pot, nn)
+                self.correct(blk, t_next, xi, vi, a1, j1, pot)
+                prev = blk
+                self.blocks += 1
+                self.particle_steps += len(blk)
+                if t_next == t_end:
+                    break
+        finally:
+            backend.close()
+
+    def correct(self, blk, t_next, xp, vp, a1, j1, pot):
+        dt = (t_next - self.t[blk])[:, None]
+        a0 = self.a[blk]
+        j0 = self.j[blk]
+        # get_derivs (hdyn_ev.C:1235-1250)
+        bt2 = -3 * (a0 - a1) - dt * (2 * j0 + j1)
+        at3 = 2 * (a0 - a1) + dt * (j0 + j1)
+        # corrector (hdyn_ev.C:1717-1718)
+        self.x[blk] = xp + (0.05 * at3 + ONE12 * bt2) * dt * dt
+        self.v[blk] = vp + (0.25 * at3 + ONE3 * bt2) * dt
+        self.a[blk] = a1
+        self.j[blk] = j1
+        self.pot[blk] = pot
+        self.k18[blk] = bt2 / (9.0 * dt * dt)
+        self.t[blk] = t_next
+        # Aarseth criterion (hdyn_ev.C:627-655)
+        d = dt[:, 0]
+        dt2 = d * d
+        dt4 = dt2 * dt2
+        dt6 = dt4 * dt2
+        l2 = 36 * (at3 * at3).sum(1) / dt6
+        k2 = 4 * (bt2 * bt2).sum(1) / dt4
+        jj = (j1 * j1).sum(1)
+        aa = (a1 * a1).sum(1)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            nat = self.eta * np.sqrt((np.sqrt(aa * k2) + jj) / (np.sqrt(jj * l2) + k2))
+        nat = np.where(nat > 0, nat, np.inf)
+        # power-of-two rule (hdyn_ev.C:866-887)
+        T = self.dt[blk]
+        nd = T.copy()
+        half = nat < T
+        nd[half] = 0.5 * T[half]
+        two = (~half) & (nat >= 2 * T) & (np.fmod(t_next, 2 * T) == 0.0) & (2 * T <= self.dt_max)
+        nd[two] = 2 * T[two]
+        self.dt[blk] = nd
+
+    def energy(self, energies_fn):
+        """Total energy with a caller-supplied exact pair-sum (synchronised state)."""
+        ep, ek = energies_fn(self.m, self.x, self.v, self.eps2)
+        return ep + ek

@@ -56,6 +56,7 @@ struct SysHdr {
     unsigned long long pmin;      // bits of the smallest candidate step (positive double)
     long long steps;
     int off, n, phase, nsplit, jchunk, prec;   // prec: 32 or 64
+    int lane, pad;                             // 0 heavy, 1 light
 };
 
 // FP64 predicted state for the precision-64 pair law (one scalar-cache line).
@@ -394,11 +395,12 @@ __device__ __forceinline__ double pow2_below(double x)
 }
 
 __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restrict__ phase_out,
-                               long long max_steps, double dt_min)
+                               long long max_steps, double dt_min, int lane)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     SysHdr &h = H[s];
+    if (h.lane != lane || h.phase < INIT || h.phase > FINAL) return;
     const double nat = __longlong_as_double((long long)h.pmin);
     if (h.phase == INIT) {
         double dt = fmin(nat, h.dt_max);
@@ -492,7 +494,7 @@ struct Job {
 struct SlotInfo {
     bool busy = false;
     long long job = -1;
-    int off = 0, n = 0, nsplit = 1;
+    int off = 0, n = 0, nsplit = 1, lane = 0;
     std::vector<double> m, x0, v0;
     double eps2 = 0, t_end = 0;
 };
@@ -510,6 +512,22 @@ struct GraphExec {
     int chunk_grid = 0, item_grid = 0, steps = 0;
 };
 
+// Systems run in two lanes with their own work lists, graphs and streams:
+// lane 1 holds light systems (n <= light_max, e.g. three-body scatterings),
+// lane 0 the rest.  The light lane keeps stepping while a heavy graph runs,
+// so a light system's latency does not grow with the heavy step time.
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Chunk *d_chunks = nullptr, *h_chunks = nullptr;
+    FItem *d_items = nullptr, *h_items = nullptr;
+    int *d_counts = nullptr, *h_counts = nullptr;   // [0] chunks, [1] items
+    int chunk_cap = 0, item_cap = 0;
+    int n_chunks = 0, n_items = 0;
+    std::vector<GraphExec> graphs;
+    long long steps = 0;
+};
+
 struct Engine {
     bool open = false;
     hipStream_t stream = nullptr;
@@ -518,14 +536,7 @@ struct Engine {
     int max_slots = 0;
     SysHdr *d_hdr = nullptr;
     int *d_phase = nullptr, *h_phase = nullptr;
-    Chunk *d_chunks = nullptr;
-    FItem *d_items = nullptr;
-    int *d_counts = nullptr;          // [0] chunks, [1] items
-    int *h_counts = nullptr;
-    Chunk *h_chunks = nullptr;
-    FItem *h_items = nullptr;
-    int chunk_cap = 0, item_cap = 0;
-    int n_chunks = 0, n_items = 0;
+    Lane lanes[2];
     bool items_dirty = true;
     // staging
     double *h_stage = nullptr, *d_stage = nullptr;
@@ -541,14 +552,13 @@ struct Engine {
     long long next_id = 0;
     std::unordered_map<long long, Result> results;
     std::vector<long long> finished;
-    std::vector<GraphExec> graphs;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // options
     int steps_per_graph = 8;
     int jsplit_target = 512;
     int max_split = 16;
     int lookahead = 64;
     int precision = 64;
+    int light_max = 64;
     double dt_max = 0.125;
     long long max_steps = 50000000;
     // stats
@@ -619,25 +629,29 @@ int ensure_out(size_t particles)
     return 0;
 }
 
-int ensure_items(int chunks, int items)
+void destroy_graphs(Lane &L)
 {
-    if (chunks > X.chunk_cap) {
-        int cap = std::max(chunks, X.chunk_cap * 2);
-        hfree(X.h_chunks); dfree(X.d_chunks);
-        ECHK(hipHostMalloc((void **)&X.h_chunks, sizeof(Chunk) * cap, hipHostMallocDefault));
-        ECHK(hipMalloc((void **)&X.d_chunks, sizeof(Chunk) * cap));
-        X.chunk_cap = cap;
-        for (auto &g : X.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        X.graphs.clear();
+    for (auto &g : L.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    L.graphs.clear();
+}
+
+int ensure_items(Lane &L, int chunks, int items)
+{
+    if (chunks > L.chunk_cap) {
+        int cap = std::max(chunks, L.chunk_cap * 2);
+        hfree(L.h_chunks); dfree(L.d_chunks);
+        ECHK(hipHostMalloc((void **)&L.h_chunks, sizeof(Chunk) * cap, hipHostMallocDefault));
+        ECHK(hipMalloc((void **)&L.d_chunks, sizeof(Chunk) * cap));
+        L.chunk_cap = cap;
+        destroy_graphs(L);
     }
-    if (items > X.item_cap) {
-        int cap = std::max(items, X.item_cap * 2);
-        hfree(X.h_items); dfree(X.d_items);
-        ECHK(hipHostMalloc((void **)&X.h_items, sizeof(FItem) * cap, hipHostMallocDefault));
-        ECHK(hipMalloc((void **)&X.d_items, sizeof(FItem) * cap));
-        X.item_cap = cap;
-        for (auto &g : X.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        X.graphs.clear();
+    if (items > L.item_cap) {
+        int cap = std::max(items, L.item_cap * 2);
+        hfree(L.h_items); dfree(L.d_items);
+        ECHK(hipHostMalloc((void **)&L.h_items, sizeof(FItem) * cap, hipHostMallocDefault));
+        ECHK(hipMalloc((void **)&L.d_items, sizeof(FItem) * cap));
+        L.item_cap = cap;
+        destroy_graphs(L);
     }
     return 0;
 }
@@ -777,6 +791,7 @@ int admit()
         si.off = off;
         si.n = it->n;
         si.nsplit = nsplit_for(it->n);
+        si.lane = it->n <= X.light_max ? 1 : 0;
         si.m = it->m;
         si.x0 = it->x;
         si.v0 = it->v;
@@ -789,6 +804,7 @@ int admit()
         h.pmin = 0x7FF0000000000000ull;
         h.steps = 0;
         h.off = off; h.n = it->n; h.phase = INIT; h.nsplit = si.nsplit; h.prec = it->prec;
+        h.lane = si.lane;
         h.jchunk = ((it->n + si.nsplit - 1) / si.nsplit + kTile - 1) / kTile * kTile;
         if (si.nsplit > 1) {
             // split boundaries on whole tiles; recompute the split count
@@ -839,32 +855,36 @@ int admit()
 
 int rebuild_items()
 {
-    int nc = 0, ni = 0;
-    for (int s = 0; s < X.max_slots; ++s)
-        if (X.slots[s].busy) {
-            const int ch = (X.slots[s].n + 63) / 64;
-            nc += ch;
-            ni += ch * X.slots[s].nsplit;
+    for (int ln = 0; ln < 2; ++ln) {
+        Lane &L = X.lanes[ln];
+        int nc = 0, ni = 0;
+        for (int s = 0; s < X.max_slots; ++s)
+            if (X.slots[s].busy && X.slots[s].lane == ln) {
+                const int ch = (X.slots[s].n + 63) / 64;
+                nc += ch;
+                ni += ch * X.slots[s].nsplit;
+            }
+        if (ensure_items(L, std::max(nc, 1), std::max(ni, 1))) return -1;
+        int c = 0, w = 0;
+        for (int s = 0; s < X.max_slots; ++s) {
+            const SlotInfo &si = X.slots[s];
+            if (!si.busy || si.lane != ln) continue;
+            for (int i0 = 0; i0 < si.n; i0 += 64) {
+                L.h_chunks[c++] = Chunk{s, i0};
+                for (int sp = 0; sp < si.nsplit; ++sp) L.h_items[w++] = FItem{s, i0, sp, 0};
+            }
         }
-    if (ensure_items(std::max(nc, 1), std::max(ni, 1))) return -1;
-    int c = 0, w = 0;
-    for (int s = 0; s < X.max_slots; ++s) {
-        const SlotInfo &si = X.slots[s];
-        if (!si.busy) continue;
-        for (int i0 = 0; i0 < si.n; i0 += 64) {
-            X.h_chunks[c++] = Chunk{s, i0};
-            for (int sp = 0; sp < si.nsplit; ++sp) X.h_items[w++] = FItem{s, i0, sp, 0};
-        }
+        L.n_chunks = nc;
+        L.n_items = ni;
+        L.h_counts[0] = nc;
+        L.h_counts[1] = ni;
+        if (nc) ECHK(hipMemcpyAsync(L.d_chunks, L.h_chunks, sizeof(Chunk) * nc,
+                                    hipMemcpyHostToDevice, X.stream));
+        if (ni) ECHK(hipMemcpyAsync(L.d_items, L.h_items, sizeof(FItem) * ni,
+                                    hipMemcpyHostToDevice, X.stream));
+        ECHK(hipMemcpyAsync(L.d_counts, L.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice,
+                            X.stream));
     }
-    X.n_chunks = nc;
-    X.n_items = ni;
-    X.h_counts[0] = nc;
-    X.h_counts[1] = ni;
-    if (nc) ECHK(hipMemcpyAsync(X.d_chunks, X.h_chunks, sizeof(Chunk) * nc, hipMemcpyHostToDevice,
-                                X.stream));
-    if (ni) ECHK(hipMemcpyAsync(X.d_items, X.h_items, sizeof(FItem) * ni, hipMemcpyHostToDevice,
-                                X.stream));
-    ECHK(hipMemcpyAsync(X.d_counts, X.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice, X.stream));
     ECHK(hipStreamSynchronize(X.stream));
     X.items_dirty = false;
     return 0;
@@ -877,38 +897,40 @@ int bucket(int n)
     return b;
 }
 
-void enqueue_step(int cgrid, int igrid)
+void enqueue_step(int ln, int cgrid, int igrid)
 {
-    hipLaunchKernelGGL(k_ens_predict, dim3(cgrid), dim3(64), 0, X.stream, X.P, X.d_hdr, X.d_chunks,
-                       X.d_counts);
-    hipLaunchKernelGGL(k_ens_force, dim3(igrid), dim3(64), 0, X.stream, X.P, X.d_hdr, X.d_items,
-                       X.d_counts + 1);
-    hipLaunchKernelGGL(k_ens_correct, dim3(cgrid), dim3(64), 0, X.stream, X.P, X.d_hdr, X.d_chunks,
-                       X.d_counts);
-    hipLaunchKernelGGL(k_ens_finalize, dim3((X.max_slots + 255) / 256), dim3(256), 0, X.stream,
-                       X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30);
+    Lane &L = X.lanes[ln];
+    hipLaunchKernelGGL(k_ens_predict, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                       L.d_chunks, L.d_counts);
+    hipLaunchKernelGGL(k_ens_force, dim3(igrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                       L.d_items, L.d_counts + 1);
+    hipLaunchKernelGGL(k_ens_correct, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                       L.d_chunks, L.d_counts);
+    hipLaunchKernelGGL(k_ens_finalize, dim3((X.max_slots + 255) / 256), dim3(256), 0, L.stream,
+                       X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30, ln);
 }
 
-// A graph of steps_per_graph engine steps for the current grid buckets.
-int get_graph(hipGraphExec_t &exec)
+// A graph of steps_per_graph engine steps of lane ln for its grid buckets.
+int get_graph(int ln, hipGraphExec_t &exec)
 {
-    const int cg = bucket(X.n_chunks), ig = bucket(X.n_items);
-    for (auto &g : X.graphs)
+    Lane &L = X.lanes[ln];
+    const int cg = bucket(L.n_chunks), ig = bucket(L.n_items);
+    for (auto &g : L.graphs)
         if (g.chunk_grid == cg && g.item_grid == ig && g.steps == X.steps_per_graph) {
             exec = g.exec;
             return 0;
         }
     hipGraph_t graph;
-    ECHK(hipStreamBeginCapture(X.stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < X.steps_per_graph; ++s) enqueue_step(cg, ig);
-    ECHK(hipStreamEndCapture(X.stream, &graph));
+    ECHK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < X.steps_per_graph; ++s) enqueue_step(ln, cg, ig);
+    ECHK(hipStreamEndCapture(L.stream, &graph));
     GraphExec ge;
     ge.chunk_grid = cg;
     ge.item_grid = ig;
     ge.steps = X.steps_per_graph;
     ECHK(hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
-    X.graphs.push_back(ge);
+    L.graphs.push_back(ge);
     exec = ge.exec;
     return 0;
 }
@@ -938,8 +960,14 @@ int g6e_open(long long hbm_budget_bytes, int max_systems)
     if (max_systems < 1 || hbm_budget_bytes <= 0)
         return efail(-1, "g6e_open: bad budget %lld / systems %d", hbm_budget_bytes, max_systems);
     ECHK(hipStreamCreate(&X.stream));
-    ECHK(hipEventCreate(&X.ev0));
-    ECHK(hipEventCreate(&X.ev1));
+    for (Lane &L : X.lanes) {
+        ECHK(hipStreamCreate(&L.stream));
+        ECHK(hipEventCreate(&L.ev0));
+        ECHK(hipEventCreate(&L.ev1));
+        ECHK(hipMalloc((void **)&L.d_counts, sizeof(int) * 2));
+        ECHK(hipHostMalloc((void **)&L.h_counts, sizeof(int) * 2, hipHostMallocDefault));
+        L.steps = 0;
+    }
     X.P.max_split = X.max_split;
     const size_t bpp = bytes_per_particle(X.max_split);
     const long long cap = std::max(64LL, hbm_budget_bytes / (long long)bpp);
@@ -966,8 +994,6 @@ int g6e_open(long long hbm_budget_bytes, int max_systems)
     ECHK(hipMalloc((void **)&X.d_hstage, sizeof(SysHdr) * max_systems));
     ECHK(hipHostMalloc((void **)&X.h_hslots, sizeof(int) * max_systems, hipHostMallocDefault));
     ECHK(hipMalloc((void **)&X.d_hslots, sizeof(int) * max_systems));
-    ECHK(hipMalloc((void **)&X.d_counts, sizeof(int) * 2));
-    ECHK(hipHostMalloc((void **)&X.h_counts, sizeof(int) * 2, hipHostMallocDefault));
     // all slots FREE
     for (int s = 0; s < max_systems; ++s) {
         memset(&X.h_hstage[s], 0, sizeof(SysHdr));
@@ -998,20 +1024,23 @@ int g6e_close(void)
 {
     if (!X.open) return 0;
     (void)hipStreamSynchronize(X.stream);
-    for (auto &g : X.graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    X.graphs.clear();
+    for (Lane &L : X.lanes) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        destroy_graphs(L);
+        hfree(L.h_chunks); dfree(L.d_chunks); hfree(L.h_items); dfree(L.d_items);
+        dfree(L.d_counts); hfree(L.h_counts);
+        L.chunk_cap = L.item_cap = 0;
+        if (L.ev0) (void)hipEventDestroy(L.ev0);
+        if (L.ev1) (void)hipEventDestroy(L.ev1);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+        L = Lane();
+    }
     dfree(X.P.x); dfree(X.P.v); dfree(X.P.a); dfree(X.P.j); dfree(X.P.xp); dfree(X.P.vp);
     dfree(X.P.m); dfree(X.P.pot); dfree(X.P.pot0); dfree(X.P.jp); dfree(X.P.jp64); dfree(X.P.part);
     dfree(X.d_hdr); dfree(X.d_phase); hfree(X.h_phase);
     hfree(X.h_hstage); dfree(X.d_hstage); hfree(X.h_hslots); dfree(X.d_hslots);
-    dfree(X.d_counts); hfree(X.h_counts);
-    hfree(X.h_chunks); dfree(X.d_chunks); hfree(X.h_items); dfree(X.d_items);
-    X.chunk_cap = X.item_cap = 0;
     hfree(X.h_stage); dfree(X.d_stage); X.stage_cap = 0;
     hfree(X.h_out); dfree(X.d_out); hfree(X.h_src); dfree(X.d_src); X.out_cap = 0;
-    if (X.ev0) (void)hipEventDestroy(X.ev0);
-    if (X.ev1) (void)hipEventDestroy(X.ev1);
-    X.ev0 = X.ev1 = nullptr;
     (void)hipStreamDestroy(X.stream);
     X.stream = nullptr;
     X.slots.clear();
@@ -1049,16 +1078,42 @@ int g6e_run(int steps)
         if (admit()) return -1;
         if (X.items_dirty && rebuild_items()) return -1;
         if (busy_count() == 0 || done >= steps) break;
-        hipGraphExec_t exec;
-        if (get_graph(exec)) return -1;
-        ECHK(hipEventRecord(X.ev0, X.stream));
-        ECHK(hipGraphLaunch(exec, X.stream));
-        ECHK(hipEventRecord(X.ev1, X.stream));
+        Lane &H = X.lanes[0], &Lt = X.lanes[1];
+        const bool heavy = H.n_chunks > 0, light = Lt.n_chunks > 0;
+        hipGraphExec_t eh = nullptr, el = nullptr;
+        if (heavy && get_graph(0, eh)) return -1;
+        if (light && get_graph(1, el)) return -1;
+        double ms_total = 0;
+        if (heavy) {
+            ECHK(hipEventRecord(H.ev0, H.stream));
+            ECHK(hipGraphLaunch(eh, H.stream));
+            ECHK(hipEventRecord(H.ev1, H.stream));
+        }
+        if (light) {
+            // keep the light lane stepping until the heavy graph is done
+            int replays = 0;
+            do {
+                ECHK(hipEventRecord(Lt.ev0, Lt.stream));
+                ECHK(hipGraphLaunch(el, Lt.stream));
+                ECHK(hipEventRecord(Lt.ev1, Lt.stream));
+                ECHK(hipStreamSynchronize(Lt.stream));
+                float ms = 0;
+                if (!heavy && hipEventElapsedTime(&ms, Lt.ev0, Lt.ev1) == hipSuccess)
+                    ms_total += ms;
+                Lt.steps += X.steps_per_graph;
+                ++replays;
+            } while (heavy && hipStreamQuery(H.stream) == hipErrorNotReady && replays < 4096);
+        }
+        if (heavy) {
+            ECHK(hipStreamSynchronize(H.stream));
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, H.ev0, H.ev1) == hipSuccess) ms_total += ms;
+            H.steps += X.steps_per_graph;
+        }
         ECHK(hipMemcpyAsync(X.h_phase, X.d_phase, sizeof(int) * X.max_slots, hipMemcpyDeviceToHost,
                             X.stream));
         ECHK(hipStreamSynchronize(X.stream));
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, X.ev0, X.ev1) == hipSuccess) X.dev_seconds += ms * 1e-3;
+        X.dev_seconds += ms_total * 1e-3;
         X.engine_steps += X.steps_per_graph;
         done += X.steps_per_graph;
     }
@@ -1113,6 +1168,7 @@ int g6e_set_option(const char *key, long long value)
     } else if (k == "lookahead") X.lookahead = (int)std::max(1LL, value);
     else if (k == "dt_max_bits") X.dt_max = std::ldexp(1.0, (int)value);
     else if (k == "max_steps") X.max_steps = std::max(1LL, value);
+    else if (k == "light_max") X.light_max = (int)std::max(0LL, value);
     else if (k == "precision") {
         if (value != 32 && value != 64) return efail(-1, "precision must be 32 or 64");
         X.precision = (int)value;
@@ -1129,6 +1185,9 @@ long long g6e_get_option(const char *key)
     if (k == "lookahead") return X.lookahead;
     if (k == "max_steps") return X.max_steps;
     if (k == "precision") return X.precision;
+    if (k == "light_max") return X.light_max;
+    if (k == "light_steps") return X.lanes[1].steps;
+    if (k == "heavy_steps") return X.lanes[0].steps;
     if (k == "pool_capacity") return X.P.cap;
     if (k == "resident") return busy_count();
     if (k == "queued") return (long long)X.queue.size();

@@ -1,0 +1,136 @@
+"""Ensemble workloads on the batched engine: a fixed batch (all jobs at once)
+and a served Poisson stream (continuous batching, p50/p99 latency).
+
+Jobs are generated before the clock starts; latency is measured from each
+job's scheduled arrival to the moment the host sees it finished, so queueing
+for slots / HBM budget is included.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from .ens import Ensemble, binary_single, plummer_cluster
+
+GB = 1 << 30
+
+
+def cluster_job(n, rng, t_end, eta=0.1, prec=32):
+    m, x, v = plummer_cluster(n, rng)
+    return dict(kind="cluster", n=n, m=m, x=x, v=v, t_end=t_end, eta=eta, eps2=(4.0 / n) ** 2,
+                prec=prec)
+
+
+def scatter_job(rng, t_end, eta=0.1, r_init=10.0, prec=64):
+    m, x, v = binary_single(rng, r_init=r_init)
+    return dict(kind="3body", n=3, m=m, x=x, v=v, t_end=t_end, eta=eta, eps2=0.0, prec=prec)
+
+
+def cluster_batch(count, nmin, nmax, seed, t_end, eta=0.1, prec=32):
+    """`count` Plummer clusters with N uniform in [nmin, nmax]."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, count)
+    return [cluster_job(int(n), rng, t_end, eta, prec) for n in sizes]
+
+
+def _submit(eng, j):
+    eng.option("precision", j.get("prec", 64))
+    return eng.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"])
+
+
+def serving_mix(count, seed, frac_3body=0.7, nmin=256, nmax=4096, t3=32.0, tc=1.0 / 32,
+                eta=0.1):
+    """The serving workload: 70 % binary-single scatterings (FP64 pairs), 30 %
+    clusters (FP32 pairs)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(count):
+        if rng.uniform() < frac_3body:
+            jobs.append(scatter_job(rng, t3, eta))
+        else:
+            jobs.append(cluster_job(int(rng.integers(nmin, nmax + 1)), rng, tc, eta))
+    return jobs
+
+
+def _pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
+
+
+def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, **opt):
+    """Submit every job at t = 0 and run to completion."""
+    eng = Ensemble(budget, max_systems, steps_per_graph=chunk, **opt)
+    try:
+        t0 = time.perf_counter()
+        ids = [_submit(eng, j) for j in jobs]
+        done_at = {}
+        results = {}
+        while True:
+            left = eng.run(chunk)
+            now = time.perf_counter() - t0
+            for jid in eng.poll():
+                done_at[jid] = now
+                results[jid] = eng.result(jid)
+            if left == 0:
+                break
+        wall = time.perf_counter() - t0
+        st = eng.stats()
+    finally:
+        eng.close()
+    lat = [done_at[i] for i in ids]
+    de = [abs((results[i]["e1"] - results[i]["e0"]) / results[i]["e0"]) for i in ids]
+    return dict(jobs=len(jobs), wall_s=wall, systems_per_s=len(jobs) / wall,
+                pairs=st["pairs"], pairs_per_s=st["pairs"] / wall,
+                device_s=st["device_seconds"], engine_steps=st["engine_steps"],
+                p50_s=_pct(lat, 50), p99_s=_pct(lat, 99),
+                failed=sum(results[i]["failed"] for i in ids),
+                de_median=_pct(de, 50), de_max=float(np.max(de)))
+
+
+def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, **opt):
+    """Serve `jobs` arriving as a Poisson stream of `rate` per second."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
+    eng = Ensemble(budget, max_systems, steps_per_graph=chunk, **opt)
+    try:
+        t0 = time.perf_counter()
+        nxt = 0
+        arrived = {}
+        done_at = {}
+        results = {}
+        kind = {}
+        while len(done_at) < len(jobs):
+            now = time.perf_counter() - t0
+            while nxt < len(jobs) and arrivals[nxt] <= now:
+                j = jobs[nxt]
+                jid = _submit(eng, j)
+                arrived[jid] = arrivals[nxt]
+                kind[jid] = j["kind"]
+                nxt += 1
+            left = eng.run(chunk)
+            now = time.perf_counter() - t0
+            for jid in eng.poll():
+                done_at[jid] = now
+                results[jid] = eng.result(jid)
+            if left == 0 and nxt < len(jobs):
+                wait = arrivals[nxt] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+        wall = time.perf_counter() - t0
+        st = eng.stats()
+    finally:
+        eng.close()
+    lat = {k: [] for k in ("3body", "cluster")}
+    allat = []
+    for jid, t in done_at.items():
+        d = t - arrived[jid]
+        lat[kind[jid]].append(d)
+        allat.append(d)
+    return dict(jobs=len(jobs), rate_req_s=rate, offered_s=float(arrivals[-1]), wall_s=wall,
+                throughput_req_s=len(jobs) / wall, pairs_per_s=st["pairs"] / wall,
+                device_busy=st["device_seconds"] / wall,
+                p50_s=_pct(allat, 50), p99_s=_pct(allat, 99),
+                p50_3body_s=_pct(lat["3body"], 50), p99_3body_s=_pct(lat["3body"], 99),
+                p50_cluster_s=_pct(lat["cluster"], 50), p99_cluster_s=_pct(lat["cluster"], 99),
+                failed=sum(r["failed"] for r in results.values()))

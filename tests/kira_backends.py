@@ -1,0 +1,83 @@
+"""Force backends for the block-step driver used by the config-1 tests.
+
+HostBackend   -- kira's host path (config 0): every j predicted on the host
+                 with predict_loworder (include/_dyn_.h:290-295), FP64 direct
+                 sum (the oracle).  This is the all-CPU reference run.
+CheckedG6     -- libg6 through the ABI, plus a host mirror of j-memory; after
+                 each force call the oracle recomputes the same block from the
+                 same inputs (GRAPE-form prediction, bitwise the device's) and
+                 the per-block errors are recorded.
+"""
+import numpy as np
+
+import oracle
+from starlab_amd.blockstep import ONE3, G6Backend
+
+
+class HostBackend:
+    def open(self, n):
+        self.n = n
+        self.x = np.zeros((n, 3)); self.v = np.zeros((n, 3))
+        self.a = np.zeros((n, 3)); self.j = np.zeros((n, 3))
+        self.t = np.zeros(n); self.m = np.zeros(n)
+        self.idx = np.arange(n, dtype=np.int32)
+
+    def send_j(self, idx, tj, dtj, m, x, v, a, j, k18):
+        self.t[idx] = tj; self.m[idx] = m; self.x[idx] = x; self.v[idx] = v
+        self.a[idx] = a; self.j[idx] = j
+
+    def force(self, nj, t, idx, xi, vi, eps2):
+        dt = (t - self.t)[:, None]
+        dt3 = dt * ONE3
+        dt2 = dt * 0.5
+        xp = ((self.j * dt3 + self.a) * dt2 + self.v) * dt + self.x
+        vp = (self.j * dt2 + self.a) * dt + self.v
+        acc, jerk, pot, nn, _ = oracle.force(xi, vi, idx, xp, vp, self.m, self.idx, eps2)
+        return acc, jerk, pot, nn
+
+    def close(self):
+        pass
+
+
+class CheckedG6(G6Backend):
+    def __init__(self, g):
+        super().__init__(g)
+        self.worst = dict(acc_norm=0.0, jerk_norm=0.0, pot_rel=0.0)
+        self.acc_errs = []
+        self.nn_mismatch = 0
+        self.checked_blocks = 0
+
+    def open(self, n):
+        super().open(n)
+        self.mir = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), a2=np.zeros((n, 3)),
+                        j6=np.zeros((n, 3)), k18=np.zeros((n, 3)), t=np.zeros(n), m=np.zeros(n))
+        self.idx = np.arange(n, dtype=np.int32)
+
+    def send_j(self, idx, tj, dtj, m, x, v, a, j, k18):
+        super().send_j(idx, tj, dtj, m, x, v, a, j, k18)
+        M = self.mir
+        M["t"][idx] = tj; M["m"][idx] = m; M["x"][idx] = x; M["v"][idx] = v
+        M["a2"][idx] = a / 2.0; M["j6"][idx] = j / 6.0; M["k18"][idx] = k18
+
+    def force(self, nj, t, idx, xi, vi, eps2):
+        acc, jerk, pot, nn = super().force(nj, t, idx, xi, vi, eps2)
+        M = self.mir
+        xp, vp = oracle.predict(M["x"], M["v"], M["a2"], M["j6"], M["k18"], M["t"], t, 4)
+        racc, rjerk, rpot, rnn, _ = oracle.force(xi, vi, idx, xp, vp, M["m"], self.idx, eps2)
+        sa = np.zeros(len(idx)); sj = np.zeros(len(idx))
+        for k, i in enumerate(idx):
+            d = xp - xi[k]; w = vp - vi[k]
+            r2 = (d * d).sum(1) + eps2
+            r2[i] = np.inf
+            sa[k] = (M["m"] / r2).sum()
+            sj[k] = (M["m"] * 4.0 * np.sqrt((w * w).sum(1)) / (r2 * np.sqrt(r2))).sum()
+        ea = np.linalg.norm(acc - racc, axis=1) / sa
+        self.acc_errs.append(ea)
+        self.worst["acc_norm"] = max(self.worst["acc_norm"], float(ea.max()))
+        self.worst["jerk_norm"] = max(self.worst["jerk_norm"],
+                                      float((np.linalg.norm(jerk - rjerk, axis=1) / sj).max()))
+        self.worst["pot_rel"] = max(self.worst["pot_rel"],
+                                    float((np.abs(pot - rpot) / np.abs(rpot)).max()))
+        self.nn_mismatch += int((nn != rnn).sum())
+        self.checked_blocks += 1
+        return acc, jerk, pot, nn

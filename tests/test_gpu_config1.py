@@ -1,0 +1,61 @@
+"""Config 1: plain kira-style integration of a Plummer sphere (N = 1024,
+seed 42 as in the reference's init Testfile, equal masses, N-body units,
+eps = 0, eta = 0.1) from t = 0 to t = 1 with individual power-of-two block
+steps, forces from libg6 through the GRAPE-6 ABI.
+
+* every block step: libg6's acc/jerk/pot agree with the FP64 oracle on the
+  same inputs and the nearest-neighbour index is exact.  Bars: normwise acc
+  error (|da| / sum_j |a_ij|) <= 1e-6 for 99.9 % of particle steps and
+  <= 2e-6 for every one (a single dominant pair term carries up to ~1.1e-6
+  from FP32 rounding of r^2, rsqrt and r^-3); jerk <= 2e-6; pot <= 1e-6;
+* the energy error |dE/E| at t = 1 stays within a factor 10 (+1e-8) of the
+  all-CPU FP64 run (kira's host force path) from the same initial state.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from starlab_amd.blockstep import BlockHermite
+from tests.kira_backends import CheckedG6, HostBackend
+
+pytestmark = pytest.mark.gpu
+
+
+def _energy(b):
+    return b.energy(lambda m, x, v, e: oracle.energies(m, x, v, e))
+
+
+def test_config1_plummer_1024_to_t1():
+    from starlab_amd.g6 import GRAPE6
+    m, x, v = oracle.makeplummer(1024, 42)
+
+    cpu = BlockHermite(m, x, v, eta=0.1, eps2=0.0)
+    e0 = _energy(cpu)
+    t0 = time.perf_counter()
+    cpu.run(HostBackend(), 1.0)
+    t_cpu = time.perf_counter() - t0
+    de_cpu = abs((_energy(cpu) - e0) / e0)
+
+    gpu = BlockHermite(m, x, v, eta=0.1, eps2=0.0)
+    be = CheckedG6(GRAPE6())
+    t0 = time.perf_counter()
+    gpu.run(be, 1.0)
+    t_gpu = time.perf_counter() - t0
+    de_gpu = abs((_energy(gpu) - e0) / e0)
+    print(f"cpu: blocks {cpu.blocks} steps {cpu.particle_steps} dE/E {de_cpu:.3e} {t_cpu:.1f}s",
+          flush=True)
+    print(f"gpu: blocks {gpu.blocks} steps {gpu.particle_steps} dE/E {de_gpu:.3e} {t_gpu:.1f}s "
+          f"checked {be.checked_blocks} worst {be.worst} nn_mismatch {be.nn_mismatch}", flush=True)
+    assert np.all(gpu.t == 1.0)
+    assert be.checked_blocks == gpu.blocks + 1
+    assert be.nn_mismatch == 0
+    errs = np.concatenate(be.acc_errs)
+    print(f"acc normwise error: median {np.median(errs):.2e} p99.9 "
+          f"{np.percentile(errs, 99.9):.2e} max {errs.max():.2e}", flush=True)
+    assert np.percentile(errs, 99.9) <= 1e-6
+    assert be.worst["acc_norm"] <= 2e-6
+    assert be.worst["jerk_norm"] <= 2e-6
+    assert be.worst["pot_rel"] <= 1e-6
+    assert de_gpu <= 10 * de_cpu + 1e-8
