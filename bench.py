@@ -122,6 +122,7 @@ def main():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ens", action="store_true", help="skip the ensemble / serving legs")
+    ap.add_argument("--no-tune", action="store_true", help="skip the force-shape autotuner")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "libg6 drives one GPU per process; this build measures "
@@ -140,6 +141,17 @@ def main():
     n = a.n
     log(f"bench: generating Plummer N={n}")
     inp = make_input(g, n)
+
+    tune = None
+    if not a.no_tune:
+        log("bench: autotuning force shapes")
+        t0 = time.perf_counter()
+        nb = g.autotune(n, n, EPS2)
+        tab = g.tuning()
+        full = tab.get(max(0, (n - 1).bit_length()))
+        tune = {"seconds": round(time.perf_counter() - t0, 3), "buckets": nb,
+                "full_sweep_shape": dict(zip(("variant", "tile", "waves", "lanes"), full))
+                if full else None}
 
     for _ in range(a.warmup):
         assert g.self_sweep(n, 0, n, 0.0, EPS2) == 0
@@ -162,7 +174,7 @@ def main():
     force_ms = kms / max(kcalls, 1)
     achieved = FLOP_PER_PAIR * pairs_per_step / (force_ms * 1e-3) / 1e12
     breakdown = {}
-    for k in ("predict", "force", "reduce", "refine"):
+    for k in ("predict", "force", "reduce"):
         c, ms = g.kernel_time(k)
         breakdown[k] = round(ms / max(c, 1), 4)
     wg, nsplit, jchunk = g.geometry()
@@ -192,8 +204,10 @@ def main():
                      "kernel": "k_force", "kernel_ms": force_ms,
                      "flop_per_pair": FLOP_PER_PAIR},
         "kernel_ms": breakdown,
-        "force_geometry": {"workgroups": wg, "nsplit": nsplit, "jchunk": jchunk},
+        "force_geometry": {"workgroups": wg, "nsplit": nsplit, "jchunk": jchunk,
+                           "tile": g.option("last_tile")},
         "nn_refinements": refinements,
+        "autotune": tune,
     }
     g.close()
     if not a.no_ens:

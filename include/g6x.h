@@ -50,12 +50,16 @@ int g6x_sync(int cluster_id);
  *   "force_variant"  -1 auto, 0 scalar-load, 1 LDS-staged     G6_FORCE_VARIANT
  *   "lds_min_ni"     auto picks the LDS kernel from this ni  G6_LDS_MIN_NI
  *   "xcd_remap"      1 = keep each j-split on one XCD         G6_XCD_REMAP
+ *   "tile"           j per FP32 tile: 0 auto, 64, 128, 256    G6_TILE
+ *   "last_tile"      (read only) tile of the last force launch
+ *   "reduce_lanes"   lanes per i in the split reduction: 0 auto, 4..64
  * Returns 0, or -1 for an unknown key. */
 int       g6x_set_option(const char *key, long long value);
 long long g6x_get_option(const char *key);
 
 /* Per-kernel event timing, accumulated since the last reset (needs
- * "timing" = 1).  kernel: 0 predictor, 1 force, 2 reduce, 3 nn refine,
+ * "timing" = 1).  kernel: 0 predictor, 1 force, 2 reduce (includes the
+ * FP64 nearest-neighbour rescan of ambiguous i), 3 unused (0 launches),
  * 4 neighbour list, 5 j scatter.  Returns launches; *ms = total ms. */
 int  g6x_kernel_time(int kernel, double *ms);
 void g6x_reset_timing(void);
@@ -66,6 +70,23 @@ int  g6x_last_force_geometry(int *workgroups, int *nsplit, int *jchunk);
 
 /* Number of nearest-neighbour refinements (FP64 rescans) in the last call. */
 int  g6x_last_refinements(void);
+
+/* Autotuner.  For each i-block size bucket (bucket b: 2^(b-1) < ni <= 2^b)
+ * up to ni_max, times force + reduction for every candidate shape (kernel
+ * variant, j-tile length 64/128/256, waves per launch 2048..32768, then the
+ * reduction's lanes per i) against the first nj j-particles as loaded, and
+ * keeps the fastest.  Later force calls of that size use the tuned shape
+ * unless an explicit option overrides it.  Takes ~1 s at nj = 65536.
+ * Returns the number of buckets tuned, or -1. */
+int  g6x_autotune(int cluster_id, int nj, int ni_max, double eps2);
+/* Tuned shape of one bucket; returns 1 if tuned, 0 if not, -1 bad bucket. */
+int  g6x_get_tuning(int bucket, int *variant, int *tile, int *waves, int *lanes);
+int  g6x_clear_tuning(void);
+/* Tuning table as text ("bucket variant tile waves lanes" per line).  The
+ * library also loads the file named by G6_TUNE_FILE at g6_open_.
+ * save returns 0 / -1; load returns the entries read or -1. */
+int  g6x_save_tuning(const char *path);
+int  g6x_load_tuning(const char *path);
 
 /* Text of the last error (empty if none). */
 const char *g6x_last_error(void);

@@ -48,6 +48,11 @@ G6X = {
     "g6x_reset_timing": (None, []),
     "g6x_last_force_geometry": (_i, [_pi, _pi, _pi]),
     "g6x_last_refinements": (_i, []),
+    "g6x_autotune": (_i, [_i, _i, _i, C.c_double]),
+    "g6x_get_tuning": (_i, [_i, _pi, _pi, _pi, _pi]),
+    "g6x_clear_tuning": (_i, []),
+    "g6x_save_tuning": (_i, [C.c_char_p]),
+    "g6x_load_tuning": (_i, [C.c_char_p]),
     "g6x_last_error": (C.c_char_p, []),
 }
 

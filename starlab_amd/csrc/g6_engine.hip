@@ -40,12 +40,28 @@ struct Options {
     int device = 0;
     int xcd_remap = 1;
     int force_variant = -1;   // -1 auto, 0 scalar-load kernel, 1 LDS kernel
+    int tile = 0;             // j per FP32 tile of the scalar-load kernel: 0 auto, 64/128/256
+    int reduce_lanes = 0;     // lanes per i in the split reduction: 0 auto, 4..64
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every
                               // size so far, so off unless asked for)
 };
 
 struct TimedPair { int kernel; hipEvent_t a, b; };
+
+// Force geometry chosen per i-block size bucket: bucket b holds ni in
+// (2^(b-1), 2^b].  waves = 0 marks an untuned bucket.
+constexpr int kBuckets = 21;
+struct Shape {
+    int variant = 0, tile = 0, waves = 0, lanes = 0;
+};
+
+int bucket_of(int ni)
+{
+    int b = 0;
+    while (b < kBuckets - 1 && (1 << b) < ni) ++b;
+    return b;
+}
 
 struct Engine {
     Options opt;
@@ -72,9 +88,9 @@ struct Engine {
     double *d_vp64 = nullptr;   // [3][dcap]
     int *d_jindex = nullptr;
 
-    int scap = 0;               // scatter staging capacity
-    JRaw *h_stage = nullptr; int *h_stage_addr = nullptr; int *h_stage_idx = nullptr;
-    JRaw *d_stage = nullptr; int *d_stage_addr = nullptr; int *d_stage_idx = nullptr;
+    int scap = 0;               // scatter staging capacity (records)
+    char *h_stage = nullptr;    // pinned: n records, n addresses, n index fields
+    char *d_stage = nullptr;
 
     bool pred_valid = false;
     double pred_ti = 0;
@@ -93,10 +109,8 @@ struct Engine {
 
     // ---- results ----------------------------------------------------------
     int rcap = 0;
-    char *d_res = nullptr;      // acc[3n] jerk[3n] pot[n] nn[n]
+    char *d_res = nullptr;      // acc[3n] jerk[3n] pot[n] nn[n] flags[n]
     char *h_res = nullptr;      // pinned
-    int *d_refine = nullptr;    // [0] = count, [1..] = list
-    int *h_refine_count = nullptr;
 
     // ---- neighbour lists --------------------------------------------------
     size_t nbcap_entries = 0;
@@ -113,7 +127,11 @@ struct Engine {
     int last_refinements = 0;
     std::vector<double> last_h2;
     int last_include_self = 1;
-    int geom_wg = 0, geom_nsplit = 0, geom_jchunk = 0, geom_variant = 0;
+    int geom_wg = 0, geom_nsplit = 0, geom_jchunk = 0, geom_variant = 0, geom_tile = 0;
+    int geom_lanes = 0;
+    Shape tuned[kBuckets];
+    bool forcing_shape = false;   // the autotuner's candidate overrides everything
+    Shape forced;
 
     // ---- timing -------------------------------------------------------------
     std::vector<hipEvent_t> ev_pool;
@@ -166,6 +184,7 @@ void read_env()
     o.xcd_remap = (int)env_ll("G6_XCD_REMAP", o.xcd_remap);
     o.force_variant = (int)env_ll("G6_FORCE_VARIANT", o.force_variant);
     o.lds_min_ni = (int)env_ll("G6_LDS_MIN_NI", o.lds_min_ni);
+    o.tile = (int)env_ll("G6_TILE", o.tile);
     if (o.npipes < 1) o.npipes = 1;
     if (o.nb_cap < 1) o.nb_cap = 1;
 }
@@ -287,23 +306,14 @@ int ensure_stage(int n)
 {
     if (n <= E.scap) return 0;
     int cap = std::max(n, std::max(256, E.scap * 2));
-    hfree(E.h_stage); hfree(E.h_stage_addr); hfree(E.h_stage_idx);
-    dfree(E.d_stage); dfree(E.d_stage_addr); dfree(E.d_stage_idx);
+    hfree(E.h_stage);
+    dfree(E.d_stage);
     E.scap = 0;
-    HIPCHK(hipHostMalloc((void **)&E.h_stage, sizeof(JRaw) * (size_t)cap, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&E.h_stage_addr, sizeof(int) * (size_t)cap, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&E.h_stage_idx, sizeof(int) * (size_t)cap, hipHostMallocDefault));
-    HIPCHK(hipMalloc((void **)&E.d_stage, sizeof(JRaw) * (size_t)cap));
-    HIPCHK(hipMalloc((void **)&E.d_stage_addr, sizeof(int) * (size_t)cap));
-    HIPCHK(hipMalloc((void **)&E.d_stage_idx, sizeof(int) * (size_t)cap));
+    const size_t bytes = (sizeof(JRaw) + 2 * sizeof(int)) * (size_t)cap;
+    HIPCHK(hipHostMalloc((void **)&E.h_stage, bytes, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void **)&E.d_stage, bytes));
     E.scap = cap;
     return 0;
-}
-
-__global__ void k_scatter_index(int *jindex, const int *addr, const int *idx, int n)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) jindex[addr[k]] = idx[k];
 }
 
 // Push dirty j records to HBM: the whole used range when many changed,
@@ -327,23 +337,20 @@ int flush_j(int nj)
         if (ensure_stage(nd)) return -1;
         // the staging buffers may still be read by an earlier copy
         HIPCHK(hipStreamSynchronize(E.stream));
+        // one packed upload: records, then addresses, then index fields
+        JRaw *rec = reinterpret_cast<JRaw *>(E.h_stage);
+        int *addr = reinterpret_cast<int *>(rec + nd);
         for (int k = 0; k < nd; ++k) {
             const int a = E.dirty[k];
-            E.h_stage[k] = E.h_jraw[a];
-            E.h_stage_addr[k] = a;
-            E.h_stage_idx[k] = E.h_jindex[a];
+            rec[k] = E.h_jraw[a];
+            addr[k] = a;
+            addr[nd + k] = E.h_jindex[a];
         }
-        HIPCHK(hipMemcpyAsync(E.d_stage, E.h_stage, sizeof(JRaw) * (size_t)nd,
-                              hipMemcpyHostToDevice, E.stream));
-        HIPCHK(hipMemcpyAsync(E.d_stage_addr, E.h_stage_addr, sizeof(int) * (size_t)nd,
-                              hipMemcpyHostToDevice, E.stream));
-        HIPCHK(hipMemcpyAsync(E.d_stage_idx, E.h_stage_idx, sizeof(int) * (size_t)nd,
+        HIPCHK(hipMemcpyAsync(E.d_stage, E.h_stage, (sizeof(JRaw) + 2 * sizeof(int)) * (size_t)nd,
                               hipMemcpyHostToDevice, E.stream));
         KTimer t(K_SCATTER);
-        HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_stage, E.d_stage_addr, nd));
-        hipLaunchKernelGGL(k_scatter_index, dim3((nd + 255) / 256), dim3(256), 0,
-                           E.stream, E.d_jindex, E.d_stage_addr, E.d_stage_idx, nd);
-        HIPCHK(hipGetLastError());
+        HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
+                              reinterpret_cast<const JRaw *>(E.d_stage), nd));
         t.stop();
     }
     for (int a : E.dirty) E.dirty_flag[a] = 0;
@@ -371,22 +378,23 @@ int predict(int nj, bool force)
 // i-block buffer: 10 four-byte arrays then 4 double arrays (x64 y64 z64 h2).
 size_t ibuf_bytes(int cap) { return (size_t)cap * (10 * 4 + 4 * 8); }
 
+// results: 7 doubles, nn and the refinement flag per i
+size_t results_bytes(int ni) { return (size_t)ni * (7 * sizeof(double) + 2 * sizeof(int)); }
+
 int ensure_i(int ni)
 {
     if (ni <= E.icap) return 0;
     int cap = std::max((ni + 63) & ~63, 256);
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
-    hfree(E.h_res); dfree(E.d_res); dfree(E.d_refine); hfree(E.h_refine_count);
+    hfree(E.h_res); dfree(E.d_res);
     E.icap = 0;
     HIPCHK(hipHostMalloc((void **)&E.h_ibuf, ibuf_bytes(cap), hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E.d_ibuf, ibuf_bytes(cap)));
     HIPCHK(hipHostMalloc((void **)&E.h_slots, sizeof(int) * (size_t)cap, hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E.d_slots, sizeof(int) * (size_t)cap));
-    const size_t rb = (size_t)cap * (7 * sizeof(double) + sizeof(int));
+    const size_t rb = results_bytes(cap);
     HIPCHK(hipHostMalloc((void **)&E.h_res, rb, hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E.d_res, rb));
-    HIPCHK(hipMalloc((void **)&E.d_refine, sizeof(int) * ((size_t)cap + 1)));
-    HIPCHK(hipHostMalloc((void **)&E.h_refine_count, sizeof(int), hipHostMallocDefault));
     E.icap = cap;
     return 0;
 }
@@ -418,12 +426,16 @@ Results results_at(char *base, int ni)
     r.jerk = d + 3 * (size_t)ni;
     r.pot = d + 6 * (size_t)ni;
     r.nn = reinterpret_cast<int *>(d + 7 * (size_t)ni);
-    r.refine_count = E.d_refine;
-    r.refine_list = E.d_refine + 1;
+    r.flags = r.nn + ni;
     return r;
 }
 
-size_t results_bytes(int ni) { return (size_t)ni * (7 * sizeof(double) + sizeof(int)); }
+int count_refined(const Results &h, int ni)
+{
+    int c = 0;
+    for (int k = 0; k < ni; ++k) c += h.flags[k];
+    return c;
+}
 
 int ensure_partials(size_t bytes)
 {
@@ -452,60 +464,75 @@ Partials partials_at(int ld, int nsplit)
 
 size_t partials_bytes(int ld, int nsplit) { return (size_t)ld * nsplit * (7 * 8 + 3 * 4); }
 
-int force_variant(int ni)
-{
-    if (E.opt.force_variant == 0 || E.opt.force_variant == 1) return E.opt.force_variant;
-    return ni >= E.opt.lds_min_ni ? 1 : 0;
-}
+struct Geometry {
+    int variant, tile, nsplit, jchunk, lanes;
+};
 
-// j-split: enough waves to fill the chip, whole tiles per split.
-void choose_split(int ni, int nj, int variant, int &nsplit, int &jchunk)
+// Force geometry: kernel variant, j-tile length (FP32 partial-sum span),
+// j-split and the reduction's lanes per i.  Precedence: the autotuner's
+// candidate while it measures, explicit options, the tuned table for this
+// i-block size, then the built-in rule (enough waves to fill the chip; short
+// tiles when a small i-block must spread its j over many waves).
+Geometry choose_geometry(int ni, int nj)
 {
-    const int per_wave = force_igroup(variant) / (variant == 1 ? 4 : 1);
+    Shape sh;
+    const Shape &t = E.tuned[bucket_of(ni)];
+    if (E.forcing_shape) {
+        sh = E.forced;
+    } else {
+        if (t.waves > 0) sh = t;
+        if (E.opt.force_variant == 0 || E.opt.force_variant == 1) sh.variant = E.opt.force_variant;
+        else if (t.waves == 0) sh.variant = ni >= E.opt.lds_min_ni ? 1 : 0;
+        if (E.opt.tile == 64 || E.opt.tile == 128 || E.opt.tile == kTile) sh.tile = E.opt.tile;
+        if (E.opt.reduce_lanes > 0) sh.lanes = E.opt.reduce_lanes;
+        if (t.waves == 0 || E.opt.waves_target != Options().waves_target)
+            sh.waves = E.opt.waves_target;
+    }
+    Geometry g;
+    g.variant = sh.variant;
+    const int per_wave = force_igroup(g.variant) / (g.variant == 1 ? 4 : 1);
     const int n_iw = std::max(1, (ni + per_wave - 1) / per_wave);
-    const int ntiles = std::max(1, (nj + kTile - 1) / kTile);
-    int s = E.opt.nsplit > 0 ? E.opt.nsplit
-                              : std::max(1, (E.opt.waves_target + n_iw - 1) / n_iw);
-    s = std::min(s, ntiles);
-    const int tiles_per = (ntiles + s - 1) / s;
-    jchunk = tiles_per * kTile;
-    nsplit = (nj + jchunk - 1) / jchunk;
-    if (nsplit < 1) nsplit = 1;
+    const int target = std::max(1, sh.waves);
+    if (g.variant == 1) g.tile = kTile;
+    else if (sh.tile == 64 || sh.tile == 128 || sh.tile == kTile) g.tile = sh.tile;
+    else g.tile = (long long)n_iw * ((nj + kTile - 1) / kTile) >= target ? kTile : 64;
+    const int ntiles = std::max(1, (nj + g.tile - 1) / g.tile);
+    int sp = (!E.forcing_shape && E.opt.nsplit > 0) ? E.opt.nsplit
+                                                    : std::max(1, (target + n_iw - 1) / n_iw);
+    sp = std::min(sp, ntiles);
+    const int tiles_per = (ntiles + sp - 1) / sp;
+    g.jchunk = tiles_per * g.tile;
+    g.nsplit = std::max(1, (nj + g.jchunk - 1) / g.jchunk);
+    g.lanes = sh.lanes > 0 ? sh.lanes : reduce_lanes(g.nsplit);
+    return g;
 }
 
-// Enqueue force + reduction + refinement for an i-block already in d_ibuf.
+// Enqueue force + reduction (with the nearest-neighbour refinement) for an
+// i-block already in d_ibuf.
 int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r)
 {
-    int nsplit, jchunk;
-    const int variant = force_variant(ni);
-    choose_split(ni, nj, variant, nsplit, jchunk);
+    const Geometry g = choose_geometry(ni, nj);
     const int ld = ib.ld;
-    if (ensure_partials(partials_bytes(ld, nsplit))) return -1;
-    Partials p = partials_at(ld, nsplit);
-    E.geom_nsplit = nsplit;
-    E.geom_jchunk = jchunk;
-    E.geom_wg = ((ni + force_igroup(variant) - 1) / force_igroup(variant)) * nsplit;
-    E.geom_variant = variant;
-    HIPCHK(hipMemsetAsync(E.d_refine, 0, sizeof(int), E.stream));
+    if (ensure_partials(partials_bytes(ld, g.nsplit))) return -1;
+    Partials p = partials_at(ld, g.nsplit);
+    E.geom_nsplit = g.nsplit;
+    E.geom_jchunk = g.jchunk;
+    E.geom_tile = g.tile;
+    E.geom_lanes = g.lanes;
+    E.geom_wg = ((ni + force_igroup(g.variant) - 1) / force_igroup(g.variant)) * g.nsplit;
+    E.geom_variant = g.variant;
     {
         KTimer t(K_FORCE);
-        HIPCHK(launch_force(E.stream, E.d_jpred, nj, ib, ni, eps2, nsplit, jchunk, p,
-                            E.opt.xcd_remap != 0, variant));
+        HIPCHK(launch_force(E.stream, E.d_jpred, nj, ib, ni, eps2, g.nsplit, g.jchunk, g.tile,
+                            p, E.opt.xcd_remap != 0, g.variant));
         t.stop();
     }
     {
         KTimer t(K_REDUCE);
-        HIPCHK(launch_reduce(E.stream, p, nsplit, ni, E.d_jpred, nj, ib, eps2,
-                             E.d_jindex, r));
+        HIPCHK(launch_reduce(E.stream, p, g.nsplit, ni, E.d_jpred, nj, ib, eps2, g.tile,
+                             E.d_xp64, E.dcap, E.d_jindex, r, g.lanes));
         t.stop();
     }
-    {
-        KTimer t(K_REFINE);
-        HIPCHK(launch_refine(E.stream, E.d_xp64, E.dcap, E.d_jindex, nj, ib, ni, r));
-        t.stop();
-    }
-    HIPCHK(hipMemcpyAsync(E.h_refine_count, E.d_refine, sizeof(int),
-                          hipMemcpyDeviceToHost, E.stream));
     return 0;
 }
 
@@ -514,6 +541,8 @@ int ensure_open(const char *who)
     if (!E.open) return fail(-1, "%s: device not open (call g6_open_ first)", who);
     return 0;
 }
+
+int load_tuning(const char *path);
 
 int do_open(int dev)
 {
@@ -527,6 +556,9 @@ int do_open(int dev)
     if (dev < 0 || dev >= n) dev = 0;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreate(&E.stream));
+    if (const char *tf = getenv("G6_TUNE_FILE")) {
+        if (*tf && load_tuning(tf) < 0) E.last_error.clear();   // a missing file is not an error
+    }
     HIPCHK(hipEventCreateWithFlags(&E.ev_done, hipEventDisableTiming));
     E.device = dev;
     E.open = true;
@@ -546,11 +578,11 @@ int do_close()
     (void)hipStreamSynchronize(E.stream);
     harvest_timing();
     free_device_j();
-    dfree(E.d_stage); dfree(E.d_stage_addr); dfree(E.d_stage_idx);
-    hfree(E.h_stage); hfree(E.h_stage_addr); hfree(E.h_stage_idx);
+    dfree(E.d_stage);
+    hfree(E.h_stage);
     E.scap = 0;
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
-    hfree(E.h_res); dfree(E.d_res); dfree(E.d_refine); hfree(E.h_refine_count);
+    hfree(E.h_res); dfree(E.d_res);
     E.icap = 0;
     dfree(E.d_part); E.pbytes = 0;
     dfree(E.d_nbcount); dfree(E.d_nblist); hfree(E.h_nbcount); hfree(E.h_nblist);
@@ -660,9 +692,9 @@ int lasthalf(int ni, double (*acc)[3], double (*jerk)[3], double *pot, int *nn)
     HIPCHK(hipEventSynchronize(E.ev_done));
     HIPCHK(hipGetLastError());
     harvest_timing();
-    E.last_refinements = *E.h_refine_count;
     const int n = std::min(ni, E.last_ni);
     Results h = results_at(E.h_res, E.last_ni);
+    E.last_refinements = count_refined(h, E.last_ni);
     memcpy(acc, h.acc, sizeof(double) * 3 * (size_t)n);
     memcpy(jerk, h.jerk, sizeof(double) * 3 * (size_t)n);
     memcpy(pot, h.pot, sizeof(double) * (size_t)n);
@@ -765,6 +797,133 @@ int get_nl(int pipe, int max_length, int *nblen, int *nbl)
     }
     *nblen = c;
     return c > E.opt.nb_cap ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: for each i-block size bucket, time force + reduction (HIP events
+// on the library stream) for every candidate shape against the j-memory as
+// loaded, and keep the fastest.  The i-block is the first ni j-particles.
+// ---------------------------------------------------------------------------
+int self_block(int nj, int i_first, int ni, IBlock &ib)
+{
+    if (ensure_i(ni)) return -1;
+    const int ld = (ni + 63) & ~63;
+    HIPCHK(launch_iprep_self(E.stream, E.d_jpred, E.d_xp64, E.dcap, i_first, ni,
+                             reinterpret_cast<float *>(E.d_ibuf), ld));
+    ib = iblock_at(E.d_ibuf, ld);
+    ib.x64 = E.d_xp64 + i_first;
+    ib.y64 = E.d_xp64 + (size_t)E.dcap + i_first;
+    ib.z64 = E.d_xp64 + 2 * (size_t)E.dcap + i_first;
+    return 0;
+}
+
+// Mean ms of force + reduction for one shape (after one untimed launch).
+double time_shape(const Shape &sh, int nj, int ni, const IBlock &ib, float eps2, int reps)
+{
+    E.forcing_shape = true;
+    E.forced = sh;
+    Results r = results_at(E.d_res, ni);
+    hipEvent_t a = nullptr, b = nullptr;
+    double ms_per = -1;
+    if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess &&
+        enqueue_force(ni, nj, eps2, ib, r) == 0 &&
+        hipEventRecord(a, E.stream) == hipSuccess) {
+        bool ok = true;
+        for (int k = 0; k < reps && ok; ++k) ok = enqueue_force(ni, nj, eps2, ib, r) == 0;
+        float ms = 0;
+        if (ok && hipEventRecord(b, E.stream) == hipSuccess &&
+            hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess)
+            ms_per = ms / reps;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    E.forcing_shape = false;
+    return ms_per;
+}
+
+int autotune(int nj, int ni_max, double eps2)
+{
+    if (ensure_open("g6x_autotune")) return -1;
+    if (E.calc_pending) return fail(-1, "g6x_autotune: a calculation is in flight");
+    if (nj <= 0 || nj > E.n_addr) return fail(-1, "g6x_autotune: nj = %d outside j-memory", nj);
+    ni_max = std::min(ni_max, nj);
+    if (flush_j(nj)) return -1;
+    if (predict(nj, true)) return -1;
+    const int saved_timing = E.opt.timing;
+    E.opt.timing = 0;
+    int tuned = 0;
+    for (int b = 0; b < kBuckets && (1 << b) <= 2 * ni_max - 1; ++b) {
+        const int ni = std::min(1 << b, ni_max);
+        IBlock ib;
+        if (self_block(nj, 0, ni, ib)) { E.opt.timing = saved_timing; return -1; }
+        // about 2 ms of work per candidate, at least 3 launches
+        const double est_ms = (double)ni * nj / 1.2e9;
+        const int reps = std::max(3, std::min(50, (int)(2.0 / std::max(est_ms, 1e-3))));
+        Shape best;
+        double best_ms = 1e30;
+        const int waves[] = {2048, 4096, 8192, 16384, 32768};
+        for (int variant = 0; variant < 2; ++variant) {
+            if (variant == 1 && ni < 512) continue;
+            for (int tile : {64, 128, kTile}) {
+                if (variant == 1 && tile != kTile) continue;
+                for (int w : waves) {
+                    Shape sh;
+                    sh.variant = variant; sh.tile = tile; sh.waves = w; sh.lanes = 0;
+                    const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
+                    if (ms < 0) { E.opt.timing = saved_timing; return fail(-1, "g6x_autotune: launch failed"); }
+                    if (ms < best_ms) { best_ms = ms; best = sh; }
+                }
+            }
+        }
+        // reduction lanes for the winning force shape
+        Shape lb = best;
+        for (int lanes : {4, 8, 16, 32, 64}) {
+            Shape sh = best;
+            sh.lanes = lanes;
+            const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
+            if (ms >= 0 && ms < best_ms) { best_ms = ms; lb = sh; }
+        }
+        E.tuned[b] = lb;
+        ++tuned;
+    }
+    E.opt.timing = saved_timing;
+    E.pred_valid = false;
+    return tuned;
+}
+
+int save_tuning(const char *path)
+{
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(-1, "g6x_save_tuning: cannot write %s", path);
+    fprintf(f, "# libg6 force shapes: bucket (ni <= 2^bucket) variant tile waves lanes\n");
+    for (int b = 0; b < kBuckets; ++b) {
+        const Shape &t = E.tuned[b];
+        if (t.waves > 0) fprintf(f, "%d %d %d %d %d\n", b, t.variant, t.tile, t.waves, t.lanes);
+    }
+    fclose(f);
+    return 0;
+}
+
+int load_tuning(const char *path)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(-1, "g6x_load_tuning: cannot read %s", path);
+    char line[256];
+    int n = 0;
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        int b, v, t, w, l;
+        if (sscanf(line, "%d %d %d %d %d", &b, &v, &t, &w, &l) != 5) continue;
+        if (b < 0 || b >= kBuckets || (v != 0 && v != 1) || w <= 0 ||
+            (t != 64 && t != 128 && t != kTile) || (v == 1 && t != kTile) ||
+            (l != 0 && l != 4 && l != 8 && l != 16 && l != 32 && l != 64))
+            continue;
+        Shape &s = E.tuned[b];
+        s.variant = v; s.tile = t; s.waves = w; s.lanes = l;
+        ++n;
+    }
+    fclose(f);
+    return n;
 }
 
 }  // namespace
@@ -888,14 +1047,8 @@ int g6x_self_sweep(int, int nj, int i_first, int ni, double ti, double eps2)
     E.ti = ti;
     if (flush_j(nj)) return -1;
     if (predict(nj, true)) return -1;
-    if (ensure_i(ni)) return -1;
-    const int ld = (ni + 63) & ~63;
-    HIPCHK(launch_iprep_self(E.stream, E.d_jpred, E.d_xp64, E.dcap, i_first, ni,
-                             reinterpret_cast<float *>(E.d_ibuf), ld));
-    IBlock ib = iblock_at(E.d_ibuf, ld);
-    ib.x64 = E.d_xp64 + i_first;
-    ib.y64 = E.d_xp64 + (size_t)E.dcap + i_first;
-    ib.z64 = E.d_xp64 + 2 * (size_t)E.dcap + i_first;
+    IBlock ib;
+    if (self_block(nj, i_first, ni, ib)) return -1;
     Results r = results_at(E.d_res, ni);
     if (enqueue_force(ni, nj, (float)eps2, ib, r)) return -1;
     E.last_ni = ni;
@@ -912,8 +1065,8 @@ int g6x_read_results(int, int ni, double *acc, double *jerk, double *pot, int *n
                           E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
     harvest_timing();
-    E.last_refinements = *E.h_refine_count;
     Results h = results_at(E.h_res, E.last_ni);
+    E.last_refinements = count_refined(h, E.last_ni);
     if (acc) memcpy(acc, h.acc, sizeof(double) * 3 * (size_t)ni);
     if (jerk) memcpy(jerk, h.jerk, sizeof(double) * 3 * (size_t)ni);
     if (pot) memcpy(pot, h.pot, sizeof(double) * (size_t)ni);
@@ -964,6 +1117,12 @@ int g6x_set_option(const char *key, long long value)
     else if (k == "xcd_remap") o.xcd_remap = (int)value;
     else if (k == "force_variant") o.force_variant = (int)value;
     else if (k == "lds_min_ni") o.lds_min_ni = (int)value;
+    else if (k == "tile") o.tile = (int)value;
+    else if (k == "reduce_lanes") {
+        if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32 && value != 64)
+            return -1;
+        o.reduce_lanes = (int)value;
+    }
     else return -1;
     E.pred_valid = false;
     return 0;
@@ -983,6 +1142,9 @@ long long g6x_get_option(const char *key)
     if (k == "xcd_remap") return o.xcd_remap;
     if (k == "force_variant") return o.force_variant;
     if (k == "lds_min_ni") return o.lds_min_ni;
+    if (k == "tile") return o.tile;
+    if (k == "last_tile") return E.geom_tile;
+    if (k == "reduce_lanes") return o.reduce_lanes;
     return -1;
 }
 
@@ -1007,6 +1169,33 @@ int g6x_last_force_geometry(int *workgroups, int *nsplit, int *jchunk)
 }
 
 int g6x_last_refinements(void) { return E.last_refinements; }
+
+int g6x_autotune(int, int nj, int ni_max, double eps2)
+{
+    read_env();
+    return autotune(nj, ni_max, eps2);
+}
+
+int g6x_get_tuning(int bucket, int *variant, int *tile, int *waves, int *lanes)
+{
+    if (bucket < 0 || bucket >= kBuckets) return -1;
+    const Shape &t = E.tuned[bucket];
+    if (variant) *variant = t.variant;
+    if (tile) *tile = t.tile;
+    if (waves) *waves = t.waves;
+    if (lanes) *lanes = t.lanes;
+    return t.waves > 0 ? 1 : 0;
+}
+
+int g6x_clear_tuning(void)
+{
+    for (Shape &t : E.tuned) t = Shape();
+    return 0;
+}
+
+int g6x_save_tuning(const char *path) { return save_tuning(path ? path : ""); }
+
+int g6x_load_tuning(const char *path) { return load_tuning(path ? path : ""); }
 
 const char *g6x_last_error(void) { return E.last_error.c_str(); }
 

@@ -49,20 +49,20 @@ struct IBlock {
 struct Partials {
     double *c[7];          // ax ay az jx jy jz pot
     float *best;           // smallest FP32 r^2 (+inf: none)
-    int *btile;            // first address of the 256-j tile holding it
+    int *btile;            // first address of the j-tile holding it
     float *second;         // runner-up FP32 r^2
     int ld;                // stride between splits (= IBlock ld)
 };
 
-// Final results (caller layout: double[ni][3] acc, jerk; pot[ni]; nn[ni]).
+// Final results (caller layout: double[ni][3] acc, jerk; pot[ni]; nn[ni]),
+// plus flags[ni] = 1 where the nearest neighbour needed the FP64 rescan.
 struct Results {
     double *acc, *jerk, *pot;
     int *nn;
-    int *refine_count;     // device counter of ambiguous nearest neighbours
-    int *refine_list;      // their i slots
+    int *flags;
 };
 
-constexpr int kTile = 256;           // j per FP32 partial-sum tile
+constexpr int kTile = 256;           // largest j per FP32 partial-sum tile
 // Nearest-neighbour ordering by FP32 r^2 is trusted only when the runner-up
 // exceeds the best by more than this relative margin (16 FP32 ulps; the
 // FP32 r^2 of a double-single difference is within ~4 ulps of the exact
@@ -71,8 +71,7 @@ constexpr float kNNBand = 1.9073486328125e-06f;   // 2^-19
 
 // Kernel launchers (g6_kernels.hip).  All enqueue on `s` and return the
 // launch status.
-hipError_t launch_scatter(hipStream_t s, JRaw *jraw, const JRaw *staged,
-                          const int *addr, int n);
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n);
 hipError_t launch_predict(hipStream_t s, const JRaw *jraw, int nj, double ti,
                           int order, JPred *jpred, double *xp64, double *vp64,
                           int cap);
@@ -84,13 +83,13 @@ hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
 int force_igroup(int variant);
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
-                        int jchunk, const Partials &p, bool xcd_remap, int variant);
+                        int jchunk, int tile, const Partials &p, bool xcd_remap, int variant);
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          const JPred *jpred, int nj, const IBlock &ib, float eps2,
-                         const int *jindex, const Results &r);
-hipError_t launch_refine(hipStream_t s, const double *xp64, int cap,
-                         const int *jindex, int nj, const IBlock &ib, int ni,
-                         const Results &r);
+                         int tile, const double *xp64, int cap, const int *jindex,
+                         const Results &r, int lanes);
+// lanes per i the reduction uses by default for nsplit splits (4..64)
+int reduce_lanes(int nsplit);
 hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,
                          const int *jindex, int nj, const IBlock &ib,
                          const double *h2, const int *slots, int nslots,

@@ -22,19 +22,23 @@ namespace g6 {
 // ---------------------------------------------------------------------------
 // j-memory update: copy staged records to their addresses.
 // ---------------------------------------------------------------------------
-__global__ void k_scatter(JRaw *__restrict__ jraw, const JRaw *__restrict__ staged,
-                          const int *__restrict__ addr, int n)
+// The staged buffer holds n records, then n addresses, then n index fields.
+__global__ void k_scatter(JRaw *__restrict__ jraw, int *__restrict__ jindex,
+                          const JRaw *__restrict__ staged, int n)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) jraw[addr[k]] = staged[k];
+    if (k >= n) return;
+    const int *addr = reinterpret_cast<const int *>(staged + n);
+    const int a = addr[k];
+    jraw[a] = staged[k];
+    jindex[a] = addr[n + k];
 }
 
-hipError_t launch_scatter(hipStream_t s, JRaw *jraw, const JRaw *staged,
-                          const int *addr, int n)
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s,
-                       jraw, staged, addr, n);
+                       jraw, jindex, staged, n);
     return hipGetLastError();
 }
 
@@ -137,7 +141,8 @@ hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
 // Force kernel.  One wave = 64 i-particles (one per lane) against one j-split;
 // j is wave-uniform, so its record comes through scalar loads into SGPRs and
 // every VALU instruction does 64 pair interactions.  j is processed in tiles
-// of kTile: FP32 partial sums per tile, folded into FP64 at the tile end.
+// of `tile` (64, 128 or 256) j: FP32 partial sums per tile, folded into FP64
+// at the tile end.
 //
 // Nearest neighbour: per lane, the smallest FP32 r^2 (v_min) and the
 // runner-up (v_med3 of best, runner-up, r^2) in each tile; at the tile end
@@ -230,7 +235,7 @@ __device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
 
 __global__ __launch_bounds__(64) void k_force(
     const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
-    int jchunk, int n_igroups, Partials P, int remap)
+    int jchunk, int tile, int n_igroups, Partials P, int remap)
 {
     int b = blockIdx.x;
     if (remap) {
@@ -258,8 +263,8 @@ __global__ __launch_bounds__(64) void k_force(
     float rbest = __builtin_inff(), rsecond = __builtin_inff();
     int rtile = -1;
 
-    for (int t = jbeg; t < jend; t += kTile) {
-        const int tn = min(kTile, jend - t);
+    for (int t = jbeg; t < jend; t += tile) {
+        const int tn = min(tile, jend - t);
         Acc32 a;
         a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.kx = a.ky = a.kz = a.p = 0.f;
         a.best = a.second = __builtin_inff();
@@ -506,7 +511,7 @@ int force_igroup(int variant) { return variant == 1 ? kLdsWG * kLdsI : 64; }
 
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
-                        int jchunk, const Partials &p, bool xcd_remap, int variant)
+                        int jchunk, int tile, const Partials &p, bool xcd_remap, int variant)
 {
     if (ni <= 0) return hipSuccess;
     const int n_igroups = (ni + force_igroup(variant) - 1) / force_igroup(variant);
@@ -521,148 +526,213 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                                ib, ni, eps2, jchunk, n_igroups, p, remap);
     } else {
         hipLaunchKernelGGL(k_force, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
-                           eps2, jchunk, n_igroups, p, remap);
+                           eps2, jchunk, tile, n_igroups, p, remap);
     }
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Reduction over j-splits (fixed split order: deterministic) into the
-// caller's layout.  The nearest neighbour is located inside its tile by
-// recomputing FP32 r^2 exactly as the force kernel did; when the runner-up is
-// within kNNBand of the best, the i is queued for the FP64 rescan.
+// Reduction over j-splits into the caller's layout.  A group of L = 2^LOG2L
+// lanes handles one i (64 / L i per wave): few splits use small groups, many
+// splits (small i-blocks) a whole wave.
+//
+// Lane l of a group sums splits l, l + L, ... in order; a butterfly of xor
+// shuffles inside the group then adds across lanes.  Every level adds the
+// same two values in both partner lanes (IEEE addition commutes), so all
+// lanes of the group end with one result and the sum depends only on the
+// split geometry: deterministic.
+//
+// Nearest neighbour: the (FP32 r^2, tile) minimum over splits, lowest tile
+// on ties, and the runner-up over everything else.  The address inside the
+// winning tile is found by recomputing FP32 r^2 exactly as the force kernel
+// did (first match in ascending address).  When the runner-up lies within
+// kNNBand of the best, or nothing matched, the group rescans all j in FP64
+// (d^2 evaluated as the FP64 CPU path, lexicographic (d^2, address) minimum)
+// and flags the i as refined.
+//
+// Every branch below depends only on values that are equal across the
+// group, and shuffles stay inside the group, so groups of one wave may
+// diverge freely.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_reduce(Partials P, int nsplit, int ni,
-                                                const JPred *__restrict__ jpred,
-                                                int nj, IBlock ib, float eps2,
-                                                const int *__restrict__ jindex,
-                                                Results R)
+template <int L>
+__device__ __forceinline__ double group_sum(double v)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni) return;
-    double A[7] = {0, 0, 0, 0, 0, 0, 0};
-    float best = __builtin_inff(), second = __builtin_inff();
-    int tile = -1;
-    for (int s = 0; s < nsplit; ++s) {
-        const size_t o = (size_t)s * P.ld + i;
 #pragma unroll
-        for (int c = 0; c < 7; ++c) A[c] += P.c[c][o];
-        const float b = P.best[o];
-        if (b < best) {
-            second = __builtin_fminf(best, P.second[o]);
-            best = b;
-            tile = P.btile[o];
-        } else {
-            second = __builtin_fminf(second, b);
-        }
-    }
-    R.acc[3 * i + 0] = A[0];
-    R.acc[3 * i + 1] = A[1];
-    R.acc[3 * i + 2] = A[2];
-    R.jerk[3 * i + 0] = A[3];
-    R.jerk[3 * i + 1] = A[4];
-    R.jerk[3 * i + 2] = A[5];
-    R.pot[i] = A[6];
-
-    int nn = 0;
-    bool ambiguous = tile < 0;
-    if (tile >= 0) {
-        const float xih = ib.xh[i], yih = ib.yh[i], zih = ib.zh[i];
-        const float xil = ib.xl[i], yil = ib.yl[i], zil = ib.zl[i];
-        const int excl = ib.excl[i];
-        const int tend = min(tile + kTile, nj);
-        const uint32_t bb = __float_as_uint(best);
-        int found = -1;
-        // batches of 8 independent record loads keep several in flight
-        for (int j0 = tile; j0 < tend && found < 0; j0 += 8) {
-            uint32_t r2b[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = min(j0 + u, tend - 1);
-                const JPred &J = jpred[j];
-                float dx, dy, dz;
-                r2b[u] = __float_as_uint(ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih,
-                                               zih, xil, yil, zil, eps2, dx, dy, dz));
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = j0 + u;
-                if (found < 0 && j < tend && j != excl && r2b[u] == bb) found = j;
-            }
-        }
-        if (found >= 0) nn = jindex[found];
-        else ambiguous = true;
-        if (!(second > best * (1.0f + kNNBand))) ambiguous = true;
-    }
-    R.nn[i] = nn;
-    if (ambiguous && nj > 1) {
-        const int f = atomicAdd(R.refine_count, 1);
-        R.refine_list[f] = i;
-    }
+    for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
 }
 
-hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
-                         const JPred *jpred, int nj, const IBlock &ib, float eps2,
-                         const int *jindex, const Results &r)
-{
-    if (ni <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3((ni + 255) / 256), dim3(256), 0, s, p,
-                       nsplit, ni, jpred, nj, ib, eps2, jindex, r);
-    return hipGetLastError();
-}
+constexpr int kReduceWaves = 4;   // waves per reduction workgroup
 
-// ---------------------------------------------------------------------------
-// FP64 nearest-neighbour rescan for flagged i (one workgroup per i).
-// d^2 = dx*dx + dy*dy + dz*dz evaluated exactly as the FP64 CPU path.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_refine(const double *__restrict__ xp64,
-                                                int cap, const int *__restrict__ jindex,
-                                                int nj, IBlock ib, Results R)
+template <int LOG2L>
+__global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
+    Partials P, int nsplit, int ni, const JPred *__restrict__ jpred, int nj, IBlock ib,
+    float eps2, int tile_len, const double *__restrict__ xp64, int cap,
+    const int *__restrict__ jindex, Results R)
 {
 #pragma clang fp contract(off)
-    __shared__ double sd[4];
-    __shared__ int sa[4];
-    const int cnt = *R.refine_count;
-    for (int f = blockIdx.x; f < cnt; f += gridDim.x) {
-        const int i = R.refine_list[f];
+    constexpr int L = 1 << LOG2L;
+    const int lane = threadIdx.x & 63;
+    const int l = lane & (L - 1);
+    const int i = (blockIdx.x * kReduceWaves + (threadIdx.x >> 6)) * (64 / L) + (lane >> LOG2L);
+    if (i >= ni) return;   // whole group
+
+    double A[7] = {0, 0, 0, 0, 0, 0, 0};
+    float best = __builtin_inff(), second = __builtin_inff();
+    int tile = INT_MAX;
+    // splits l, l + L, ... in order; loads issued four splits at a time
+    constexpr int U = 4;
+    for (int s0 = l; s0 < nsplit; s0 += U * L) {
+        double v[U][7];
+        float vb[U], vs[U];
+        int vt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int s = min(s0 + u * L, nsplit - 1);
+            const size_t o = (size_t)s * P.ld + i;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) v[u][c] = P.c[c][o];
+            vb[u] = P.best[o];
+            vs[u] = P.second[o];
+            vt[u] = P.btile[o];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (s0 + u * L >= nsplit) break;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) A[c] += v[u][c];
+            const float b = vb[u];
+            if (b < best) {
+                second = __builtin_fminf(best, vs[u]);
+                best = b;
+                tile = vt[u];
+            } else {
+                second = __builtin_fminf(second, b);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 7; ++c) A[c] = group_sum<L>(A[c]);
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off);
+        const float os = __shfl_xor(second, off);
+        const int ot = __shfl_xor(tile, off);
+        const bool take = ob < best || (ob == best && ot < tile);
+        const float loser = take ? best : ob;
+        second = __builtin_fminf(__builtin_fminf(second, os), loser);
+        best = take ? ob : best;
+        tile = take ? ot : tile;
+    }
+    if (l == 0) {
+        R.acc[3 * i + 0] = A[0];
+        R.acc[3 * i + 1] = A[1];
+        R.acc[3 * i + 2] = A[2];
+        R.jerk[3 * i + 0] = A[3];
+        R.jerk[3 * i + 1] = A[4];
+        R.jerk[3 * i + 2] = A[5];
+        R.pot[i] = A[6];
+    }
+
+    const int excl = ib.excl[i];
+    int nn = 0;
+    bool ambiguous = !(best < __builtin_inff()) || tile < 0 || tile == INT_MAX;
+    if (!ambiguous) {
+        const float xih = ib.xh[i], yih = ib.yh[i], zih = ib.zh[i];
+        const float xil = ib.xl[i], yil = ib.yl[i], zil = ib.zl[i];
+        const int tend = min(tile + tile_len, nj);
+        const uint32_t bb = __float_as_uint(best);
+        const int gshift = lane & ~(L - 1);
+        int found = INT_MAX;
+        for (int j0 = tile; j0 < tend; j0 += L) {
+            const int j = j0 + l;
+            bool hit = false;
+            if (j < tend) {
+                const JPred &J = jpred[j];
+                float dx, dy, dz;
+                const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih, xil,
+                                       yil, zil, eps2, dx, dy, dz);
+                hit = j != excl && __float_as_uint(r2) == bb;
+            }
+            unsigned long long m = __ballot(hit) >> gshift;
+            if (L < 64) m &= (1ull << L) - 1ull;
+            if (m) {
+                found = j0 + __ffsll((long long)m) - 1;
+                break;
+            }
+        }
+        if (found == INT_MAX) ambiguous = true;
+        else nn = jindex[found];
+        if (!(second > best * (1.0f + kNNBand))) ambiguous = true;
+    }
+    const bool refine = ambiguous && nj > 1;
+    if (refine) {
         const double xi = ib.x64[i], yi = ib.y64[i], zi = ib.z64[i];
-        const int excl = ib.excl[i];
-        double best = __builtin_inf();
+        double bd = __builtin_inf();
         int ba = INT_MAX;
-        for (int j = threadIdx.x; j < nj; j += blockDim.x) {
+        for (int j = l; j < nj; j += L) {
             if (j == excl) continue;
             const double dx = xp64[j] - xi;
             const double dy = xp64[(size_t)cap + j] - yi;
             const double dz = xp64[2 * (size_t)cap + j] - zi;
             const double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 < best) { best = d2; ba = j; }
+            if (d2 < bd) { bd = d2; ba = j; }
         }
-        // (d2, address) lexicographic minimum across the workgroup
-        for (int off = 32; off > 0; off >>= 1) {
-            const double od = __shfl_xor(best, off);
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) {
+            const double od = __shfl_xor(bd, off);
             const int oa = __shfl_xor(ba, off);
-            if (od < best || (od == best && oa < ba)) { best = od; ba = oa; }
+            if (od < bd || (od == bd && oa < ba)) { bd = od; ba = oa; }
         }
-        const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) { sd[w] = best; sa[w] = ba; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
-                if (sd[k] < best || (sd[k] == best && sa[k] < ba)) { best = sd[k]; ba = sa[k]; }
-            if (ba != INT_MAX) R.nn[i] = jindex[ba];
-        }
-        __syncthreads();
+        if (ba != INT_MAX) nn = jindex[ba];
+    }
+    if (l == 0) {
+        R.nn[i] = nn;
+        R.flags[i] = refine ? 1 : 0;
     }
 }
 
-hipError_t launch_refine(hipStream_t s, const double *xp64, int cap,
-                         const int *jindex, int nj, const IBlock &ib, int ni,
-                         const Results &r)
+// Lanes per i for the reduction: about two splits per lane, 4 to 64 lanes.
+int reduce_lanes(int nsplit)
+{
+    int L = 4;
+    while (L < 64 && 2 * L < nsplit) L *= 2;
+    return L;
+}
+
+hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
+                         const JPred *jpred, int nj, const IBlock &ib, float eps2,
+                         int tile, const double *xp64, int cap, const int *jindex,
+                         const Results &r, int lanes)
 {
     if (ni <= 0) return hipSuccess;
-    const int blocks = ni < 512 ? ni : 512;
-    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(256), 0, s, xp64, cap,
-                       jindex, nj, ib, r);
+    if (lanes <= 0) lanes = reduce_lanes(nsplit);
+    const int per_wg = kReduceWaves * (64 / lanes);
+    const dim3 grid((ni + per_wg - 1) / per_wg), block(64 * kReduceWaves);
+    switch (lanes) {
+    case 4:
+        hipLaunchKernelGGL(k_reduce<2>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+                           tile, xp64, cap, jindex, r);
+        break;
+    case 8:
+        hipLaunchKernelGGL(k_reduce<3>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+                           tile, xp64, cap, jindex, r);
+        break;
+    case 16:
+        hipLaunchKernelGGL(k_reduce<4>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+                           tile, xp64, cap, jindex, r);
+        break;
+    case 32:
+        hipLaunchKernelGGL(k_reduce<5>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+                           tile, xp64, cap, jindex, r);
+        break;
+    case 64:
+        hipLaunchKernelGGL(k_reduce<6>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+                           tile, xp64, cap, jindex, r);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

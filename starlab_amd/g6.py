@@ -191,6 +191,30 @@ class GRAPE6:
     def refinements(self) -> int:
         return self.L.g6x_last_refinements()
 
+    def autotune(self, nj: int, ni_max: int, eps2: float = 0.0) -> int:
+        n = self.L.g6x_autotune(0, int(nj), int(ni_max), float(eps2))
+        if n < 0:
+            raise RuntimeError(self.last_error())
+        return n
+
+    def tuning(self):
+        """Tuned shapes: {bucket: (variant, tile, waves, lanes)}."""
+        out = {}
+        for b in range(21):
+            v, t, w, l = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            if self.L.g6x_get_tuning(b, C.byref(v), C.byref(t), C.byref(w), C.byref(l)) == 1:
+                out[b] = (v.value, t.value, w.value, l.value)
+        return out
+
+    def clear_tuning(self):
+        self.L.g6x_clear_tuning()
+
+    def save_tuning(self, path: str) -> int:
+        return self.L.g6x_save_tuning(str(path).encode())
+
+    def load_tuning(self, path: str) -> int:
+        return self.L.g6x_load_tuning(str(path).encode())
+
     def last_error(self) -> str:
         s = self.L.g6x_last_error()
         return s.decode() if s else ""

@@ -25,12 +25,17 @@ def g():
     dev.close()
 
 
-@pytest.fixture(params=[0, 1], ids=["smem", "lds"])
+@pytest.fixture(params=[(0, 64), (0, 128), (0, 256), (1, 256)],
+                ids=["smem64", "smem128", "smem256", "lds"])
 def variant(g, request):
-    """Run a test on the scalar-load kernel and on the LDS-staged kernel."""
-    g.option("force_variant", request.param)
-    yield request.param
+    """Run a test on the scalar-load kernel with each j-tile length and on the
+    LDS-staged kernel."""
+    v, tile = request.param
+    g.option("force_variant", v)
+    g.option("tile", tile)
+    yield v
     g.option("force_variant", -1)
+    g.option("tile", 0)
 
 
 def _predict(s, ti, order=4):
@@ -286,3 +291,67 @@ def test_full_size_65536_sample(g):
     ep, _ = oracle.energies(m, xp, vp, 1e-8)
     assert abs(0.5 * (m * pot).sum() - ep) < 1e-7 * abs(ep)
     assert np.all(pot < 0) and np.all((nn >= 0) & (nn < n))
+
+
+@pytest.mark.parametrize("nsplit", [0, 7, 300])
+def test_small_blocks_many_splits(g, nsplit):
+    """kira's small blocks: a few i against many j, spread over many j-splits
+    (more splits than lanes of the reduction wave, a split count that does
+    not divide the j range, and the automatic choice)."""
+    s = plummer_state(20000, seed=29)
+    load_j(g, s)
+    g.option("nsplit", nsplit)
+    try:
+        rng = np.random.default_rng(5)
+        for ni in (1, 3, 64, 65):
+            isel = rng.choice(20000, ni, replace=False)
+            errs = _abi_force_vs_oracle(g, s, 0.0, 0.0, isel)
+            _check(errs, f"ni={ni} nsplit={nsplit} tile={g.option('last_tile')}")
+    finally:
+        g.option("nsplit", 0)
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_reduce_lanes_per_i(g, lanes):
+    """Every lanes-per-i shape of the split reduction, on a split count (47)
+    that is not a multiple of any of them and an i count that leaves partial
+    waves."""
+    s = plummer_state(6000, seed=31)
+    load_j(g, s)
+    g.option("nsplit", 47)
+    g.option("reduce_lanes", lanes)
+    try:
+        isel = np.random.default_rng(lanes).choice(6000, 300, replace=False)
+        errs = _abi_force_vs_oracle(g, s, 0.0, 1e-8, isel)
+        _check(errs, f"lanes={lanes}")
+    finally:
+        g.option("nsplit", 0)
+        g.option("reduce_lanes", 0)
+
+
+def test_autotune_table_and_parity(g, tmp_path):
+    """The autotuner fills one shape per i-block bucket, forces computed with
+    tuned shapes keep parity, and the table survives a save/load round
+    trip."""
+    s = plummer_state(20000, seed=37)
+    load_j(g, s)
+    try:
+        n = g.autotune(20000, 700, 1e-8)
+        tab = g.tuning()
+        assert n == len(tab) == 11                      # buckets 1, 2, 4, ..., 1024 (ni <= 700)
+        for b, (v, tile, waves, lanes) in tab.items():
+            assert v in (0, 1) and tile in (64, 128, 256) and waves > 0
+            assert lanes in (0, 4, 8, 16, 32, 64)         # 0: the split-count rule won
+        rng = np.random.default_rng(9)
+        for ni in (1, 5, 64, 300, 700):
+            isel = rng.choice(20000, ni, replace=False)
+            errs = _abi_force_vs_oracle(g, s, 0.0, 1e-8, isel)
+            _check(errs, f"tuned ni={ni} shape={tab[max(0, int(np.ceil(np.log2(ni))))]}")
+        path = tmp_path / "shapes.txt"
+        assert g.save_tuning(path) == 0
+        g.clear_tuning()
+        assert g.tuning() == {}
+        assert g.load_tuning(path) == len(tab)
+        assert g.tuning() == tab
+    finally:
+        g.clear_tuning()
