@@ -123,6 +123,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ens", action="store_true", help="skip the ensemble / serving legs")
     ap.add_argument("--no-tune", action="store_true", help="skip the force-shape autotuner")
+    ap.add_argument("--tune-file", default=None,
+                    help="force-shape table: loaded if it exists (no tuning), else written "
+                         "after tuning")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 256 MiB streaming kernel (read + write), the "
+                         "reference for scaling HBM traffic counters, and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "libg6 drives one GPU per process; this build measures "
@@ -134,24 +140,39 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
 
+    if a.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy": "torch.neg float32", "bytes_read": src.numel() * 4,
+                          "bytes_written": dst.numel() * 4, "launches": 5}))
+        return
+
     from starlab_amd.g6 import GRAPE6
     g = GRAPE6()
     if g.open() != 0:
         raise SystemExit(f"g6_open_ failed: {g.last_error()}")
     n = a.n
+    tune = None
+    if a.tune_file and os.path.exists(a.tune_file):
+        nb = g.load_tuning(a.tune_file)
+        tune = {"loaded": a.tune_file, "buckets": nb}
     log(f"bench: generating Plummer N={n}")
     inp = make_input(g, n)
 
-    tune = None
-    if not a.no_tune:
+    if tune is None and not a.no_tune:
         log("bench: autotuning force shapes")
         t0 = time.perf_counter()
         nb = g.autotune(n, n, EPS2)
-        tab = g.tuning()
-        full = tab.get(max(0, (n - 1).bit_length()))
-        tune = {"seconds": round(time.perf_counter() - t0, 3), "buckets": nb,
-                "full_sweep_shape": dict(zip(("variant", "tile", "waves", "lanes"), full))
-                if full else None}
+        tune = {"seconds": round(time.perf_counter() - t0, 3), "buckets": nb}
+        if a.tune_file:
+            g.save_tuning(a.tune_file)
+    if tune is not None:
+        full = g.tuning().get(max(0, (n - 1).bit_length()))
+        tune["full_sweep_shape"] = (dict(zip(("variant", "tile", "waves", "lanes"), full))
+                                    if full else None)
 
     for _ in range(a.warmup):
         assert g.self_sweep(n, 0, n, 0.0, EPS2) == 0

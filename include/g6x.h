@@ -52,7 +52,7 @@ int g6x_sync(int cluster_id);
  *   "xcd_remap"      1 = keep each j-split on one XCD         G6_XCD_REMAP
  *   "tile"           j per FP32 tile: 0 auto, 64, 128, 256    G6_TILE
  *   "last_tile"      (read only) tile of the last force launch
- *   "reduce_lanes"   lanes per i in the split reduction: 0 auto, 4..64
+ *   "reduce_lanes"   lanes per i in the split reduction: 0 auto, 4..64  G6_REDUCE_LANES
  * Returns 0, or -1 for an unknown key. */
 int       g6x_set_option(const char *key, long long value);
 long long g6x_get_option(const char *key);

@@ -185,6 +185,7 @@ void read_env()
     o.force_variant = (int)env_ll("G6_FORCE_VARIANT", o.force_variant);
     o.lds_min_ni = (int)env_ll("G6_LDS_MIN_NI", o.lds_min_ni);
     o.tile = (int)env_ll("G6_TILE", o.tile);
+    o.reduce_lanes = (int)env_ll("G6_REDUCE_LANES", o.reduce_lanes);
     if (o.npipes < 1) o.npipes = 1;
     if (o.nb_cap < 1) o.nb_cap = 1;
 }
