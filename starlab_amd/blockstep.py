@@ -24,6 +24,7 @@ C ABI; tests supply an FP64 host backend to obtain the all-CPU reference run.
 """
 from __future__ import annotations
 
+import heapq
 import math
 
 import numpy as np
@@ -33,16 +34,24 @@ ONE12 = 1.0 / 12.0
 
 
 class G6Backend:
-    """libg6 through the GRAPE-6 ABI, one particle per g6_set_j_particle_."""
+    """libg6 through the GRAPE-6 ABI.  j updates go one particle per
+    g6_set_j_particle_ call, as kira makes them, or (bulk=True) through
+    g6x_set_j_particles, which stores the same records in one call."""
 
-    def __init__(self, g):
+    def __init__(self, g, bulk=False):
         self.g = g
+        self.bulk = bulk
 
     def open(self, n):
         assert self.g.open() == 0, self.g.last_error()
         self.npipes = self.g.npipes()
 
     def send_j(self, idx, tj, dtj, m, x, v, a, j, k18):
+        if self.bulk:
+            rc = self.g.set_j_particles(idx, idx, tj, dtj, m, k18, j / 6.0, a / 2.0, v, x)
+            if rc:
+                raise RuntimeError(self.g.last_error())
+            return
         for k, i in enumerate(idx):
             rc = self.g.set_j_particle(int(i), int(i), float(tj[k]), float(dtj[k]), float(m[k]),
                                        k18[k], j[k] / 6.0, a[k] / 2.0, v[k], x[k])
@@ -107,7 +116,24 @@ class BlockHermite:
         vp = (self.j[idx] * dt2 + self.a[idx]) * dt + self.v[idx]
         return xp, vp
 
+    def _schedule(self, idx):
+        """File particles idx under their next time t + dt.  Times are exact
+        binary fractions, so equal next times compare equal."""
+        tn = self.t[idx] + self.dt[idx]
+        for u in np.unique(tn):
+            u = float(u)
+            grp = idx[tn == u]
+            lst = self._sched.get(u)
+            if lst is None:
+                self._sched[u] = [grp]
+                heapq.heappush(self._heap, u)
+            else:
+                lst.append(grp)
+
     def run(self, backend, t_end, on_block=None):
+        """Advance every particle to t_end.  Each block is the set of
+        particles with the smallest t + dt, in ascending index order
+        (kira's fast_get_nodes_to_move, hdyn_schedule.C:641-745)."""
         n = self.n
         allp = np.arange(n, dtype=np.int32)
         backend.open(n)
@@ -118,13 +144,15 @@ class BlockHermite:
             a, j, pot, nn = backend.force(n, 0.0, allp, self.x, self.v, self.eps2)
             self.a[:], self.j[:], self.pot[:] = a, j, pot
             self.dt[:] = first_step(self.eta, a, j, pot, min(self.dt_max, t_end))
+            self._sched, self._heap = {}, []
+            self._schedule(allp)
             prev = allp
             while True:
-                tn = self.t + self.dt
-                t_next = tn.min()
+                t_next = self._heap[0]
                 if t_next > t_end:
                     break
-                blk = np.flatnonzero(tn == t_next).astype(np.int32)
+                heapq.heappop(self._heap)
+                blk = np.sort(np.concatenate(self._sched.pop(t_next))).astype(np.int32)
                 # previous block's particles go to j-memory (hdyn_grape6.C:3112-3113)
                 backend.send_j(prev, self.t[prev], self.dt[prev], self.m[prev], self.x[prev],
                                self.v[prev], self.a[prev], self.j[prev], self.k18[prev])
@@ -133,6 +161,7 @@ class BlockHermite:
                 if on_block is not None:
                     on_block(self, t_next, blk, xi, vi, a1, j1, pot, nn)
                 self.correct(blk, t_next, xi, vi, a1, j1, pot)
+                self._schedule(blk)
                 prev = blk
                 self.blocks += 1
                 self.particle_steps += len(blk)

@@ -40,12 +40,20 @@ class HostBackend:
 
 
 class CheckedG6(G6Backend):
-    def __init__(self, g):
-        super().__init__(g)
+    """every: check every `every`-th force call; max_i: at most this many i
+    of a checked call (a seeded sample) go to the oracle."""
+
+    def __init__(self, g, bulk=False, every=1, max_i=None, seed=0):
+        super().__init__(g, bulk)
         self.worst = dict(acc_norm=0.0, jerk_norm=0.0, pot_rel=0.0)
         self.acc_errs = []
         self.nn_mismatch = 0
         self.checked_blocks = 0
+        self.checked_i = 0
+        self.calls = 0
+        self.every = every
+        self.max_i = max_i
+        self.rng = np.random.default_rng(seed)
 
     def open(self, n):
         super().open(n)
@@ -60,7 +68,15 @@ class CheckedG6(G6Backend):
         M["a2"][idx] = a / 2.0; M["j6"][idx] = j / 6.0; M["k18"][idx] = k18
 
     def force(self, nj, t, idx, xi, vi, eps2):
-        acc, jerk, pot, nn = super().force(nj, t, idx, xi, vi, eps2)
+        acc_all, jerk_all, pot_all, nn_all = super().force(nj, t, idx, xi, vi, eps2)
+        self.calls += 1
+        if (self.calls - 1) % self.every:
+            return acc_all, jerk_all, pot_all, nn_all
+        sel = np.arange(len(idx))
+        if self.max_i is not None and len(idx) > self.max_i:
+            sel = np.sort(self.rng.choice(len(idx), self.max_i, replace=False))
+        acc, jerk, pot, nn = acc_all[sel], jerk_all[sel], pot_all[sel], nn_all[sel]
+        idx, xi, vi = idx[sel], xi[sel], vi[sel]
         M = self.mir
         xp, vp = oracle.predict(M["x"], M["v"], M["a2"], M["j6"], M["k18"], M["t"], t, 4)
         racc, rjerk, rpot, rnn, _ = oracle.force(xi, vi, idx, xp, vp, M["m"], self.idx, eps2)
@@ -80,4 +96,5 @@ class CheckedG6(G6Backend):
                                     float((np.abs(pot - rpot) / np.abs(rpot)).max()))
         self.nn_mismatch += int((nn != rnn).sum())
         self.checked_blocks += 1
-        return acc, jerk, pot, nn
+        self.checked_i += len(idx)
+        return acc_all, jerk_all, pot_all, nn_all

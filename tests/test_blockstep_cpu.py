@@ -28,3 +28,20 @@ def test_small_plummer_energy_and_schedule():
     e1 = b.energy(lambda m, x, v, e: oracle.energies(m, x, v, e))
     assert abs((e1 - e0) / e0) < 1e-6
     assert b.particle_steps < len(seen) * 64   # individual steps, not shared
+
+
+def test_scheduler_picks_the_min_time_set():
+    """Each block is exactly {i : t_i + dt_i = min_k (t_k + dt_k)}, ascending."""
+    m, x, v = oracle.makeplummer(128, 7)
+    b = BlockHermite(m, x, v, eta=0.05, eps2=1e-4)
+    seen = []
+
+    def check(s, t, blk, *r):
+        tn = s.t + s.dt
+        want = np.flatnonzero(tn == tn.min())
+        assert tn.min() == t
+        np.testing.assert_array_equal(blk, want)
+        seen.append(len(blk))
+
+    b.run(HostBackend(), 0.125, on_block=check)
+    assert len(seen) > 10 and min(seen) < 128
