@@ -72,7 +72,9 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * "max_steps" (per system; beyond it the job fails),
  * "light_max" (systems with n <= light_max run in the light lane, which
  * keeps stepping on its own stream while a heavy-lane graph runs; default
- * 64).
+ * 64), "fused_steps" (with light_max <= 64 the light lane runs a fused
+ * kernel, one wave per system, up to this many steps per launch; 0 uses
+ * step graphs there too; default 256).
  * Returns 0, or -1 for an unknown key. */
 int       g6e_set_option(const char *key, long long value);
 long long g6e_get_option(const char *key);

@@ -84,6 +84,45 @@ constexpr double kOneThird = 1.0 / 3.0;
 constexpr double kOne12 = 1.0 / 12.0;
 
 // ---------------------------------------------------------------------------
+// Per-particle pieces shared by the step kernels and the fused kernel, so
+// both paths perform the same operations in the same order.
+// ---------------------------------------------------------------------------
+// predict_loworder, include/_dyn_.h:290-295
+__device__ __forceinline__ void ens_predict_one(const double x[3], const double v[3],
+                                                const double a[3], const double j[3], double dt,
+                                                double xp[3], double vp[3])
+{
+#pragma clang fp contract(off)
+    const double dt3 = dt * kOneThird, dt2 = dt * 0.5;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xp[k] = ((j[k] * dt3 + a[k]) * dt2 + v[k]) * dt + x[k];
+        vp[k] = (j[k] * dt2 + a[k]) * dt + v[k];
+    }
+}
+
+__device__ __forceinline__ JPred make_jpred(const double xp[3], const double vp[3], double m)
+{
+#pragma clang fp contract(off)
+    JPred q;
+    q.xh = (float)xp[0]; q.xl = (float)(xp[0] - (double)q.xh);
+    q.yh = (float)xp[1]; q.yl = (float)(xp[1] - (double)q.yh);
+    q.zh = (float)xp[2]; q.zl = (float)(xp[2] - (double)q.zh);
+    q.vx = (float)vp[0]; q.vy = (float)vp[1]; q.vz = (float)vp[2];
+    q.m = (float)m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q.pad[k] = 0.f;
+    return q;
+}
+
+__device__ __forceinline__ JPred64 make_jpred64(const double xp[3], const double vp[3], double m)
+{
+    JPred64 d;
+    d.x = xp[0]; d.y = xp[1]; d.z = xp[2]; d.m = m;
+    d.vx = vp[0]; d.vy = vp[1]; d.vz = vp[2]; d.pad = 0.0;
+    return d;
+}
+
 __global__ __launch_bounds__(64) void k_ens_predict(Pool P, const SysHdr *__restrict__ H,
                                                     const Chunk *__restrict__ chunks,
                                                     const int *__restrict__ nchunks)
@@ -99,31 +138,20 @@ __global__ __launch_bounds__(64) void k_ens_predict(Pool P, const SysHdr *__rest
     const size_t p = (size_t)h.off + i;
     const size_t cap = P.cap;
     const double dt = h.phase == STEP ? h.dt : 0.0;
-    const double dt3 = dt * kOneThird, dt2 = dt * 0.5;
-    double xp[3], vp[3];
+    double x[3], v[3], a[3], jk[3], xp[3], vp[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double x = P.x[k * cap + p], v = P.v[k * cap + p];
-        const double a = P.a[k * cap + p], jk = P.j[k * cap + p];
-        // predict_loworder, include/_dyn_.h:290-295
-        xp[k] = ((jk * dt3 + a) * dt2 + v) * dt + x;
-        vp[k] = (jk * dt2 + a) * dt + v;
+        x[k] = P.x[k * cap + p]; v[k] = P.v[k * cap + p];
+        a[k] = P.a[k * cap + p]; jk[k] = P.j[k * cap + p];
+    }
+    ens_predict_one(x, v, a, jk, dt, xp, vp);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
         P.xp[k * cap + p] = xp[k];
         P.vp[k * cap + p] = vp[k];
     }
-    JPred q;
-    q.xh = (float)xp[0]; q.xl = (float)(xp[0] - (double)q.xh);
-    q.yh = (float)xp[1]; q.yl = (float)(xp[1] - (double)q.yh);
-    q.zh = (float)xp[2]; q.zl = (float)(xp[2] - (double)q.zh);
-    q.vx = (float)vp[0]; q.vy = (float)vp[1]; q.vz = (float)vp[2];
-    q.m = (float)P.m[p];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) q.pad[k] = 0.f;
-    P.jp[p] = q;
-    JPred64 d;
-    d.x = xp[0]; d.y = xp[1]; d.z = xp[2]; d.m = P.m[p];
-    d.vx = vp[0]; d.vy = vp[1]; d.vz = vp[2]; d.pad = 0.0;
-    P.jp64[p] = d;
+    P.jp[p] = make_jpred(xp, vp, P.m[p]);
+    P.jp64[p] = make_jpred64(xp, vp, P.m[p]);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,6 +328,71 @@ __device__ __forceinline__ double dot3(const double *u, const double *v)
     return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
 }
 
+// One particle's state as the corrector sees it.
+struct PState {
+    double x[3], v[3], a[3], j[3], pot, pot0;
+};
+
+// Corrector and step candidate for one particle, given its predicted state
+// and the new force; updates the fields the phase writes and returns the
+// candidate step (+inf: none).
+__device__ __forceinline__ double ens_correct_one(PState &st, const double xp[3],
+                                                  const double vp[3], const double a1[3],
+                                                  const double j1[3], double pot, int phase,
+                                                  double dt, double eta, int prec)
+{
+#pragma clang fp contract(off)
+    double cand = __builtin_inf();
+    if (phase == INIT) {
+        // set_first_timestep, hdyn_ev.C:377-410: eta/16 * sqrt(min(a^2/j^2, -pot/a^2))
+        for (int k = 0; k < 3; ++k) { st.a[k] = a1[k]; st.j[k] = j1[k]; }
+        st.pot = pot;
+        st.pot0 = pot;
+        const double eta_init = eta / 16;
+        const double asq = dot3(a1, a1), jsq = dot3(j1, j1);
+        double dt_adot = 1e300, dt_ff = 1e300;
+        if (jsq > 0) dt_adot = asq / jsq;
+        if (pot < 0 && asq > 0) dt_ff = -pot / asq;
+        cand = eta_init * sqrt(fmin(dt_adot, dt_ff));
+        cand = fmin(cand, eta_init);
+    } else if (phase == STEP) {
+        double bt2[3], at3[3];
+        for (int k = 0; k < 3; ++k) {
+            const double a0 = st.a[k], j0 = st.j[k];
+            // get_derivs, hdyn_ev.C:1235-1250
+            bt2[k] = -3 * (a0 - a1[k]) - dt * (2 * j0 + j1[k]);
+            at3[k] = 2 * (a0 - a1[k]) + dt * (j0 + j1[k]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            // hdyn_ev.C:1717-1718
+            st.x[k] = xp[k] + (0.05 * at3[k] + kOne12 * bt2[k]) * dt * dt;
+            st.v[k] = vp[k] + (0.25 * at3[k] + kOneThird * bt2[k]) * dt;
+            st.a[k] = a1[k];
+            st.j[k] = j1[k];
+        }
+        st.pot = pot;
+        // Aarseth criterion, hdyn_ev.C:627-655
+        const double dt2 = dt * dt, dt4 = dt2 * dt2, dt6 = dt4 * dt2;
+        double l2 = 36 * dot3(at3, at3) / dt6;
+        double k2 = 4 * dot3(bt2, bt2) / dt4;
+        if (prec != 64) {
+            // remove the FP32 force noise: a0 - a1 carries ~2 kNoiseFP32 |a|
+            const double na = kNoiseFP32 * sqrt(dot3(a1, a1));
+            const double at3m = fmax(sqrt(dot3(at3, at3)) - 4 * na, 0.0);
+            const double bt2m = fmax(sqrt(dot3(bt2, bt2)) - 6 * na, 0.0);
+            l2 = 36 * at3m * at3m / dt6;
+            k2 = 4 * bt2m * bt2m / dt4;
+        }
+        const double j2 = dot3(j1, j1);
+        const double asq = dot3(a1, a1);
+        cand = eta * sqrt((sqrt(asq * k2) + j2) / (sqrt(j2 * l2) + k2));
+        if (!(cand > 0)) cand = __builtin_inf();
+    } else {   // FINAL: potential at the exact final positions
+        st.pot = pot;
+    }
+    return cand;
+}
+
 __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__ H,
                                                     const Chunk *__restrict__ chunks,
                                                     const int *__restrict__ nchunks)
@@ -330,54 +423,29 @@ __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__
             j1[0] += in[3 * cap + p]; j1[1] += in[4 * cap + p]; j1[2] += in[5 * cap + p];
             pot += in[6 * cap + p];
         }
+        PState st;
+        double xp[3], vp[3];
+        for (int k = 0; k < 3; ++k) {
+            st.x[k] = P.x[k * cap + p]; st.v[k] = P.v[k * cap + p];
+            st.a[k] = P.a[k * cap + p]; st.j[k] = P.j[k * cap + p];
+            xp[k] = P.xp[k * cap + p]; vp[k] = P.vp[k * cap + p];
+        }
+        st.pot = 0; st.pot0 = 0;
+        cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, h.dt, h.eta, h.prec);
         if (phase == INIT) {
-            // set_first_timestep, hdyn_ev.C:377-410: eta/16 * sqrt(min(a^2/j^2, -pot/a^2))
-            for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = a1[k]; P.j[k * cap + p] = j1[k]; }
-            P.pot[p] = pot;
-            P.pot0[p] = pot;
-            const double eta_init = h.eta / 16;
-            const double asq = dot3(a1, a1), jsq = dot3(j1, j1);
-            double dt_adot = 1e300, dt_ff = 1e300;
-            if (jsq > 0) dt_adot = asq / jsq;
-            if (pot < 0 && asq > 0) dt_ff = -pot / asq;
-            cand = eta_init * sqrt(fmin(dt_adot, dt_ff));
-            cand = fmin(cand, eta_init);
+            for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = st.a[k]; P.j[k * cap + p] = st.j[k]; }
+            P.pot[p] = st.pot;
+            P.pot0[p] = st.pot0;
         } else if (phase == STEP) {
-            const double dt = h.dt;
-            double bt2[3], at3[3], a0[3], j0[3];
             for (int k = 0; k < 3; ++k) {
-                a0[k] = P.a[k * cap + p];
-                j0[k] = P.j[k * cap + p];
-                // get_derivs, hdyn_ev.C:1235-1250
-                bt2[k] = -3 * (a0[k] - a1[k]) - dt * (2 * j0[k] + j1[k]);
-                at3[k] = 2 * (a0[k] - a1[k]) + dt * (j0[k] + j1[k]);
+                P.x[k * cap + p] = st.x[k];
+                P.v[k * cap + p] = st.v[k];
+                P.a[k * cap + p] = st.a[k];
+                P.j[k * cap + p] = st.j[k];
             }
-            for (int k = 0; k < 3; ++k) {
-                // hdyn_ev.C:1717-1718
-                P.x[k * cap + p] = P.xp[k * cap + p] + (0.05 * at3[k] + kOne12 * bt2[k]) * dt * dt;
-                P.v[k * cap + p] = P.vp[k * cap + p] + (0.25 * at3[k] + kOneThird * bt2[k]) * dt;
-                P.a[k * cap + p] = a1[k];
-                P.j[k * cap + p] = j1[k];
-            }
-            P.pot[p] = pot;
-            // Aarseth criterion, hdyn_ev.C:627-655
-            const double dt2 = dt * dt, dt4 = dt2 * dt2, dt6 = dt4 * dt2;
-            double l2 = 36 * dot3(at3, at3) / dt6;
-            double k2 = 4 * dot3(bt2, bt2) / dt4;
-            if (h.prec != 64) {
-                // remove the FP32 force noise: a0 - a1 carries ~2 kNoiseFP32 |a|
-                const double na = kNoiseFP32 * sqrt(dot3(a1, a1));
-                const double at3m = fmax(sqrt(dot3(at3, at3)) - 4 * na, 0.0);
-                const double bt2m = fmax(sqrt(dot3(bt2, bt2)) - 6 * na, 0.0);
-                l2 = 36 * at3m * at3m / dt6;
-                k2 = 4 * bt2m * bt2m / dt4;
-            }
-            const double j2 = dot3(j1, j1);
-            const double asq = dot3(a1, a1);
-            cand = h.eta * sqrt((sqrt(asq * k2) + j2) / (sqrt(j2 * l2) + k2));
-            if (!(cand > 0)) cand = __builtin_inf();
-        } else {   // FINAL: potential at the exact final positions
-            P.pot[p] = pot;
+            P.pot[p] = st.pot;
+        } else {
+            P.pot[p] = st.pot;
         }
     }
     // wave minimum, then one atomic per wave
@@ -394,14 +462,11 @@ __device__ __forceinline__ double pow2_below(double x)
     return ldexp(1.0, e - 1);
 }
 
-__global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restrict__ phase_out,
-                               long long max_steps, double dt_min, int lane)
+// t += dt and kira's step rules for one system, given the smallest step
+// candidate `nat` of its particles.
+__device__ __forceinline__ void ens_finalize_hdr(SysHdr &h, double nat, long long max_steps,
+                                                 double dt_min)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    SysHdr &h = H[s];
-    if (h.lane != lane || h.phase < INIT || h.phase > FINAL) return;
-    const double nat = __longlong_as_double((long long)h.pmin);
     if (h.phase == INIT) {
         double dt = fmin(nat, h.dt_max);
         dt = pow2_below(dt);
@@ -428,8 +493,116 @@ __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restri
     } else if (h.phase == FINAL) {
         h.phase = DONE;
     }
+}
+
+__global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restrict__ phase_out,
+                               long long max_steps, double dt_min, int lane)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    SysHdr &h = H[s];
+    if (h.lane != lane || h.phase < INIT || h.phase > FINAL) return;
+    const double nat = __longlong_as_double((long long)h.pmin);
+    ens_finalize_hdr(h, nat, max_steps, dt_min);
     h.pmin = 0x7FF0000000000000ull;   // +inf
     phase_out[s] = h.phase;
+}
+
+// ---------------------------------------------------------------------------
+// Fused kernel for light systems (n <= 64): one wave owns one system, one
+// lane per particle, and runs whole engine steps (predict, force, correct,
+// step rules) in a loop of up to max_iters steps, leaving early when the
+// system is done.  Predicted j live in LDS; state stays in registers and is
+// written back at the end.  The arithmetic is the step kernels' (shared
+// helpers above, one j-split, j ascending), so a system's trajectory is the
+// same bit for bit on either path.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_ens_fused(Pool P, SysHdr *__restrict__ H,
+                                                  const Chunk *__restrict__ list,
+                                                  const int *__restrict__ nlist,
+                                                  int *__restrict__ phase_out,
+                                                  long long max_steps, double dt_min,
+                                                  int max_iters)
+{
+#pragma clang fp contract(off)
+    __shared__ JPred64 s64[64];
+    __shared__ JPred s32[64];
+    const int w = blockIdx.x;
+    if (w >= *nlist) return;
+    const int slot = list[w].slot;
+    SysHdr h = H[slot];
+    if (h.phase < INIT || h.phase > FINAL || h.n > 64) return;
+    const int n = h.n;
+    const int i = threadIdx.x;
+    const bool act = i < n;
+    const int ic = act ? i : n - 1;
+    const size_t cap = P.cap;
+    const size_t p = (size_t)h.off + ic;
+    const double m = P.m[p];
+    PState st;
+    for (int k = 0; k < 3; ++k) {
+        st.x[k] = P.x[k * cap + p]; st.v[k] = P.v[k * cap + p];
+        st.a[k] = P.a[k * cap + p]; st.j[k] = P.j[k * cap + p];
+    }
+    st.pot = P.pot[p];
+    st.pot0 = P.pot0[p];
+    const int self = act ? i : -1;
+    double xp[3], vp[3];
+    for (int it = 0; it < max_iters; ++it) {
+        const int phase = h.phase;
+        ens_predict_one(st.x, st.v, st.a, st.j, phase == STEP ? h.dt : 0.0, xp, vp);
+        if (act) {
+            s64[i] = make_jpred64(xp, vp, m);
+            s32[i] = make_jpred(xp, vp, m);
+        }
+        __syncthreads();
+        double a1[3], j1[3], pot;
+        if (h.prec == 64) {
+            const JPred64 I = s64[ic];
+            DAcc acc;
+            acc.ax = acc.ay = acc.az = acc.jx = acc.jy = acc.jz = acc.p = 0.0;
+            for (int jj = 0; jj < n; ++jj) ens_pair64<true>(s64[jj], jj, self, I, h.eps2, acc);
+            a1[0] = acc.ax; a1[1] = acc.ay; a1[2] = acc.az;
+            j1[0] = acc.jx; j1[1] = acc.jy; j1[2] = acc.jz;
+            pot = acc.p;
+        } else {
+            const JPred I = s32[ic];
+            EAcc e;
+            e.ax = e.ay = e.az = e.jx = e.jy = e.jz = e.kx = e.ky = e.kz = e.p = 0.f;
+            ens_tile<true>(s32, n, (unsigned)self, I, (float)h.eps2, e);
+            double A[7] = {0, 0, 0, 0, 0, 0, 0};
+            A[0] += (double)e.ax;
+            A[1] += (double)e.ay;
+            A[2] += (double)e.az;
+            A[3] += (double)e.jx - 3.0 * (double)e.kx;
+            A[4] += (double)e.jy - 3.0 * (double)e.ky;
+            A[5] += (double)e.jz - 3.0 * (double)e.kz;
+            A[6] += (double)e.p;
+            a1[0] = A[0]; a1[1] = A[1]; a1[2] = A[2];
+            j1[0] = A[3]; j1[1] = A[4]; j1[2] = A[5];
+            pot = A[6];
+        }
+        __syncthreads();   // LDS is rewritten by the next step's prediction
+        double cand = __builtin_inf();
+        if (act) cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, h.dt, h.eta, h.prec);
+        for (int off = 32; off > 0; off >>= 1) cand = fmin(cand, __shfl_xor(cand, off));
+        ens_finalize_hdr(h, cand, max_steps, dt_min);   // every lane, same values
+        if (h.phase == DONE || h.phase == FAILED) break;
+    }
+    if (act) {
+        for (int k = 0; k < 3; ++k) {
+            P.x[k * cap + p] = st.x[k]; P.v[k * cap + p] = st.v[k];
+            P.a[k * cap + p] = st.a[k]; P.j[k * cap + p] = st.j[k];
+            P.xp[k * cap + p] = xp[k]; P.vp[k * cap + p] = vp[k];
+        }
+        P.pot[p] = st.pot;
+        P.pot0[p] = st.pot0;
+    }
+    if (i == 0) {
+        h.pmin = 0x7FF0000000000000ull;
+        H[slot] = h;
+        phase_out[slot] = h.phase;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -559,6 +732,7 @@ struct Engine {
     int lookahead = 64;
     int precision = 64;
     int light_max = 64;
+    int fused_steps = 256;            // light lane: steps per fused launch (0: graphs)
     double dt_max = 0.125;
     long long max_steps = 50000000;
     // stats
@@ -1082,7 +1256,7 @@ int g6e_run(int steps)
         const bool heavy = H.n_chunks > 0, light = Lt.n_chunks > 0;
         hipGraphExec_t eh = nullptr, el = nullptr;
         if (heavy && get_graph(0, eh)) return -1;
-        if (light && get_graph(1, el)) return -1;
+        if (light && !(X.fused_steps > 0 && X.light_max <= 64) && get_graph(1, el)) return -1;
         double ms_total = 0;
         if (heavy) {
             ECHK(hipEventRecord(H.ev0, H.stream));
@@ -1091,10 +1265,18 @@ int g6e_run(int steps)
         }
         if (light) {
             // keep the light lane stepping until the heavy graph is done
+            const bool fused = X.fused_steps > 0 && X.light_max <= 64;
             int replays = 0;
             do {
                 ECHK(hipEventRecord(Lt.ev0, Lt.stream));
-                ECHK(hipGraphLaunch(el, Lt.stream));
+                if (fused) {
+                    hipLaunchKernelGGL(k_ens_fused, dim3(Lt.n_chunks), dim3(64), 0, Lt.stream, X.P,
+                                       X.d_hdr, Lt.d_chunks, Lt.d_counts, X.d_phase, X.max_steps,
+                                       1e-30, X.fused_steps);
+                    ECHK(hipGetLastError());
+                } else {
+                    ECHK(hipGraphLaunch(el, Lt.stream));
+                }
                 ECHK(hipEventRecord(Lt.ev1, Lt.stream));
                 ECHK(hipStreamSynchronize(Lt.stream));
                 float ms = 0;
@@ -1169,6 +1351,7 @@ int g6e_set_option(const char *key, long long value)
     else if (k == "dt_max_bits") X.dt_max = std::ldexp(1.0, (int)value);
     else if (k == "max_steps") X.max_steps = std::max(1LL, value);
     else if (k == "light_max") X.light_max = (int)std::max(0LL, value);
+    else if (k == "fused_steps") X.fused_steps = (int)std::max(0LL, value);
     else if (k == "precision") {
         if (value != 32 && value != 64) return efail(-1, "precision must be 32 or 64");
         X.precision = (int)value;
@@ -1186,6 +1369,7 @@ long long g6e_get_option(const char *key)
     if (k == "max_steps") return X.max_steps;
     if (k == "precision") return X.precision;
     if (k == "light_max") return X.light_max;
+    if (k == "fused_steps") return X.fused_steps;
     if (k == "light_steps") return X.lanes[1].steps;
     if (k == "heavy_steps") return X.lanes[0].steps;
     if (k == "pool_capacity") return X.P.cap;

@@ -37,7 +37,9 @@ def _jobs(seed, sizes, t_end, eta=0.05):
     return out
 
 
-DEFAULTS = dict(jsplit_target=512, precision=64, max_steps=50_000_000, steps_per_graph=8)
+DEFAULTS = dict(jsplit_target=512, precision=64, max_steps=50_000_000, steps_per_graph=8,
+                fused_steps=256, light_max=64)
+PATHS = pytest.mark.parametrize("fused", [256, 0], ids=["fused", "graphs"])
 
 
 def _run(jobs, budget=2 * GB, max_systems=4096, **opt):
@@ -53,9 +55,11 @@ def _run(jobs, budget=2 * GB, max_systems=4096, **opt):
     return [res[i] for i in ids], stats
 
 
-def test_fp64_bitwise_equals_oracle():
+@PATHS
+def test_fp64_bitwise_equals_oracle(fused):
+    """Light systems (n <= 64) run in the fused kernel or in step graphs."""
     jobs = _jobs(1, [3, 3, 16, 64, 200, 700, 1300], t_end=1.0 / 32)
-    res, _ = _run(jobs, jsplit_target=0)
+    res, _ = _run(jobs, jsplit_target=0, fused_steps=fused)
     for j, r in zip(jobs, res):
         xf, vf, t, steps, e0, e1, rc = oracle.hermite_shared(j["m"], j["x"], j["v"], j["t_end"],
                                                             j["eta"], j["eps2"])
@@ -98,13 +102,14 @@ def test_energy_conservation_vs_oracle():
                 assert de_gpu <= 10 * de_cpu + 1e-5
 
 
+@PATHS
 @pytest.mark.parametrize("prec", [64, 32])
-def test_alone_equals_mixed_bitwise(prec):
+def test_alone_equals_mixed_bitwise(prec, fused):
     target = _jobs(3, [500], t_end=1.0 / 8)[0]
     target3 = _jobs(4, [3], t_end=2.0)[0]
-    alone, _ = _run([target, target3], precision=prec)
+    alone, _ = _run([target, target3], precision=prec, fused_steps=fused)
     mix = _jobs(5, [3, 64, 2048, 3, 700, 3, 128, 1500, 3, 3, 40], t_end=1.0 / 8)
-    e = Ensemble(2 * GB, 64, precision=prec)
+    e = Ensemble(2 * GB, 64, precision=prec, fused_steps=fused)
     try:
         others = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in mix[:6]]
         e.run(16)                                   # the mix is already running
@@ -116,7 +121,8 @@ def test_alone_equals_mixed_bitwise(prec):
         res = e.run_until_idle()
     finally:
         e.close()
-        e.option("precision", 64)
+        for k, v in DEFAULTS.items():
+            e.option(k, v)
     for ref, got in ((alone[0], res[a]), (alone[1], res[b])):
         np.testing.assert_array_equal(got["x"], ref["x"])
         np.testing.assert_array_equal(got["v"], ref["v"])
@@ -143,3 +149,17 @@ def test_step_limit_marks_failure():
         e.close()
         e.option("max_steps", 50_000_000)      # engine options outlive the engine
     assert res[jid]["failed"] and res[jid]["t"] < 1.0
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fused_kernel_equals_step_graphs_bitwise(prec):
+    """The fused light-system kernel performs the step kernels' arithmetic:
+    same trajectories, step counts and energies bit for bit."""
+    jobs = _jobs(9, [3, 3, 3, 5, 17, 40, 64], t_end=1.0)
+    fused, _ = _run(jobs, precision=prec, fused_steps=256)
+    graphs, _ = _run(jobs, precision=prec, fused_steps=0)
+    for f, g in zip(fused, graphs):
+        assert f["steps"] == g["steps"] and f["t"] == g["t"] == 1.0
+        np.testing.assert_array_equal(f["x"], g["x"])
+        np.testing.assert_array_equal(f["v"], g["v"])
+        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
