@@ -53,6 +53,8 @@ def lib():
                                   _ip, C.c_double, _dp, _dp, _dp]
         L.or_neighbours.argtypes = [C.c_int, _dp, _dp, _ip, C.c_int, C.c_int,
                                     _dp, _ip, C.c_int, _ip, _ip]
+        L.or_g4_force.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
+                                  _dp, _dp, _dp, C.c_int, _ip, _ip]
         L.or_energies.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_hermite_shared.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
@@ -121,6 +123,20 @@ def neighbours(xi, h2, excl, include_self: bool, xj, jindex, cap: int):
     lib().or_neighbours(ni, xi, _d(h2), _i(excl), int(include_self), _d(xj).shape[0],
                         _d(xj), _i(jindex), cap, counts, lists)
     return counts, lists.reshape(ni, max(cap, 1))
+
+
+def g4_force(xi, vi, eps2, h2, xj, vj, mj, cap: int = 1024):
+    """GRAPE-4 semantics (reference simulator pair loop, pot added): returns
+    acc, jerk, pot, counts, lists[ni, cap] (j addresses, -1 padded)."""
+    xi = _d(xi).reshape(-1, 3); ni = xi.shape[0]; nj = _d(xj).reshape(-1, 3).shape[0]
+    eps2 = _d(np.broadcast_to(np.asarray(eps2, dtype=np.float64), (ni,)))
+    h2 = _d(np.broadcast_to(np.asarray(h2, dtype=np.float64), (ni,)))
+    acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)
+    counts = np.zeros(ni, np.int32)
+    lists = np.full(ni * max(cap, 1), -1, np.int32)
+    lib().or_g4_force(ni, xi, _d(vi), eps2, h2, nj, _d(xj), _d(vj), _d(mj), acc, jerk, pot,
+                      max(cap, 1), counts, lists)
+    return acc, jerk, pot, counts, lists.reshape(ni, max(cap, 1))
 
 
 def hermite_shared(m, x, v, t_end, eta, eps2, dt_max=0.125, max_steps=10**8):

@@ -330,6 +330,55 @@ void or_neighbours(int ni, const double *xi, const double *h2, const int *excl,
 }
 
 /* ------------------------------------------------------------------------ */
+/* GRAPE-4 semantics: the pair loop of the reference's IEEE GRAPE-4          */
+/* simulator, src/harpsims/harp3fastsim.c:268-322 (per-i softening and       */
+/* neighbour radius doubled as at :223-228; a pair with r^2 = 0 is skipped,  */
+/* :301), in its operation order, plus pot = -sum m/r over the same pairs    */
+/* (the simulator leaves pot at 0).  Lists hold j addresses in ascending     */
+/* order, the i itself included when r^2 = 2 eps2 < 2 h2.                    */
+/* ------------------------------------------------------------------------ */
+
+void or_g4_force(int ni, const double *xi, const double *vi, const double *eps2,
+                 const double *h2, int nj, const double *xj, const double *vj,
+                 const double *mj, double *acc, double *jerk, double *pot,
+                 int cap, int *counts, int *lists)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ni; i++) {
+        const double e = 2.0 * eps2[i], h = 2.0 * h2[i];
+        const double x0 = xi[3*i], y0 = xi[3*i+1], z0 = xi[3*i+2];
+        const double u0 = vi[3*i], v0 = vi[3*i+1], w0 = vi[3*i+2];
+        double ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0, p = 0;
+        int c = 0;
+        for (int j = 0; j < nj; j++) {
+            double dx = xj[3*j] - x0, dy = xj[3*j+1] - y0, dz = xj[3*j+2] - z0;
+            double du = vj[3*j] - u0, dv = vj[3*j+1] - v0, dw = vj[3*j+2] - w0;
+            double r2 = e + dx * dx + dy * dy + dz * dz;
+            double rv = dx * du + dy * dv + dz * dw;
+            if (r2 < h) {
+                if (counts && c < cap) lists[(long)i * cap + c] = j;
+                c++;
+            }
+            if (r2 > 0.0) {
+                double r2i = 1.0 / r2;
+                double rinv = sqrt(r2i);
+                double r3i = mj[j] * r2i * rinv;
+                double q = 3.0 * rv * r2i;
+                ax += dx * r3i; ay += dy * r3i; az += dz * r3i;
+                jx += (du - dx * q) * r3i;
+                jy += (dv - dy * q) * r3i;
+                jz += (dw - dz * q) * r3i;
+                p -= mj[j] * rinv;
+            }
+        }
+        acc[3*i] = ax; acc[3*i+1] = ay; acc[3*i+2] = az;
+        jerk[3*i] = jx; jerk[3*i+1] = jy; jerk[3*i+2] = jz;
+        pot[i] = p;
+        if (counts) counts[i] = c;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Energies (each pair once).                                                */
 /* ------------------------------------------------------------------------ */
 

@@ -25,6 +25,9 @@
  *   or_neighbours                 neighbour sphere {j : r^2 < h2_i}
  *                                 (hdyn_grape6.C:2062-2066, 2302-2315)
  *   or_energies                   calculate_energies (src/node/dyn/util, hdyn_tt.C:932-972)
+ *   or_g4_force                   GRAPE-4 pair loop of the reference's IEEE simulator,
+ *                                 src/harpsims/harp3fastsim.c:223-228, 268-322
+ *                                 (acc, jerk, neighbour lists; pot added)
  */
 #ifndef G6_ORACLE_H
 #define G6_ORACLE_H
@@ -72,6 +75,16 @@ void or_force_ld(const double *xi, const double *vi, int excl,
 void or_neighbours(int ni, const double *xi, const double *h2, const int *excl,
                    int include_self, int nj, const double *xj, const int *jindex,
                    int cap, int *counts, int *lists);
+
+/* --- GRAPE-4 semantics ------------------------------------------------------ */
+/* Per i: r^2 = 2 eps2[i] + d^2 (left to right), pairs with r^2 = 0 skipped,
+ * acc, jerk and pot = -sum m/r; counts[i] / lists[i*cap + k] = ascending j
+ * addresses with r^2 < 2 h2[i] (counts may exceed cap; counts = NULL skips
+ * the lists). */
+void or_g4_force(int ni, const double *xi, const double *vi, const double *eps2,
+                 const double *h2, int nj, const double *xj, const double *vj,
+                 const double *mj, double *acc, double *jerk, double *pot,
+                 int cap, int *counts, int *lists);
 
 /* --- energies ---------------------------------------------------------------*/
 /* Pairwise potential (each pair once, softened) and kinetic energy. */

@@ -1,8 +1,10 @@
-"""ctypes prototypes of every entry point libg6.so exports.
+"""ctypes prototypes of every entry point libg6.so and libg4x.so export.
 
 ``GRAPE6`` lists the entry points of include/grape6.h (the reference ABI,
 include/grape6.h:18-102 of the Starlab tree); ``G6X`` those of include/g6x.h;
-``G6E`` those of include/g6ens.h (the ensemble engine).
+``G6E`` those of include/g6ens.h (the ensemble engine).  ``GRAPE4`` and
+``G4X`` are those of libg4x.so: include/grape4.h (reference
+include/grape4.h:22-45) and include/g4x.h.
 """
 from __future__ import annotations
 
@@ -71,9 +73,52 @@ G6E = {
 }
 
 
-def declare(lib: C.CDLL) -> None:
-    for table in (GRAPE6, G6X, G6E):
+# libg4x.so: the GRAPE-4 ABI (include/grape4.h; reference include/grape4.h:22-45)
+GRAPE4 = {
+    "h3open_": (None, []),
+    "h3close_": (None, []),
+    "h3npipe_": (_i, []),
+    "h3setnboards_": (None, [_pi]),
+    "h3getnboards_": (_i, []),
+    "h3wait_": (C.c_uint, []),
+    "h3setti_": (None, [_pd]),
+    "h3setmode_": (None, [_pi, _pi]),
+    "h3setled_": (None, [_pi, _pi]),
+    "h3jpmax_": (_i, []),
+    "h3jpdma_indirect_": (None, [_pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pi]),
+    "h3mjpdma_indirect_": (None, [_pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pi, _pi]),
+    "h3mjpdma_start_": (None, [_pi]),
+    "h3mjpdma_flush_": (None, []),
+    "h3calc_": (None, [_pi, _pi, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "h3calc_firsthalf_": (None, [_pi, _pi, _vp, _vp, _vp, _vp]),
+    "h3calc_lasthalf_": (None, [_pi, _vp, _vp, _vp]),
+    "h3nbread_": (None, [_pi]),
+    "h3nblist_": (_i, [_pi, _pi, _vp]),
+    "count_nblist_low": (_i, [_i, _i]),
+    "h3setdebuglevel_": (None, [_pi]),
+    "h3checkdmabuffers_": (None, []),
+    "set_time_check_mode": (None, [_i]),
+}
+
+G4X = {
+    "g4x_errors": (C.c_longlong, []),
+    "g4x_last_error": (C.c_char_p, []),
+    "g4x_set_option": (_i, [C.c_char_p, C.c_longlong]),
+    "g4x_get_option": (C.c_longlong, [C.c_char_p]),
+}
+
+
+def _apply(lib: C.CDLL, tables) -> None:
+    for table in tables:
         for name, (res, args) in table.items():
             f = getattr(lib, name)
             f.restype = res
             f.argtypes = args
+
+
+def declare(lib: C.CDLL) -> None:
+    _apply(lib, (GRAPE6, G6X, G6E))
+
+
+def declare_g4(lib: C.CDLL) -> None:
+    _apply(lib, (GRAPE4, G4X))

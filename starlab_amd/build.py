@@ -1,10 +1,16 @@
-"""Build libg6.so (HIP, gfx950) in-tree with hipcc.
+"""Build libg6.so and libg4x.so (HIP, gfx950) in-tree with hipcc.
 
     python -m starlab_amd.build [--asm DIR]
 
-The library is compiled for gfx950 only.  All HIP code is built with packed
-FP32 instructions disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``);
-the host pass warns that it does not know the feature, which is harmless.
+libg6.so  exports the GRAPE-6 ABI (include/grape6.h), its extensions
+          (include/g6x.h) and the ensemble engine (include/g6ens.h);
+libg4x.so exports the GRAPE-4 ABI (include/grape4.h) and include/g4x.h.
+Both link the same force kernels and engine core; each has its own engine
+state.  Symbols other than the declared C entry points are hidden.
+
+The code is compiled for gfx950 only, with packed FP32 instructions disabled
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``); the host pass warns
+that it does not know the feature, which is harmless.
 """
 from __future__ import annotations
 
@@ -13,16 +19,22 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libg6.so")
+LIBG4 = os.path.join(HERE, "libg4x.so")
 
-SOURCES = ["g6_kernels.hip", "g6_engine.hip", "g6_ens.hip"]
+LIBS = {
+    LIB: ["g6_kernels.hip", "g6_core.hip", "g6_engine.hip", "g6_ens.hip"],
+    LIBG4: ["g6_kernels.hip", "g6_core.hip", "g4_engine.hip"],
+}
 ARCH = "gfx950"
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
@@ -35,19 +47,35 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libg6")
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source into starlab_amd/libg6.so (always recompiles)."""
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, "-shared", "-o", tmp,
-           *[os.path.join(CSRC, s) for s in SOURCES],
-           f"-I{os.path.join(ROOT, 'include')}"]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libg6.so")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"hipcc failed: {' '.join(cmd[-3:])}")
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every source and link starlab_amd/libg6.so and libg4x.so
+    (always recompiles)."""
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sorted({s for v in LIBS.values() for s in v})
+    inc = f"-I{os.path.join(ROOT, 'include')}"
+
+    def obj(s):
+        return os.path.join(OBJ, s.replace(".hip", ".o"))
+
+    def compile_one(s):
+        _run([hipcc(), *HIP_FLAGS, "-c", "-o", obj(s), os.path.join(CSRC, s), inc], verbose)
+
+    with ThreadPoolExecutor(max_workers=min(4, len(srcs))) as ex:
+        list(ex.map(compile_one, srcs))
+    for lib, parts in LIBS.items():
+        tmp = lib + ".tmp"
+        _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+              *[obj(s) for s in parts]], verbose)
+        os.replace(tmp, lib)
     return LIB
 
 

@@ -1123,6 +1123,7 @@ int busy_count()
 using namespace g6;
 using namespace g6::ens;
 
+#pragma GCC visibility push(default)
 extern "C" {
 
 int g6e_open(long long hbm_budget_bytes, int max_systems)
@@ -1381,3 +1382,4 @@ long long g6e_get_option(const char *key)
 const char *g6e_last_error(void) { return X.err.c_str(); }
 
 }  // extern "C"
+#pragma GCC visibility pop

@@ -9,7 +9,7 @@
 //   double vp64[3][cap]      24 B/j  predicted FP64 velocity (read-back only)
 //   int    jindex[cap]        4 B/j  the `index` field of each address
 // i-block (npipes slots, SoA so each lane reads its own slot coalesced):
-//   IPred  10 arrays of float/int, FP64 positions 3 arrays, h2
+//   10 arrays of float/int, FP64 positions 3 arrays, h2 and the list offset e2
 // Partial sums of the j-split force: 7 FP64 arrays + 3 nn words, [split][i].
 #pragma once
 
@@ -80,10 +80,13 @@ hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
                              float *ibuf, int ld);
 // variant 0: scalar-load kernel (64 i per workgroup); 1: LDS-staged kernel
 // (512 i per workgroup).  force_igroup() gives the i per workgroup.
+// zskip: pairs whose FP32 r^2 is zero add nothing (a softening below the
+// smallest normal float then counts as zero).
 int force_igroup(int variant);
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
-                        int jchunk, int tile, const Partials &p, bool xcd_remap, int variant);
+                        int jchunk, int tile, const Partials &p, bool xcd_remap, int variant,
+                        bool zskip);
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          const JPred *jpred, int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
@@ -92,7 +95,7 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
 int reduce_lanes(int nsplit);
 hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,
                          const int *jindex, int nj, const IBlock &ib,
-                         const double *h2, const int *slots, int nslots,
+                         const double *h2, const double *e2, const int *slots, int nslots,
                          int include_self, int nb_cap, int *counts, int *lists);
 
 }  // namespace g6

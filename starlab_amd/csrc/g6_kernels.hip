@@ -175,7 +175,7 @@ __device__ __forceinline__ float ds_r2(float xjh, float yjh, float zjh, float xj
     return r2 + eps2;
 }
 
-template <bool MASK>
+template <bool MASK, bool ZSKIP>
 __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float xih,
                                      float yih, float zih, float xil, float yil,
                                      float zil, float vxi, float vyi, float vzi,
@@ -191,6 +191,7 @@ __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float
     const float rv = __builtin_fmaf(dx, dvx, __builtin_fmaf(dy, dvy, dz * dvz));
     float rinv = __builtin_amdgcn_rsqf(r2);
     float rnn = r2;
+    if (ZSKIP) rinv = r2 > 0.0f ? rinv : 0.0f;
     if (MASK) {
         const bool self = (unsigned)jj == rel;
         rinv = self ? 0.0f : rinv;
@@ -214,7 +215,7 @@ __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float
     a.best = __builtin_fminf(a.best, rnn);
 }
 
-template <bool MASK>
+template <bool MASK, bool ZSKIP>
 __device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
                                            unsigned rel, float xih, float yih,
                                            float zih, float xil, float yil,
@@ -224,15 +225,16 @@ __device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
     int jj = 0;
     for (; jj + 4 <= tn; jj += 4) {
         const JPred *g = jp + jj;
-        pair<MASK>(g[0], jj + 0, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
-        pair<MASK>(g[1], jj + 1, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
-        pair<MASK>(g[2], jj + 2, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
-        pair<MASK>(g[3], jj + 3, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK, ZSKIP>(g[0], jj + 0, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK, ZSKIP>(g[1], jj + 1, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK, ZSKIP>(g[2], jj + 2, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK, ZSKIP>(g[3], jj + 3, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
     }
     for (; jj < tn; ++jj)
-        pair<MASK>(jp[jj], jj, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
+        pair<MASK, ZSKIP>(jp[jj], jj, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
 }
 
+template <bool ZSKIP>
 __global__ __launch_bounds__(64) void k_force(
     const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
     int jchunk, int tile, int n_igroups, Partials P, int remap)
@@ -271,11 +273,11 @@ __global__ __launch_bounds__(64) void k_force(
         const unsigned rel = (unsigned)(excl - t);
         const bool mine = rel < (unsigned)tn;
         if (__ballot(mine) == 0)
-            tile_pairs<false>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
-                              vxi, vyi, vzi, eps2, a);
+            tile_pairs<false, ZSKIP>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
+                                     vxi, vyi, vzi, eps2, a);
         else
-            tile_pairs<true>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
-                             vxi, vyi, vzi, eps2, a);
+            tile_pairs<true, ZSKIP>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
+                                    vxi, vyi, vzi, eps2, a);
         A0 += (double)a.ax;
         A1 += (double)a.ay;
         A2 += (double)a.az;
@@ -322,7 +324,7 @@ struct IPos {
     float xh, yh, zh, xl, yl, zl, vx, vy, vz;
 };
 
-template <bool MASK, bool EPS>
+template <bool MASK, bool EPS, bool ZSKIP>
 __device__ __forceinline__ void pair_v(float jxh, float jyh, float jzh, float jxl,
                                        float jyl, float jzl, float jvx, float jvy,
                                        float jvz, float jm, int jj, unsigned rel,
@@ -341,6 +343,7 @@ __device__ __forceinline__ void pair_v(float jxh, float jyh, float jzh, float jx
     rv = __builtin_fmaf(dx, dvx, rv);
     float rinv = __builtin_amdgcn_rsqf(r2);
     float rnn = r2;
+    if (ZSKIP) rinv = r2 > 0.0f ? rinv : 0.0f;
     if (MASK) {
         const bool self = (unsigned)jj == rel;
         rinv = self ? 0.0f : rinv;
@@ -367,7 +370,7 @@ __device__ __forceinline__ void pair_v(float jxh, float jyh, float jzh, float jx
 constexpr int kLdsWG = 256;   // lanes per workgroup of the LDS kernel
 constexpr int kLdsI = 2;      // i-particles per lane
 
-template <bool M0, bool M1, bool EPS>
+template <bool M0, bool M1, bool EPS, bool ZSKIP>
 __device__ __forceinline__ void tile_lds(const float4 *__restrict__ sj, int tn,
                                          unsigned rel0, unsigned rel1, const IPos &I0,
                                          const IPos &I1, float eps2v, Acc32 &a0,
@@ -378,10 +381,10 @@ __device__ __forceinline__ void tile_lds(const float4 *__restrict__ sj, int tn,
         const float4 A = sj[3 * jj + 0];   // xh yh zh xl
         const float4 B = sj[3 * jj + 1];   // yl zl vx vy
         const float4 C = sj[3 * jj + 2];   // vz m - -
-        pair_v<M0, EPS>(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y, jj, rel0, I0,
-                        eps2v, a0);
-        pair_v<M1, EPS>(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y, jj, rel1, I1,
-                        eps2v, a1);
+        pair_v<M0, EPS, ZSKIP>(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y, jj, rel0,
+                               I0, eps2v, a0);
+        pair_v<M1, EPS, ZSKIP>(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y, jj, rel1,
+                               I1, eps2v, a1);
     }
 }
 
@@ -424,7 +427,7 @@ __device__ __forceinline__ IPos load_i(const IBlock &ib, int ic)
     return I;
 }
 
-template <bool EPS>
+template <bool EPS, bool ZSKIP>
 __global__ __launch_bounds__(256) void k_force_lds(
     const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
     int jchunk, int n_igroups, Partials P, int remap)
@@ -478,13 +481,13 @@ __global__ __launch_bounds__(256) void k_force_lds(
         const bool m0 = __ballot(rel0 < (unsigned)tn) != 0;
         const bool m1 = __ballot(rel1 < (unsigned)tn) != 0;
         if (!m0 && !m1)
-            tile_lds<false, false, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+            tile_lds<false, false, EPS, ZSKIP>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
         else if (m0 && !m1)
-            tile_lds<true, false, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+            tile_lds<true, false, EPS, ZSKIP>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
         else if (!m0 && m1)
-            tile_lds<false, true, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+            tile_lds<false, true, EPS, ZSKIP>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
         else
-            tile_lds<true, true, EPS>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
+            tile_lds<true, true, EPS, ZSKIP>(sj, tn, rel0, rel1, I0, I1, eps2v, a0, a1);
         acc_fold(A0, a0, t);
         acc_fold(A1, a1, t);
     }
@@ -511,21 +514,31 @@ int force_igroup(int variant) { return variant == 1 ? kLdsWG * kLdsI : 64; }
 
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
-                        int jchunk, int tile, const Partials &p, bool xcd_remap, int variant)
+                        int jchunk, int tile, const Partials &p, bool xcd_remap, int variant,
+                        bool zskip)
 {
     if (ni <= 0) return hipSuccess;
     const int n_igroups = (ni + force_igroup(variant) - 1) / force_igroup(variant);
     const int nwg = n_igroups * nsplit;
     const int remap = (xcd_remap && nwg % 8 == 0) ? 1 : 0;
+    // r^2 >= eps2 > 0 when eps2 is a normal float, so the zero-r^2 rule only
+    // needs its per-pair select without softening
+    zskip = zskip && !(eps2 >= __FLT_MIN__);
     if (variant == 1) {
-        if (eps2 != 0.0f)
-            hipLaunchKernelGGL(k_force_lds<true>, dim3(nwg), dim3(kLdsWG), 0, s, jpred, nj,
-                               ib, ni, eps2, jchunk, n_igroups, p, remap);
+        if (zskip)
+            hipLaunchKernelGGL((k_force_lds<false, true>), dim3(nwg), dim3(kLdsWG), 0, s, jpred,
+                               nj, ib, ni, 0.0f, jchunk, n_igroups, p, remap);
+        else if (eps2 != 0.0f)
+            hipLaunchKernelGGL((k_force_lds<true, false>), dim3(nwg), dim3(kLdsWG), 0, s, jpred,
+                               nj, ib, ni, eps2, jchunk, n_igroups, p, remap);
         else
-            hipLaunchKernelGGL(k_force_lds<false>, dim3(nwg), dim3(kLdsWG), 0, s, jpred, nj,
-                               ib, ni, eps2, jchunk, n_igroups, p, remap);
+            hipLaunchKernelGGL((k_force_lds<false, false>), dim3(nwg), dim3(kLdsWG), 0, s, jpred,
+                               nj, ib, ni, eps2, jchunk, n_igroups, p, remap);
+    } else if (zskip) {
+        hipLaunchKernelGGL(k_force<true>, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
+                           0.0f, jchunk, tile, n_igroups, p, remap);
     } else {
-        hipLaunchKernelGGL(k_force, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
+        hipLaunchKernelGGL(k_force<false>, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
                            eps2, jchunk, tile, n_igroups, p, remap);
     }
     return hipGetLastError();
@@ -740,10 +753,14 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
 // Neighbour lists (FP64, exact): one wave per listed i-slot; lanes sweep 64
 // consecutive j, ballot + prefix count keeps the list in ascending address.
 // counts[i] is the true count; at most nb_cap entries are stored.
+// The test is ((e2[i] + dx*dx) + dy*dy) + dz*dz < h2[i]: with e2 = 0 that is
+// the unsoftened d^2 < h2 of the GRAPE-6 front end (0 + a = a exactly); the
+// GRAPE-4 front end passes e2 = 2 eps2 and h2 = 2 h2, the operation order of
+// the reference's IEEE GRAPE-4 simulator (harp3fastsim.c:283-295).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_nblist(
     const double *__restrict__ xp64, int cap, const int *__restrict__ jindex,
-    int nj, IBlock ib, const double *__restrict__ h2,
+    int nj, IBlock ib, const double *__restrict__ h2, const double *__restrict__ e2,
     const int *__restrict__ slots, int nslots, int include_self, int nb_cap,
     int *__restrict__ counts, int *__restrict__ lists)
 {
@@ -753,6 +770,7 @@ __global__ __launch_bounds__(64) void k_nblist(
     const int i = slots[w];
     const int lane = threadIdx.x;
     const double hh = h2[i];
+    const double ee = e2[i];
     const double xi = ib.x64[i], yi = ib.y64[i], zi = ib.z64[i];
     const int excl = ib.excl[i];
     int cnt = 0;
@@ -764,7 +782,7 @@ __global__ __launch_bounds__(64) void k_nblist(
             const double dx = xp64[j] - xi;
             const double dy = xp64[(size_t)cap + j] - yi;
             const double dz = xp64[2 * (size_t)cap + j] - zi;
-            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double d2 = ((ee + dx * dx) + dy * dy) + dz * dz;
             in = d2 < hh && (include_self || j != excl);
         }
         const unsigned long long m = __ballot(in);
@@ -779,12 +797,12 @@ __global__ __launch_bounds__(64) void k_nblist(
 
 hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,
                          const int *jindex, int nj, const IBlock &ib,
-                         const double *h2, const int *slots, int nslots,
+                         const double *h2, const double *e2, const int *slots, int nslots,
                          int include_self, int nb_cap, int *counts, int *lists)
 {
     if (nslots <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_nblist, dim3(nslots), dim3(64), 0, s, xp64, cap, jindex,
-                       nj, ib, h2, slots, nslots, include_self, nb_cap, counts,
+                       nj, ib, h2, e2, slots, nslots, include_self, nb_cap, counts,
                        lists);
     return hipGetLastError();
 }

@@ -1,0 +1,287 @@
+"""GPU parity of libg4x.so (the GRAPE-4 ABI on the libg6 kernels) through its
+C ABI, against the GRAPE-4 oracle (oracle.g4_force, itself pinned bit for bit
+to the reference's IEEE GRAPE-4 simulator) and, where oracle/_ref was built,
+against that simulator directly.
+
+Bars: acc and jerk normwise error (|delta| over the sum of the pair-term
+magnitudes) <= 1e-6, pot relative error <= 1e-6, neighbour lists equal
+(membership and order).  The calls are those of kira's GRAPE-4 driver
+(hdyn_grape4.C:240-420): DMA buffers, block particles re-sent predicted to
+t_i, USE_HALF_CHIP pipe pairs padded with copies of the last particle,
+eps2 = 0 with the i-particles themselves in j-memory.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.gpu_common import ACC_TOL, JERK_TOL, POT_TOL, plummer_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g():
+    from starlab_amd.g4 import GRAPE4
+    dev = GRAPE4(exit_on_error=False)
+    dev.open()
+    assert dev.errors() == 0, dev.last_error()
+    yield dev
+    dev.close()
+
+
+@pytest.fixture(autouse=True)
+def no_new_errors(g, request):
+    """Every call of a test succeeds, except in the test of the error count."""
+    before = g.errors()
+    yield
+    if "is_counted" not in request.node.name:
+        assert g.errors() == before, g.last_error()
+
+
+def _errors(acc, jerk, pot, racc, rjerk, rpot, xi, vi, xj, vj, mj, eps2):
+    """Normwise acc/jerk and relative pot errors; pairs with r^2 = 0 excluded
+    from the term scales as from the sums."""
+    n = len(xi)
+    eps2 = np.broadcast_to(np.asarray(eps2, float), (n,))
+    sa = np.zeros(n)
+    sj = np.zeros(n)
+    for k in range(n):
+        d = xj - xi[k]
+        w = vj - vi[k]
+        r2 = (d * d).sum(1) + 2 * eps2[k]
+        r2[r2 == 0] = np.inf
+        r = np.sqrt(r2)
+        sa[k] = (mj / r2).sum()
+        sj[k] = (mj * (np.sqrt((w * w).sum(1)) * 4.0) / (r2 * r)).sum()
+    ea = np.linalg.norm(acc - racc, axis=1)
+    ej = np.linalg.norm(jerk - rjerk, axis=1)
+    return dict(acc_norm=float((ea / sa).max()), jerk_norm=float((ej / sj).max()),
+                pot_rel=float((np.abs(pot - rpot) / np.abs(rpot)).max()))
+
+
+def _check(e, label):
+    print(label, e, flush=True)
+    assert e["acc_norm"] <= ACC_TOL, e
+    assert e["jerk_norm"] <= JERK_TOL, e
+    assert e["pot_rel"] <= POT_TOL, e
+
+
+def _load(g, s):
+    g.load_all(s["x"], s["v"], s["a2"], s["j6"], s["m"], s["tj"])
+
+
+def _predict(s, ti):
+    z = np.zeros_like(s["x"])
+    return oracle.predict(s["x"], s["v"], s["a2"], s["j6"], z, s["tj"], ti, 4)
+
+
+def _kira_block(g, s, block, ti):
+    """Re-send the block's particles predicted to ti (tj = ti), as
+    jpdma_nodes(n_top, next_top, true, time) does; returns the j-memory
+    positions/velocities the device then predicts."""
+    xp, vp = _predict(s, ti)
+    x = s["x"].copy(); v = s["v"].copy(); tj = s["tj"].copy()
+    x[block] = xp[block]; v[block] = vp[block]; tj[block] = ti
+    g.mjpdma_indirect(block + 1, x, v, s["a2"], s["j6"], s["m"], tj, 0)
+    g.mjpdma_start(0)
+    s2 = dict(s, x=x, v=v, tj=tj)
+    return _predict(s2, ti)
+
+
+def test_kira_half_chip_block_eps0(g):
+    """Config-1 shape (Plummer N = 1024, eps = 0): a block of 300 particles
+    in USE_HALF_CHIP pipe pairs, padded to h3npipe_() with the last one."""
+    s = plummer_state(1024, seed=42)
+    _load(g, s)
+    ti = 2.0 ** -12
+    rng = np.random.default_rng(1)
+    block = np.sort(rng.choice(1024, 300, replace=False)).astype(np.int32)
+    xp, vp = _kira_block(g, s, block, ti)
+    npipe = g.npipe()
+    pipes = np.concatenate([np.repeat(block, 2),
+                            np.full(npipe - 2 * len(block), block[-1])])
+    g.setti(ti)
+    h2 = np.zeros(npipe)
+    g.calc_firsthalf(1024, xp[pipes], vp[pipes], np.zeros(npipe), h2)
+    assert g.wait() == 0
+    acc, jerk, pot = g.calc_lasthalf(npipe)
+    assert g.option("last_slots") == len(block)
+    racc, rjerk, rpot, _, _ = oracle.g4_force(xp[block], vp[block], 0.0, 0.0, xp, vp, s["m"])
+    for k in (0, 1):
+        _check(_errors(acc[k::2][:len(block)], jerk[k::2][:len(block)], pot[k::2][:len(block)],
+                       racc, rjerk, rpot, xp[block], vp[block], xp, vp, s["m"], 0.0),
+               f"half-chip pipe {k}")
+    # the padding pipes carry the last particle's results, bit for bit
+    tail = slice(2 * len(block), npipe)
+    np.testing.assert_array_equal(acc[tail], np.broadcast_to(acc[2 * len(block) - 1], acc[tail].shape))
+    np.testing.assert_array_equal(pot[tail], pot[2 * len(block) - 1])
+
+
+def test_dedupe_off_gives_identical_results(g):
+    """32 particles in 64 pipes: one wave of i either way, so the force
+    geometry is the same and computing each pipe separately gives the same
+    bits."""
+    s = plummer_state(2000, seed=9)
+    _load(g, s)
+    xp, vp = _predict(s, 0.0)
+    g.setti(0.0)
+    pipes = np.repeat(np.arange(0, 2000, 63)[:32], 2)
+    outs = []
+    for d in (1, 0):
+        g.option("dedupe", d)
+        outs.append(g.calc(2000, xp[pipes], vp[pipes], 1e-6, 0.0))
+    g.option("dedupe", 1)
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_per_pipe_softening_groups(g):
+    """Three softening values interleaved across the pipes: the kernels run one
+    launch per value; each pipe matches the oracle with its own eps2."""
+    s = plummer_state(3000, seed=4)
+    _load(g, s)
+    ti = 2.0 ** -11
+    xp, vp = _predict(s, ti)
+    sel = np.arange(0, 3000, 11)
+    eps2 = np.array([1e-4, 0.0, 1e-6])[np.arange(len(sel)) % 3]
+    g.setti(ti)
+    g.calc_firsthalf(3000, xp[sel], vp[sel], eps2, 0.0)
+    acc, jerk, pot = g.calc_lasthalf(len(sel))
+    assert g.option("last_groups") == 3
+    racc, rjerk, rpot, _, _ = oracle.g4_force(xp[sel], vp[sel], eps2, 0.0, xp, vp, s["m"])
+    _check(_errors(acc, jerk, pot, racc, rjerk, rpot, xp[sel], vp[sel], xp, vp, s["m"], eps2),
+           "softening groups")
+
+
+def test_neighbour_lists_exact_and_complete(g):
+    """Lists {j : 2 eps2 + d^2 < 2 h2} in ascending address, the i itself
+    included, equal to the oracle's; a store of 4 entries per pipe grows so
+    that every list comes back whole."""
+    s = plummer_state(4096, seed=12)
+    _load(g, s)
+    xp, vp = _predict(s, 0.0)
+    sel = np.arange(0, 4096, 5)
+    rng = np.random.default_rng(3)
+    h2 = rng.uniform(0.0, 0.02, len(sel))
+    h2[::7] = 0.0
+    eps2 = np.where(np.arange(len(sel)) % 2 == 0, 0.0, 1e-5)
+    old_cap = g.option("nb_cap")
+    g.option("nb_cap", 4)
+    try:
+        g.setti(0.0)
+        g.calc_firsthalf(4096, xp[sel], vp[sel], eps2, h2)
+        g.calc_lasthalf(len(sel))
+        g.nbread()
+        _, _, _, cnt, lists = oracle.g4_force(xp[sel], vp[sel], eps2, h2, xp, vp, s["m"],
+                                              cap=4096)
+        assert cnt.max() > 4
+        for k in range(len(sel)):
+            got = g.nblist(k, 4096)
+            np.testing.assert_array_equal(got, lists[k, :cnt[k]])
+            assert g.count_nblist(k) == cnt[k]
+            if h2[k] > eps2[k]:
+                assert sel[k] in got          # the i itself
+    finally:
+        g.option("nb_cap", old_cap)
+
+
+def test_dma_buffers_snapshot_and_flush(g):
+    """A DMA buffer holds the caller's values at h3mjpdma_indirect_ time;
+    j-memory changes only when the buffer is started; buffers taken with
+    buff_id < 0 are written by h3mjpdma_flush_."""
+    s = plummer_state(512, seed=21)
+    _load(g, s)
+    xp0, vp0 = _predict(s, 0.0)
+    x = s["x"].copy()
+    x2 = x + 0.125
+    g.mjpdma_indirect(np.arange(1, 257, dtype=np.int32), x2, s["v"], s["a2"], s["j6"], s["m"],
+                      s["tj"], 3)
+    x2[:] = 99.0                                  # later caller writes are not seen
+    sel = np.arange(256, 512)
+    g.setti(0.0)
+    a_before, _, _ = g.calc(512, xp0[sel], vp0[sel], 1e-6, 0.0)
+    g.mjpdma_start(3)
+    s2 = dict(s, x=x + np.where(np.arange(512)[:, None] < 256, 0.125, 0.0))
+    xp2, vp2 = _predict(s2, 0.0)
+    a_after, j_after, p_after = g.calc(512, xp0[sel], vp0[sel], 1e-6, 0.0)
+    r0 = oracle.g4_force(xp0[sel], vp0[sel], 1e-6, 0.0, xp0, vp0, s["m"])
+    r2 = oracle.g4_force(xp0[sel], vp0[sel], 1e-6, 0.0, xp2, vp2, s["m"])
+    assert np.abs(a_before - r0[0]).max() < 1e-5 * np.abs(r0[0]).max()
+    _check(_errors(a_after, j_after, p_after, r2[0], r2[1], r2[2], xp0[sel], vp0[sel], xp2, vp2,
+                   s["m"], 1e-6), "after start")
+    # flush: two auto-numbered buffers restore the original positions
+    half = np.arange(1, 129, dtype=np.int32)
+    for idx in (half, half + 128):
+        g.mjpdma_indirect(idx, s["x"], s["v"], s["a2"], s["j6"], s["m"], s["tj"], -1)
+    g.mjpdma_flush()
+    a_back, _, _ = g.calc(512, xp0[sel], vp0[sel], 1e-6, 0.0)
+    np.testing.assert_array_equal(a_back, a_before)
+
+
+def test_close_reopen_keeps_j_memory(g):
+    s = plummer_state(700, seed=2)
+    _load(g, s)
+    xp, vp = _predict(s, 0.0)
+    g.setti(0.0)
+    before = g.calc(700, xp[:50], vp[:50], 1e-6, 0.0)
+    g.close()
+    g.open()
+    g.setti(0.0)
+    after = g.calc(700, xp[:50], vp[:50], 1e-6, 0.0)
+    for a, b in zip(before, after):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_lasthalf_without_firsthalf_is_counted(g):
+    before = g.errors()
+    g.calc_lasthalf(1)
+    assert g.errors() == before + 1
+    assert "no calculation" in g.last_error()
+
+
+@pytest.mark.skipif(not oracle.Harp3Sim.available(), reason="oracle/_ref not built")
+def test_against_reference_simulator_directly(g):
+    """libg4x and the reference's own GRAPE-4 simulator on the same calls
+    (<= 1000 j, 3 pipes per call, tj spread so the predictor is exercised)."""
+    s = plummer_state(997, seed=8)
+    sim = oracle.Harp3Sim()
+    sim.set_j(s["x"], s["v"], s["a2"], s["j6"], s["m"], s["tj"])
+    _load(g, s)
+    ti = 2.0 ** -12
+    g.setti(ti)
+    for first in (0, 500, 994):
+        sel = np.arange(first, first + 3)
+        xp, vp = _predict(s, ti)
+        h2 = np.array([0.002, 0.01, 0.0])
+        acc_s, jerk_s, lists_s = sim.calc(ti, 997, xp[sel], vp[sel], 2 * 1e-6, 2 * h2)
+        g.calc_firsthalf(997, xp[sel], vp[sel], 1e-6, h2)
+        acc, jerk, pot = g.calc_lasthalf(3)
+        g.nbread()
+        rp = oracle.g4_force(xp[sel], vp[sel], 1e-6, h2, xp, vp, s["m"])[2]
+        _check(_errors(acc, jerk, pot, acc_s, jerk_s, rp, xp[sel], vp[sel], xp, vp, s["m"], 1e-6),
+               f"vs simulator at {first}")
+        for k in range(3):
+            np.testing.assert_array_equal(g.nblist(k, 997), lists_s[k])
+
+
+def test_full_size_65536_sample(g):
+    """The force-only workload's size (Plummer N = 65536, t_j spread over
+    2^-10): one full 2048-pipe half-chip call (1024 particles) checked on a
+    64-particle sample, and a sum rule over the whole call: the pipe pairs
+    agree bit for bit."""
+    s = plummer_state(65536, seed=65536)
+    _load(g, s)
+    ti = 0.0
+    xp, vp = _predict(s, ti)
+    rng = np.random.default_rng(0)
+    block = np.sort(rng.choice(65536, 1024, replace=False))
+    pipes = np.repeat(block, 2)
+    g.setti(ti)
+    g.calc_firsthalf(65536, xp[pipes], vp[pipes], 1e-8 / 2, 0.0)
+    acc, jerk, pot = g.calc_lasthalf(len(pipes))
+    np.testing.assert_array_equal(acc[0::2], acc[1::2])
+    samp = rng.choice(1024, 64, replace=False)
+    racc, rjerk, rpot, _, _ = oracle.g4_force(xp[block[samp]], vp[block[samp]], 1e-8 / 2, 0.0,
+                                              xp, vp, s["m"])
+    _check(_errors(acc[2 * samp], jerk[2 * samp], pot[2 * samp], racc, rjerk, rpot,
+                   xp[block[samp]], vp[block[samp]], xp, vp, s["m"], 1e-8 / 2), "N=65536 sample")
