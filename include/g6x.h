@@ -30,6 +30,28 @@ int g6x_set_j_particles(int cluster_id, int n, const int *address,
 int g6x_self_sweep(int cluster_id, int nj, int i_first, int ni, double ti,
                    double eps2);
 
+/* Potential only (the fast path for kira's energy computation,
+ * grape6_calculate_energies, hdyn_grape6.C:1403-1546, which reads only pot
+ * from its pot_only force calls, :477-551, :766-767): pot[k] = -sum_j m_j/r
+ * over j in [0, nj) for the j-particle at address i_first + k, all at their
+ * device-predicted positions for time ti (ti = 0 with tj = 0 is kira's
+ * unpredicted energy snapshot), itself excluded, softened with eps2.
+ * Same double-single / FP32 pair arithmetic and FP64 sums as the force
+ * path, about 2.5x fewer instructions per pair.  Synchronous. */
+int g6x_potential(int cluster_id, int nj, int i_first, int ni, double ti, double eps2,
+                  double *pot);
+
+/* Local densities (the fast path for grape6_calculate_densities,
+ * hdyn_grape6.C:4283-4449, which iterates neighbour radii until each list
+ * holds 12 stars and then calls compute_density,
+ * src/node/dyn/util/compute_density.C:69-226): for the j-particle at
+ * address i_first + q, dk2[q] = squared distance to its k-th nearest other
+ * j (FP64, exact) and density[q] = (sum of the masses of its k-1 nearest) /
+ * (4/3 pi dk2^1.5), 0 when dk2 = 0.  2 <= k <= 32 (kira uses 12), nj > k.
+ * Either output may be NULL.  Synchronous. */
+int g6x_densities(int cluster_id, int nj, int i_first, int ni, double ti, int k,
+                  double *density, double *dk2);
+
 /* Copy the results of the last g6x_self_sweep (ni entries) to the host. */
 int g6x_read_results(int cluster_id, int ni, double *acc, double *jerk,
                      double *pot, int *nnbindex);

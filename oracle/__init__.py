@@ -55,6 +55,7 @@ def lib():
                                     _dp, _ip, C.c_int, _ip, _ip]
         L.or_g4_force.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
                                   _dp, _dp, _dp, C.c_int, _ip, _ip]
+        L.or_density.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.or_energies.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_hermite_shared.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
@@ -137,6 +138,16 @@ def g4_force(xi, vi, eps2, h2, xj, vj, mj, cap: int = 1024):
     lib().or_g4_force(ni, xi, _d(vi), eps2, h2, nj, _d(xj), _d(vj), _d(mj), acc, jerk, pot,
                       max(cap, 1), counts, lists)
     return acc, jerk, pot, counts, lists.reshape(ni, max(cap, 1))
+
+
+def density(m, x, k: int = 12, i_first: int = 0, ni: int | None = None):
+    """compute_density (whole-system mode): (density, dk2) of particles
+    i_first .. i_first+ni-1."""
+    x = _d(x).reshape(-1, 3); n = x.shape[0]
+    ni = n - i_first if ni is None else ni
+    rho = np.zeros(ni); dk2 = np.zeros(ni)
+    lib().or_density(n, _d(m), x, i_first, ni, k, rho, dk2)
+    return rho, dk2
 
 
 def hermite_shared(m, x, v, t_end, eta, eps2, dt_max=0.125, max_steps=10**8):

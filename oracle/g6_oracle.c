@@ -332,8 +332,8 @@ void or_neighbours(int ni, const double *xi, const double *h2, const int *excl,
 /* ------------------------------------------------------------------------ */
 /* GRAPE-4 semantics: the pair loop of the reference's IEEE GRAPE-4          */
 /* simulator, src/harpsims/harp3fastsim.c:268-322 (per-i softening and       */
-/* neighbour radius doubled as at :223-228; a pair with r^2 = 0 is skipped,  */
-/* :301), in its operation order, plus pot = -sum m/r over the same pairs    */
+/* neighbour radius doubled as at :240-241; a pair with r^2 = 0 is skipped,  */
+/* :304), in its operation order, plus pot = -sum m/r over the same pairs    */
 /* (the simulator leaves pot at 0).  Lists hold j addresses in ascending     */
 /* order, the i itself included when r^2 = 2 eps2 < 2 h2.                    */
 /* ------------------------------------------------------------------------ */
@@ -375,6 +375,57 @@ void or_g4_force(int ni, const double *xi, const double *vi, const double *eps2,
         jerk[3*i] = jx; jerk[3*i+1] = jy; jerk[3*i+2] = jz;
         pot[i] = p;
         if (counts) counts[i] = c;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Local density from the k-th nearest neighbour: compute_density,           */
+/* src/node/dyn/util/compute_density.C:69-226, whole-system mode (every      */
+/* other particle in ascending order, :155-189).  d[0] = 0 stands for the    */
+/* particle itself; a new distance goes after equal ones; density = masses   */
+/* of neighbours 1..k-1 over (4/3) pi d_k^3.                                 */
+/* ------------------------------------------------------------------------ */
+
+#define OR_VERY_LARGE 1.0e300      /* VERY_LARGE_NUMBER, include/stdinc.h:206 */
+#define OR_PI 3.14159265358979323846
+
+void or_density(int n, const double *m, const double *x, int i_first, int ni, int k,
+                double *density, double *dk2)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int ii = 0; ii < ni; ii++) {
+        const int i = i_first + ii;
+        double nd[64], nm[64];
+        nd[0] = 0.0; nm[0] = m[i];
+        for (int q = 1; q <= k; q++) { nd[q] = OR_VERY_LARGE; nm[q] = 0.0; }
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            double dx = x[3*i] - x[3*j], dy = x[3*i+1] - x[3*j+1], dz = x[3*i+2] - x[3*j+2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 < nd[k])) continue;
+            for (int q = k - 1; q >= 0; q--) {
+                if (d2 >= nd[q]) {
+                    for (int r = k; r > q + 1; r--) { nd[r] = nd[r-1]; nm[r] = nm[r-1]; }
+                    nd[q+1] = d2; nm[q+1] = m[j];
+                    break;
+                }
+            }
+        }
+        double rho = 0.0;
+        int kk = k;
+        if (nd[kk] > 0) {
+            while (kk > 0 && nd[kk] >= OR_VERY_LARGE) kk--;
+            if (kk > 1 && nd[kk] > 0 && nd[kk] < OR_VERY_LARGE) {
+                double volume = (4.0 / 3.0) * OR_PI * pow(nd[kk], 1.5);
+                if (volume > 0) {
+                    double mass = 0;
+                    for (int q = 1; q < kk; q++) mass += nm[q];
+                    rho = mass / volume;
+                }
+            }
+        }
+        density[ii] = rho;
+        dk2[ii] = nd[k];
     }
 }
 

@@ -25,8 +25,10 @@
  *   or_neighbours                 neighbour sphere {j : r^2 < h2_i}
  *                                 (hdyn_grape6.C:2062-2066, 2302-2315)
  *   or_energies                   calculate_energies (src/node/dyn/util, hdyn_tt.C:932-972)
+ *   or_density                    compute_density, src/node/dyn/util/compute_density.C:69-226
+ *                                 (k-th nearest neighbour density, whole-system mode)
  *   or_g4_force                   GRAPE-4 pair loop of the reference's IEEE simulator,
- *                                 src/harpsims/harp3fastsim.c:223-228, 268-322
+ *                                 src/harpsims/harp3fastsim.c:240-241, 268-322
  *                                 (acc, jerk, neighbour lists; pot added)
  */
 #ifndef G6_ORACLE_H
@@ -85,6 +87,13 @@ void or_g4_force(int ni, const double *xi, const double *vi, const double *eps2,
                  const double *h2, int nj, const double *xj, const double *vj,
                  const double *mj, double *acc, double *jerk, double *pot,
                  int cap, int *counts, int *lists);
+
+/* --- densities ------------------------------------------------------------- */
+/* For particles i_first .. i_first+ni-1 of the n (x AoS): dk2 = squared
+ * distance to the k-th nearest other particle (k < 64), density = masses of
+ * the k-1 nearest / (4/3 pi dk2^1.5). */
+void or_density(int n, const double *m, const double *x, int i_first, int ni, int k,
+                double *density, double *dk2);
 
 /* --- energies ---------------------------------------------------------------*/
 /* Pairwise potential (each pair once, softened) and kinetic energy. */

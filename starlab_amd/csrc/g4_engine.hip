@@ -9,18 +9,18 @@
 // per i-block.  configure probes libg4x for h3open_ only when no GRAPE-6
 // library is found, and then defines USE_HALF_CHIP (configure.ac:214-221): kira
 // puts each i-particle in two adjacent pipes and pads the block to
-// h3npipe_() slots with copies of the last particle (hdyn_grape4.C:349-385).
+// h3npipe_() slots with copies of the last particle (hdyn_grape4.C:353-384).
 //
 // Arithmetic semantics follow the reference's IEEE GRAPE-4 simulator
 // (src/harpsims/harp3fastsim.c):
 //   - j-memory address = hostindex - 1; a DMA buffer holds a snapshot of the
-//     caller's arrays, written to j-memory by h3mjpdma_start_ (:125-201);
+//     caller's arrays, written to j-memory by h3mjpdma_start_ (:140-201);
 //   - predictor x + s(v + s(a2 + s j6)), v + 2s(a2 + 1.5 s j6), s = ti - tj
-//     (:251-265), aj = a/2 and jj = j/6 as kira stores them (:258-259);
-//   - eps2 and h2 are doubled internally (:223-228): r^2 = 2 eps2 + d^2;
-//   - a pair with r^2 = 0 (the i itself when eps2 = 0) adds nothing (:301);
+//     (:251-265), aj = a/2 and jj = j/6 as kira stores them (hdyn_grape4.C:263-264);
+//   - eps2 and h2 are doubled internally (:240-241): r^2 = 2 eps2 + d^2;
+//   - a pair with r^2 = 0 (the i itself when eps2 = 0) adds nothing (:304);
 //   - neighbour list of pipe i = {j : r^2 < 2 h2[i]} in ascending address,
-//     the i itself included (:294-297), read back per pipe by h3nblist_;
+//     the i itself included (:300-303), read back per pipe by h3nblist_;
 //   - pot is not computed by the simulator; here pot = -sum m_j / r over the
 //     same pairs, the sign of kira's own potential (hdyn_ev.C:2031).
 // Pair arithmetic is the libg6 law: FP32 on double-single positions with
@@ -33,7 +33,7 @@
 //
 // Errors: the ABI's calls return nothing, so a failing call prints
 // "libg4x: <reason>" and exits the process with status 1 (the reference
-// simulator likewise exits on out-of-range sizes, harp3fastsim.c:135-138,
+// simulator likewise exits on out-of-range sizes, harp3fastsim.c:155-158,
 // 233-236); the option "exit_on_error" = 0 (env G4_EXIT_ON_ERROR=0) keeps the
 // process alive and counts the failures instead (g4x_errors()).
 #include "g6_core.h"
@@ -45,6 +45,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace g6 {
@@ -177,15 +178,21 @@ int firsthalf(int nj, int ni, const double (*xi)[3], const double (*vi)[3],
         }
     }
     const int nu = (int)first_pipe.size();
-    // group distinct slots by softening (first-appearance order of values)
+    // group distinct slots by softening (first-appearance order of values);
+    // each group is one force launch
     std::vector<double> values;
     std::vector<int> group_of(nu);
+    std::unordered_map<uint64_t, int> group_of_bits;
     for (int u = 0; u < nu; ++u) {
         const double e = eps2[first_pipe[u]];
-        int g = 0;
-        while (g < (int)values.size() && memcmp(&values[g], &e, sizeof e) != 0) ++g;
-        if (g == (int)values.size()) values.push_back(e);
-        group_of[u] = g;
+        uint64_t bits;
+        memcpy(&bits, &e, sizeof bits);
+        auto it = group_of_bits.find(bits);
+        if (it == group_of_bits.end()) {
+            it = group_of_bits.emplace(bits, (int)values.size()).first;
+            values.push_back(e);
+        }
+        group_of[u] = it->second;
     }
     std::vector<int> count(values.size(), 0), start(values.size(), 0);
     for (int u = 0; u < nu; ++u) ++count[group_of[u]];

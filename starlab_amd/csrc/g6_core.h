@@ -26,7 +26,9 @@ namespace g6 {
 const char *lib_tag();      // "libg6" / "libg4x"
 const char *env_prefix();   // "G6_" / "G4_"
 
-enum KernelId { K_PREDICT = 0, K_FORCE, K_REDUCE, K_REFINE, K_NBLIST, K_SCATTER, K_COUNT };
+enum KernelId {
+    K_PREDICT = 0, K_FORCE, K_REDUCE, K_REFINE, K_NBLIST, K_SCATTER, K_POT, K_KNN, K_COUNT
+};
 
 struct Options {
     int predict_order = 4;
@@ -205,6 +207,12 @@ void reset_state();
 
 // device-resident sweeps and the autotuner
 int self_block(int nj, int i_first, int ni, IBlock &ib);
+// pot of the j-particles [i_first, i_first + ni) against [0, nj) at time ti
+// (self excluded), to the host array pot[ni]
+int potential_sweep(int nj, int i_first, int ni, double ti, double eps2, double *pot);
+// k-th nearest-neighbour distance^2 and density (compute_density) of the
+// j-particles [i_first, i_first + ni) among [0, nj) at time ti
+int density_sweep(int nj, int i_first, int ni, double ti, int k, double *density, double *dk2);
 int autotune(int nj, int ni_max, double eps2);
 int save_tuning(const char *path);
 int load_tuning(const char *path);

@@ -769,6 +769,97 @@ int self_block(int nj, int i_first, int ni, IBlock &ib)
     return 0;
 }
 
+// j-split of a sweep with n_iw waves of i against nj j in tiles of `tile`:
+// about waves_target waves in all (the built-in rule of choose_geometry).
+void sweep_split(int n_iw, int nj, int tile, int &nsplit, int &jchunk)
+{
+    const int target = std::max(1, E.opt.waves_target);
+    const int ntiles = std::max(1, (nj + tile - 1) / tile);
+    int sp = E.opt.nsplit > 0 ? E.opt.nsplit : std::max(1, (target + n_iw - 1) / n_iw);
+    sp = std::min(sp, ntiles);
+    jchunk = ((ntiles + sp - 1) / sp) * tile;
+    nsplit = std::max(1, (nj + jchunk - 1) / jchunk);
+}
+
+int sweep_begin(const char *who, int nj, int i_first, int ni, double ti)
+{
+    if (ensure_open(who)) return -1;
+    if (ni <= 0 || i_first < 0 || i_first + ni > nj || nj <= 0)
+        return fail(-1, "%s: bad range i_first=%d ni=%d nj=%d", who, i_first, ni, nj);
+    if (E.calc_pending) HIPCHK(hipStreamSynchronize(E.stream));
+    E.calc_pending = false;
+    E.nb_valid = false;
+    E.calc_failed = true;   // a sweep leaves no i-block for neighbour lists
+    E.ti = ti;
+    if (flush_j(nj)) return -1;
+    if (predict(nj, false)) return -1;
+    return ensure_i(ni);
+}
+
+int potential_sweep(int nj, int i_first, int ni, double ti, double eps2, double *pot)
+{
+    if (sweep_begin("g6x_potential", nj, i_first, ni, ti)) return -1;
+    IBlock ib;
+    if (self_block(nj, i_first, ni, ib)) return -1;
+    const int n_iw = (ni + 63) / 64;
+    const int tile = (long long)n_iw * ((nj + kTile - 1) / kTile) >= E.opt.waves_target ? kTile : 64;
+    int nsplit, jchunk;
+    sweep_split(n_iw, nj, tile, nsplit, jchunk);
+    if (ensure_partials((size_t)nsplit * ib.ld * sizeof(double))) return -1;
+    const Results r = results_at(E.d_res, ni);
+    {
+        KTimer t(K_POT);
+        HIPCHK(launch_pot(E.stream, E.d_jpred, nj, ib, ni, (float)eps2, nsplit, jchunk, tile,
+                          reinterpret_cast<double *>(E.d_part), r.pot, E.opt.xcd_remap != 0));
+        t.stop();
+    }
+    const Results h = results_at(E.h_res, ni);
+    HIPCHK(hipMemcpyAsync(h.pot, r.pot, sizeof(double) * (size_t)ni, hipMemcpyDeviceToHost,
+                          E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    harvest_timing();
+    memcpy(pot, h.pot, sizeof(double) * (size_t)ni);
+    E.last_ni = ni;
+    E.last_nj = nj;
+    return 0;
+}
+
+int density_sweep(int nj, int i_first, int ni, double ti, int k, double *density, double *dk2)
+{
+    if (k < 2 || k > 32) return fail(-1, "g6x_densities: k = %d outside [2, 32]", k);
+    if (nj - 1 < k) return fail(-1, "g6x_densities: k = %d but only %d other particles", k, nj - 1);
+    if (sweep_begin("g6x_densities", nj, i_first, ni, ti)) return -1;
+    const int ld = (ni + 63) & ~63;
+    const int n_iw = (ni + 63) / 64;
+    // j-splits of at least 256 j, at most 64 of them
+    int nsplit = std::max(1, (std::max(1, E.opt.waves_target) + n_iw - 1) / n_iw);
+    nsplit = std::min({nsplit, std::max(1, nj / 256), 64});
+    const int jchunk = (nj + nsplit - 1) / nsplit;
+    nsplit = (nj + jchunk - 1) / jchunk;
+    const int kmax = knn_kmax(k);
+    const size_t entries = (size_t)nsplit * kmax * ld;
+    if (ensure_partials(entries * (sizeof(double) + sizeof(int)))) return -1;
+    double *pd = reinterpret_cast<double *>(E.d_part);
+    int *pa = reinterpret_cast<int *>(pd + entries);
+    const Results r = results_at(E.d_res, ni);   // acc area: density[ni], dk2[ni]
+    {
+        KTimer t(K_KNN);
+        HIPCHK(launch_knn(E.stream, E.d_xp64, E.dcap, nj, i_first, ni, k, nsplit, jchunk, pd, pa,
+                          ld, E.d_jraw, r.acc, r.acc + ni));
+        t.stop();
+    }
+    const Results h = results_at(E.h_res, ni);
+    HIPCHK(hipMemcpyAsync(h.acc, r.acc, sizeof(double) * 2 * (size_t)ni, hipMemcpyDeviceToHost,
+                          E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    harvest_timing();
+    if (density) memcpy(density, h.acc, sizeof(double) * (size_t)ni);
+    if (dk2) memcpy(dk2, h.acc + ni, sizeof(double) * (size_t)ni);
+    E.last_ni = ni;
+    E.last_nj = nj;
+    return 0;
+}
+
 // Mean ms of force + reduction for one shape (after one untimed launch).
 double time_shape(const Shape &sh, int nj, int ni, const IBlock &ib, float eps2, int reps)
 {

@@ -182,6 +182,18 @@ int g6x_self_sweep(int, int nj, int i_first, int ni, double ti, double eps2)
     return 0;
 }
 
+int g6x_potential(int, int nj, int i_first, int ni, double ti, double eps2, double *pot)
+{
+    if (!pot) return fail(-1, "g6x_potential: pot is NULL");
+    return potential_sweep(nj, i_first, ni, ti, eps2, pot);
+}
+
+int g6x_densities(int, int nj, int i_first, int ni, double ti, int k, double *density,
+                  double *dk2)
+{
+    return density_sweep(nj, i_first, ni, ti, k, density, dk2);
+}
+
 int g6x_read_results(int, int ni, double *acc, double *jerk, double *pot, int *nn)
 {
     if (ensure_open("g6x_read_results")) return -1;

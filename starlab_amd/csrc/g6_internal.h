@@ -91,6 +91,18 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          const JPred *jpred, int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
                          const Results &r, int lanes);
+// Potential only: pot[i] = -sum_j m_j / r_ij for the i-block (self excluded
+// by address as in the force kernel); part holds nsplit * ib.ld doubles.
+hipError_t launch_pot(hipStream_t s, const JPred *jpred, int nj, const IBlock &ib, int ni,
+                      float eps2, int nsplit, int jchunk, int tile, double *part, double *pot,
+                      bool xcd_remap);
+// k-th nearest neighbours of the j-particles [i_first, i_first + ni) among
+// [0, nj) (FP64 predicted positions) and their densities; 2 <= k <= 32.
+// part_d / part_a hold nsplit * knn_kmax(k) * ld entries.
+int knn_kmax(int k);
+hipError_t launch_knn(hipStream_t s, const double *xp64, int cap, int nj, int i_first, int ni,
+                      int k, int nsplit, int jchunk, double *part_d, int *part_a, int ld,
+                      const JRaw *jraw, double *density, double *dk2);
 // lanes per i the reduction uses by default for nsplit splits (4..64)
 int reduce_lanes(int nsplit);
 hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,

@@ -750,13 +750,265 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
 }
 
 // ---------------------------------------------------------------------------
+// Potential only (kira's energy path, pot_only in hdyn_grape6.C:477-551):
+// the pair law's pot term, -m/r, with the mapping, double-single
+// positions, self-exclusion ballot and FP32-tile / FP64-fold sums of
+// k_force.  About 15 VALU instructions per pair instead of ~38.
+// part[split][ld] receives one FP64 partial per i; k_pot_reduce sums the
+// splits in order.
+// ---------------------------------------------------------------------------
+template <bool MASK>
+__device__ __forceinline__ void pair_pot(const JPred &J, int jj, unsigned rel, float xih,
+                                         float yih, float zih, float xil, float yil,
+                                         float zil, float eps2, float &p)
+{
+#pragma clang fp contract(off)
+    float dx, dy, dz;
+    const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih, xil, yil, zil,
+                           eps2, dx, dy, dz);
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    if (MASK) rinv = (unsigned)jj == rel ? 0.0f : rinv;
+    p = __builtin_fmaf(-J.m, rinv, p);
+}
+
+template <bool MASK>
+__device__ __forceinline__ float tile_pot(const JPred *__restrict__ jp, int tn, unsigned rel,
+                                          float xih, float yih, float zih, float xil,
+                                          float yil, float zil, float eps2)
+{
+    float p = 0.0f;
+    int jj = 0;
+    for (; jj + 4 <= tn; jj += 4) {
+        const JPred *g = jp + jj;
+        pair_pot<MASK>(g[0], jj + 0, rel, xih, yih, zih, xil, yil, zil, eps2, p);
+        pair_pot<MASK>(g[1], jj + 1, rel, xih, yih, zih, xil, yil, zil, eps2, p);
+        pair_pot<MASK>(g[2], jj + 2, rel, xih, yih, zih, xil, yil, zil, eps2, p);
+        pair_pot<MASK>(g[3], jj + 3, rel, xih, yih, zih, xil, yil, zil, eps2, p);
+    }
+    for (; jj < tn; ++jj)
+        pair_pot<MASK>(jp[jj], jj, rel, xih, yih, zih, xil, yil, zil, eps2, p);
+    return p;
+}
+
+__global__ __launch_bounds__(64) void k_pot(
+    const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
+    int jchunk, int tile, int n_igroups, double *__restrict__ part, int ld, int remap)
+{
+    int b = blockIdx.x;
+    if (remap) {
+        const int per = gridDim.x >> 3;
+        b = (b & 7) * per + (b >> 3);
+    }
+    const int split = b / n_igroups;
+    const int ig = b - split * n_igroups;
+    const int i = ig * 64 + threadIdx.x;
+    const int ic = i < ni ? i : ni - 1;
+    const float xih = ib.xh[ic], yih = ib.yh[ic], zih = ib.zh[ic];
+    const float xil = ib.xl[ic], yil = ib.yl[ic], zil = ib.zl[ic];
+    const int excl = i < ni ? ib.excl[ic] : -1;
+    const int jbeg = split * jchunk;
+    const int jend = min(jbeg + jchunk, nj);
+    double P = 0.0;
+    for (int t = jbeg; t < jend; t += tile) {
+        const int tn = min(tile, jend - t);
+        const unsigned rel = (unsigned)(excl - t);
+        float p;
+        if (__ballot(rel < (unsigned)tn) == 0)
+            p = tile_pot<false>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil, eps2);
+        else
+            p = tile_pot<true>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil, eps2);
+        P += (double)p;
+    }
+    if (i < ni) part[(size_t)split * ld + i] = P;
+}
+
+__global__ __launch_bounds__(256) void k_pot_reduce(const double *__restrict__ part, int nsplit,
+                                                    int ld, int ni, double *__restrict__ pot)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    double s = 0.0;
+    for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * ld + i];
+    pot[i] = s;
+}
+
+hipError_t launch_pot(hipStream_t s, const JPred *jpred, int nj, const IBlock &ib, int ni,
+                      float eps2, int nsplit, int jchunk, int tile, double *part, double *pot,
+                      bool xcd_remap)
+{
+    if (ni <= 0) return hipSuccess;
+    const int n_igroups = (ni + 63) / 64;
+    const int nwg = n_igroups * nsplit;
+    const int remap = (xcd_remap && nwg % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(k_pot, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni, eps2, jchunk,
+                       tile, n_igroups, part, ib.ld, remap);
+    hipLaunchKernelGGL(k_pot_reduce, dim3((ni + 255) / 256), dim3(256), 0, s, part, nsplit,
+                       ib.ld, ni, pot);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k-th nearest neighbours and local densities (compute_density,
+// src/node/dyn/util/compute_density.C:69-226, the k = 12 density kira asks
+// for in grape6_calculate_densities, hdyn_grape6.C:3604, 4283-4449).
+//
+// Distances are FP64, d^2 = (dx*dx + dy*dy) + dz*dz unfused, as Starlab's
+// vec dot product (include/starlab_vector.h:68-70), so the neighbour sets
+// and the k-th distance are exact.  Each lane keeps the k nearest j of its
+// i (self excluded) as a list sorted by d^2, ties in ascending address
+// (a later j is inserted after equal ones, as compute_density.C:165-176
+// does).  j is wave-uniform (scalar loads); a j-split writes its partial
+// list [split][q][ld]; k_knn_merge inserts the splits' lists in split order
+// (= ascending address), then forms
+//   density = (sum of the masses of the k-1 nearest) / (4/3 pi d_k^3)
+// (compute_density.C:200-210; 0 when d_k = 0).
+//
+// The list lives in registers: KMAX slots, fully unrolled, with the runtime
+// k <= KMAX only deciding which slot is the threshold.
+// ---------------------------------------------------------------------------
+template <int KMAX>
+struct KList {
+    double d[KMAX];
+    int a[KMAX];
+    double kth;
+};
+
+template <int KMAX>
+__device__ __forceinline__ void klist_init(KList<KMAX> &L)
+{
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) { L.d[q] = __builtin_inf(); L.a[q] = -1; }
+    L.kth = __builtin_inf();
+}
+
+// Insert (d, a) if it is nearer than the k-th entry.  Stable: an entry
+// goes after the ones with equal d already present.
+template <int KMAX>
+__device__ __forceinline__ void klist_insert(KList<KMAX> &L, int k, double d, int a)
+{
+    bool moved = false;
+    double cd = d;
+    int ca = a;
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) {
+        const bool sw = moved || cd < L.d[q];
+        const double td = L.d[q];
+        const int ta = L.a[q];
+        L.d[q] = sw ? cd : td;
+        L.a[q] = sw ? ca : ta;
+        cd = sw ? td : cd;
+        ca = sw ? ta : ca;
+        moved = sw;
+    }
+    double kth = L.d[0];
+#pragma unroll
+    for (int q = 1; q < KMAX; ++q) kth = (q == k - 1) ? L.d[q] : kth;
+    L.kth = kth;
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_knn(
+    const double *__restrict__ xp64, int cap, int nj, int i_first, int ni, int k,
+    int jchunk, int n_igroups, double *__restrict__ out_d, int *__restrict__ out_a, int ld)
+{
+#pragma clang fp contract(off)
+    const int split = blockIdx.x / n_igroups;
+    const int ig = blockIdx.x - split * n_igroups;
+    const int i = ig * 64 + threadIdx.x;
+    const int self = i_first + (i < ni ? i : ni - 1);
+    const double xi = xp64[self], yi = xp64[(size_t)cap + self];
+    const double zi = xp64[2 * (size_t)cap + self];
+    const double *__restrict__ X = xp64;
+    const double *__restrict__ Y = xp64 + cap;
+    const double *__restrict__ Z = xp64 + 2 * (size_t)cap;
+    KList<KMAX> L;
+    klist_init(L);
+    const int jbeg = split * jchunk;
+    const int jend = min(jbeg + jchunk, nj);
+    for (int j = jbeg; j < jend; ++j) {
+        const double dx = X[j] - xi;
+        const double dy = Y[j] - yi;
+        const double dz = Z[j] - zi;
+        const double d2 = (dx * dx + dy * dy) + dz * dz;
+        if (d2 < L.kth && j != self) klist_insert(L, k, d2, j);
+    }
+    if (i < ni) {
+#pragma unroll
+        for (int q = 0; q < KMAX; ++q) {
+            if (q < k) {
+                const size_t o = ((size_t)split * KMAX + q) * ld + i;
+                out_d[o] = L.d[q];
+                out_a[o] = L.a[q];
+            }
+        }
+    }
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_knn_merge(
+    const double *__restrict__ in_d, const int *__restrict__ in_a, int nsplit, int ld, int ni,
+    int k, const JRaw *__restrict__ jraw, double *__restrict__ density,
+    double *__restrict__ dk2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    KList<KMAX> L;
+    klist_init(L);
+    for (int s = 0; s < nsplit; ++s) {
+        for (int q = 0; q < k; ++q) {
+            const size_t o = ((size_t)s * KMAX + q) * ld + i;
+            const double d = in_d[o];
+            if (!(d < L.kth)) break;            // the split's list is sorted
+            klist_insert(L, k, d, in_a[o]);
+        }
+    }
+    double mass = 0.0;   // nearest first, as compute_density.C:207-208
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) {
+        if (q < k - 1 && L.a[q] >= 0) mass += jraw[L.a[q]].m;
+    }
+    const double d2 = L.kth;
+    double rho = 0.0;
+    if (d2 > 0.0) {
+        const double volume = (4.0 / 3.0) * 3.14159265358979323846 * pow(d2, 1.5);
+        if (volume > 0.0) rho = mass / volume;
+    }
+    density[i] = rho;
+    dk2[i] = d2;
+}
+
+int knn_kmax(int k) { return k <= 16 ? 16 : 32; }
+
+hipError_t launch_knn(hipStream_t s, const double *xp64, int cap, int nj, int i_first, int ni,
+                      int k, int nsplit, int jchunk, double *part_d, int *part_a, int ld,
+                      const JRaw *jraw, double *density, double *dk2)
+{
+    if (ni <= 0) return hipSuccess;
+    if (k < 2 || k > 32) return hipErrorInvalidValue;
+    const int n_igroups = (ni + 63) / 64;
+    const dim3 grid(n_igroups * nsplit), mgrid((ni + 63) / 64);
+    if (knn_kmax(k) == 16) {
+        hipLaunchKernelGGL(k_knn<16>, grid, dim3(64), 0, s, xp64, cap, nj, i_first, ni, k,
+                           jchunk, n_igroups, part_d, part_a, ld);
+        hipLaunchKernelGGL(k_knn_merge<16>, mgrid, dim3(64), 0, s, part_d, part_a, nsplit, ld,
+                           ni, k, jraw, density, dk2);
+    } else {
+        hipLaunchKernelGGL(k_knn<32>, grid, dim3(64), 0, s, xp64, cap, nj, i_first, ni, k,
+                           jchunk, n_igroups, part_d, part_a, ld);
+        hipLaunchKernelGGL(k_knn_merge<32>, mgrid, dim3(64), 0, s, part_d, part_a, nsplit, ld,
+                           ni, k, jraw, density, dk2);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Neighbour lists (FP64, exact): one wave per listed i-slot; lanes sweep 64
 // consecutive j, ballot + prefix count keeps the list in ascending address.
 // counts[i] is the true count; at most nb_cap entries are stored.
 // The test is ((e2[i] + dx*dx) + dy*dy) + dz*dz < h2[i]: with e2 = 0 that is
 // the unsoftened d^2 < h2 of the GRAPE-6 front end (0 + a = a exactly); the
 // GRAPE-4 front end passes e2 = 2 eps2 and h2 = 2 h2, the operation order of
-// the reference's IEEE GRAPE-4 simulator (harp3fastsim.c:283-295).
+// the reference's IEEE GRAPE-4 simulator (harp3fastsim.c:289-303).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_nblist(
     const double *__restrict__ xp64, int cap, const int *__restrict__ jindex,

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import load
 
-KERNELS = ("predict", "force", "reduce", "refine", "nblist", "scatter")
+KERNELS = ("predict", "force", "reduce", "refine", "nblist", "scatter", "pot", "knn")
 
 
 def _dp(a):
@@ -148,6 +148,24 @@ class GRAPE6:
     # -- extensions -------------------------------------------------------------
     def self_sweep(self, nj, i_first, ni, ti, eps2) -> int:
         return self.L.g6x_self_sweep(self._cid.value, nj, i_first, ni, ti, eps2)
+
+    def potential(self, nj, i_first, ni, ti, eps2):
+        """pot of j-particles [i_first, i_first+ni) against [0, nj) at ti
+        (include/g6x.h g6x_potential)."""
+        pot = np.zeros(ni)
+        rc = self.L.g6x_potential(self._cid.value, nj, i_first, ni, ti, eps2, _dp(pot))
+        if rc:
+            raise RuntimeError(f"g6x_potential: {self.last_error()}")
+        return pot
+
+    def densities(self, nj, i_first, ni, ti, k=12):
+        """(density, dk2) of j-particles [i_first, i_first+ni) among [0, nj)
+        (include/g6x.h g6x_densities; compute_density with k neighbours)."""
+        rho = np.zeros(ni); dk2 = np.zeros(ni)
+        rc = self.L.g6x_densities(self._cid.value, nj, i_first, ni, ti, k, _dp(rho), _dp(dk2))
+        if rc:
+            raise RuntimeError(f"g6x_densities: {self.last_error()}")
+        return rho, dk2
 
     def read_results(self, ni):
         acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)

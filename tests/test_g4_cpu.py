@@ -170,7 +170,7 @@ def test_plain_c_caller_links_and_runs(tmp_path):
 def test_g4_oracle_bitwise_vs_reference_simulator(eps2):
     """acc, jerk and neighbour lists of oracle.g4_force equal the reference
     simulator's bit for bit, the i-particles themselves in j-memory (their
-    r^2 = 0 pair skipped when eps2 = 0, harp3fastsim.c:301)."""
+    r^2 = 0 pair skipped when eps2 = 0, harp3fastsim.c:304)."""
     sim = oracle.Harp3Sim()
     n = 900
     m, x, v = oracle.makeplummer(n, 7)
