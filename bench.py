@@ -105,6 +105,54 @@ def ensemble_legs(seconds=2.0):
     return out
 
 
+def aux_legs(g, inp, reps=3):
+    """The energy and density fast paths on the same j-memory (include/g6x.h:
+    g6x_potential, pot only; g6x_densities, k = 12 nearest neighbours), each
+    timed over `reps` calls after one warm call, with its kernel time."""
+    n = len(inp["m"])
+    out = {}
+    g.option("timing", 1)
+    for name, kern, fn in (("potential", "pot", lambda: g.potential(n, 0, n, 0.0, EPS2)),
+                           ("densities_k12", "knn", lambda: g.densities(n, 0, n, 0.0, 12))):
+        fn()
+        g.reset_timing()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        wall = (time.perf_counter() - t0) / reps
+        calls, ms = g.kernel_time(kern)
+        out[name] = {"wall_ms": round(wall * 1e3, 3),
+                     "kernel_ms": round(ms / max(calls, 1), 4),
+                     "pairs_per_s_kernel": n * (n - 1) / (ms / max(calls, 1) * 1e-3)}
+    g.option("timing", 0)
+    return out
+
+
+def grape4_leg(inp, xp, vp, npipe=2048):
+    """One full sweep through the GRAPE-4 ABI (libg4x.so) the way kira drives it
+    with USE_HALF_CHIP: every particle in two adjacent pipes, 1024 particles per
+    h3calc call, host buffers in and out (PCIe and host packing included).
+    xp, vp: the j-particles predicted to t = 0 (read back from libg6), which
+    kira would have predicted on the host."""
+    from starlab_amd.g4 import GRAPE4
+    n = len(inp["m"])
+    g4 = GRAPE4()
+    g4.open()
+    g4.load_all(inp["x"], inp["v"], inp["a2"], inp["j6"], inp["m"], inp["tj"])
+    g4.setti(0.0)
+    per = npipe // 2
+    blocks = [np.repeat(np.arange(s, min(n, s + per)), 2) for s in range(0, n, per)]
+    g4.calc(n, xp[blocks[0]], vp[blocks[0]], EPS2 / 2, 0.0)      # warm
+    t0 = time.perf_counter()
+    for b in blocks:
+        g4.calc(n, xp[b], vp[b], EPS2 / 2, 0.0)
+    wall = time.perf_counter() - t0
+    slots = g4.option("last_slots")
+    g4.close()
+    return {"pairs_per_s": n * (n - 1) / wall, "wall_ms": round(wall * 1e3, 3),
+            "calls": len(blocks), "pipes_per_call": npipe, "distinct_i_per_call": slots}
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "force_n65536_traffic.json")
     try:
@@ -123,6 +171,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ens", action="store_true", help="skip the ensemble / serving legs")
     ap.add_argument("--no-tune", action="store_true", help="skip the force-shape autotuner")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the energy/density fast-path and GRAPE-4 legs")
     ap.add_argument("--tune-file", default=None,
                     help="force-shape table: loaded if it exists (no tuning), else written "
                          "after tuning")
@@ -230,7 +280,15 @@ def main():
         "nn_refinements": refinements,
         "autotune": tune,
     }
+    if not a.no_aux:
+        log("bench: energy / density fast paths")
+        out["aux"] = aux_legs(g, inp)
+        assert g.self_sweep(n, 0, n, 0.0, EPS2) == 0
+        xp, vp = g.read_predicted(n)
     g.close()
+    if not a.no_aux:
+        log("bench: GRAPE-4 ABI sweep")
+        out["aux"]["grape4_abi_sweep"] = grape4_leg(inp, xp, vp)
     if not a.no_ens:
         log("bench: ensemble and serving legs")
         ens = ensemble_legs()

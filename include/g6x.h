@@ -75,6 +75,9 @@ int g6x_sync(int cluster_id);
  *   "tile"           j per FP32 tile: 0 auto, 64, 128, 256    G6_TILE
  *   "last_tile"      (read only) tile of the last force launch
  *   "reduce_lanes"   lanes per i in the split reduction: 0 auto, 4..64  G6_REDUCE_LANES
+ *   "wpg"            waves per force workgroup, each on its own j-split, folded
+ *                    in LDS before the reduction: 0 auto, 1, 2, 4, 8, 16  G6_WPG
+ *   "last_wpg", "last_lanes"  (read only) of the last force launch
  * Returns 0, or -1 for an unknown key. */
 int       g6x_set_option(const char *key, long long value);
 long long g6x_get_option(const char *key);
@@ -82,7 +85,8 @@ long long g6x_get_option(const char *key);
 /* Per-kernel event timing, accumulated since the last reset (needs
  * "timing" = 1).  kernel: 0 predictor, 1 force, 2 reduce (includes the
  * FP64 nearest-neighbour rescan of ambiguous i), 3 unused (0 launches),
- * 4 neighbour list, 5 j scatter.  Returns launches; *ms = total ms. */
+ * 4 neighbour list, 5 j scatter, 6 potential (g6x_potential), 7 k-nearest
+ * neighbours (g6x_densities).  Returns launches; *ms = total ms. */
 int  g6x_kernel_time(int kernel, double *ms);
 void g6x_reset_timing(void);
 
@@ -104,7 +108,8 @@ int  g6x_autotune(int cluster_id, int nj, int ni_max, double eps2);
 /* Tuned shape of one bucket; returns 1 if tuned, 0 if not, -1 bad bucket. */
 int  g6x_get_tuning(int bucket, int *variant, int *tile, int *waves, int *lanes);
 int  g6x_clear_tuning(void);
-/* Tuning table as text ("bucket variant tile waves lanes" per line).  The
+/* Tuning table as text ("bucket variant tile waves lanes wpg" per line; a
+ * line without wpg uses the built-in rule for it).  The
  * library also loads the file named by G6_TUNE_FILE at g6_open_.
  * save returns 0 / -1; load returns the entries read or -1. */
 int  g6x_save_tuning(const char *path);

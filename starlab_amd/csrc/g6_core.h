@@ -42,6 +42,7 @@ struct Options {
     int force_variant = -1;   // -1 auto, 0 scalar-load kernel, 1 LDS kernel
     int tile = 0;             // j per FP32 tile of the scalar-load kernel: 0 auto, 64/128/256
     int reduce_lanes = 0;     // lanes per i in the split reduction: 0 auto, 4..64
+    int wpg = 0;              // waves per force workgroup (j-splits folded in LDS): 0 auto, 1..16
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every
                               // size so far, so off unless asked for)
@@ -54,6 +55,7 @@ struct TimedPair { int kernel; hipEvent_t a, b; };
 constexpr int kBuckets = 21;
 struct Shape {
     int variant = 0, tile = 0, waves = 0, lanes = 0;
+    int wpg = 0;   // waves per workgroup of the scalar-load kernel (0: built-in rule)
 };
 
 int bucket_of(int ni);
@@ -134,7 +136,7 @@ struct Engine {
     std::vector<double> last_h2;
     int last_include_self = 1;
     int geom_wg = 0, geom_nsplit = 0, geom_jchunk = 0, geom_variant = 0, geom_tile = 0;
-    int geom_lanes = 0;
+    int geom_lanes = 0, geom_wpg = 1;
     Shape tuned[kBuckets];
     bool forcing_shape = false;   // the autotuner's candidate overrides everything
     Shape forced;

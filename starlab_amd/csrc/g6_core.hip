@@ -70,6 +70,7 @@ void read_env()
     o.lds_min_ni = (int)env_ll("LDS_MIN_NI", o.lds_min_ni);
     o.tile = (int)env_ll("TILE", o.tile);
     o.reduce_lanes = (int)env_ll("REDUCE_LANES", o.reduce_lanes);
+    o.wpg = (int)env_ll("WPG", o.wpg);
     if (o.npipes < 1) o.npipes = 1;
     if (o.nb_cap < 1) o.nb_cap = 1;
 }
@@ -357,7 +358,11 @@ size_t partials_bytes(int ld, int nsplit) { return (size_t)ld * nsplit * (7 * 8 
 
 struct Geometry {
     int variant, tile, nsplit, jchunk, lanes;
+    int wpg;        // waves per workgroup (scalar-load kernel)
+    int npart;      // partial splits the reduction reads
 };
+
+bool valid_wpg(int w) { return w == 1 || w == 2 || w == 4 || w == 8 || w == 16; }
 
 // Force geometry: kernel variant, j-tile length (FP32 partial-sum span),
 // j-split and the reduction's lanes per i.  Precedence: the autotuner's
@@ -376,6 +381,7 @@ Geometry choose_geometry(int ni, int nj)
         else if (t.waves == 0) sh.variant = ni >= E.opt.lds_min_ni ? 1 : 0;
         if (E.opt.tile == 64 || E.opt.tile == 128 || E.opt.tile == kTile) sh.tile = E.opt.tile;
         if (E.opt.reduce_lanes > 0) sh.lanes = E.opt.reduce_lanes;
+        if (valid_wpg(E.opt.wpg)) sh.wpg = E.opt.wpg;
         if (t.waves == 0 || E.opt.waves_target != Options().waves_target)
             sh.waves = E.opt.waves_target;
     }
@@ -394,7 +400,20 @@ Geometry choose_geometry(int ni, int nj)
     const int tiles_per = (ntiles + sp - 1) / sp;
     g.jchunk = tiles_per * g.tile;
     g.nsplit = std::max(1, (nj + g.jchunk - 1) / g.jchunk);
-    g.lanes = sh.lanes > 0 ? sh.lanes : reduce_lanes(g.nsplit);
+    // waves per workgroup: the splits of a workgroup fold their sums in LDS;
+    // built-in rule 8 for many splits (small i-blocks), else 4, at most nsplit
+    // (measured at N = 65536: 16 waves per workgroup is slower at every size)
+    if (g.variant == 1) {
+        g.wpg = 1;
+    } else if (valid_wpg(sh.wpg)) {
+        g.wpg = sh.wpg;
+    } else {
+        const int cap = g.nsplit >= 64 ? 8 : 4;
+        g.wpg = 1;
+        while (2 * g.wpg <= std::min(cap, g.nsplit)) g.wpg *= 2;
+    }
+    g.npart = force_partials(g.variant, g.nsplit, g.wpg);
+    g.lanes = sh.lanes > 0 ? sh.lanes : reduce_lanes(g.npart);
     return g;
 }
 
@@ -405,23 +424,24 @@ int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r
     if (zskip && !(eps2 >= __FLT_MIN__)) eps2 = 0.0f;   // as launch_force treats it
     const Geometry g = choose_geometry(ni, nj);
     const int ld = ib.ld;
-    if (ensure_partials(partials_bytes(ld, g.nsplit))) return -1;
-    Partials p = partials_at(ld, g.nsplit);
+    if (ensure_partials(partials_bytes(ld, g.npart))) return -1;
+    Partials p = partials_at(ld, g.npart);
     E.geom_nsplit = g.nsplit;
     E.geom_jchunk = g.jchunk;
     E.geom_tile = g.tile;
     E.geom_lanes = g.lanes;
-    E.geom_wg = ((ni + force_igroup(g.variant) - 1) / force_igroup(g.variant)) * g.nsplit;
+    E.geom_wpg = g.wpg;
+    E.geom_wg = ((ni + force_igroup(g.variant) - 1) / force_igroup(g.variant)) * g.npart;
     E.geom_variant = g.variant;
     {
         KTimer t(K_FORCE);
         HIPCHK(launch_force(E.stream, E.d_jpred, nj, ib, ni, eps2, g.nsplit, g.jchunk, g.tile,
-                            p, E.opt.xcd_remap != 0, g.variant, zskip));
+                            p, E.opt.xcd_remap != 0, g.variant, zskip, g.wpg));
         t.stop();
     }
     {
         KTimer t(K_REDUCE);
-        HIPCHK(launch_reduce(E.stream, p, g.nsplit, ni, E.d_jpred, nj, ib, eps2, g.tile,
+        HIPCHK(launch_reduce(E.stream, p, g.npart, ni, E.d_jpred, nj, ib, eps2, g.tile,
                              E.d_xp64, E.dcap, E.d_jindex, r, g.lanes));
         t.stop();
     }
@@ -910,12 +930,26 @@ int autotune(int nj, int ni_max, double eps2)
             for (int tile : {64, 128, kTile}) {
                 if (variant == 1 && tile != kTile) continue;
                 for (int w : waves) {
-                    Shape sh;
-                    sh.variant = variant; sh.tile = tile; sh.waves = w; sh.lanes = 0;
-                    const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
-                    if (ms < 0) { E.opt.timing = saved_timing; return fail(-1, "g6x_autotune: launch failed"); }
-                    if (ms < best_ms) { best_ms = ms; best = sh; }
+                    for (int wpg : {1, 4, 16}) {
+                        if (variant == 1 && wpg != 1) continue;
+                        Shape sh;
+                        sh.variant = variant; sh.tile = tile; sh.waves = w; sh.lanes = 0;
+                        sh.wpg = wpg;
+                        const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
+                        if (ms < 0) { E.opt.timing = saved_timing; return fail(-1, "g6x_autotune: launch failed"); }
+                        if (ms < best_ms) { best_ms = ms; best = sh; }
+                    }
                 }
+            }
+        }
+        // neighbouring waves-per-workgroup counts for the winner
+        if (best.variant == 0) {
+            const Shape base = best;
+            for (int wpg : {2, 8}) {
+                Shape sh = base;
+                sh.wpg = wpg;
+                const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
+                if (ms >= 0 && ms < best_ms) { best_ms = ms; best = sh; }
             }
         }
         // reduction lanes for the winning force shape
@@ -938,10 +972,12 @@ int save_tuning(const char *path)
 {
     FILE *f = fopen(path, "w");
     if (!f) return fail(-1, "g6x_save_tuning: cannot write %s", path);
-    fprintf(f, "# libg6 force shapes: bucket (ni <= 2^bucket) variant tile waves lanes\n");
+    fprintf(f, "# %s force shapes: bucket (ni <= 2^bucket) variant tile waves lanes wpg\n",
+            lib_tag());
     for (int b = 0; b < kBuckets; ++b) {
         const Shape &t = E.tuned[b];
-        if (t.waves > 0) fprintf(f, "%d %d %d %d %d\n", b, t.variant, t.tile, t.waves, t.lanes);
+        if (t.waves > 0)
+            fprintf(f, "%d %d %d %d %d %d\n", b, t.variant, t.tile, t.waves, t.lanes, t.wpg);
     }
     fclose(f);
     return 0;
@@ -955,14 +991,16 @@ int load_tuning(const char *path)
     int n = 0;
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
-        int b, v, t, w, l;
-        if (sscanf(line, "%d %d %d %d %d", &b, &v, &t, &w, &l) != 5) continue;
+        int b, v, t, w, l, q = 0;   // a table without the wpg column uses the rule
+        const int got = sscanf(line, "%d %d %d %d %d %d", &b, &v, &t, &w, &l, &q);
+        if (got != 5 && got != 6) continue;
         if (b < 0 || b >= kBuckets || (v != 0 && v != 1) || w <= 0 ||
             (t != 64 && t != 128 && t != kTile) || (v == 1 && t != kTile) ||
-            (l != 0 && l != 4 && l != 8 && l != 16 && l != 32 && l != 64))
+            (l != 0 && l != 4 && l != 8 && l != 16 && l != 32 && l != 64) ||
+            (q != 0 && !valid_wpg(q)))
             continue;
         Shape &s = E.tuned[b];
-        s.variant = v; s.tile = t; s.waves = w; s.lanes = l;
+        s.variant = v; s.tile = t; s.waves = w; s.lanes = l; s.wpg = q;
         ++n;
     }
     fclose(f);

@@ -255,6 +255,11 @@ int g6x_set_option(const char *key, long long value)
     else if (k == "force_variant") o.force_variant = (int)value;
     else if (k == "lds_min_ni") o.lds_min_ni = (int)value;
     else if (k == "tile") o.tile = (int)value;
+    else if (k == "wpg") {
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+            return -1;
+        o.wpg = (int)value;
+    }
     else if (k == "reduce_lanes") {
         if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32 && value != 64)
             return -1;
@@ -282,6 +287,9 @@ long long g6x_get_option(const char *key)
     if (k == "tile") return o.tile;
     if (k == "last_tile") return E.geom_tile;
     if (k == "reduce_lanes") return o.reduce_lanes;
+    if (k == "wpg") return o.wpg;
+    if (k == "last_wpg") return E.geom_wpg;
+    if (k == "last_lanes") return E.geom_lanes;
     return -1;
 }
 

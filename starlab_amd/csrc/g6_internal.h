@@ -82,11 +82,16 @@ hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
 // (512 i per workgroup).  force_igroup() gives the i per workgroup.
 // zskip: pairs whose FP32 r^2 is zero add nothing (a softening below the
 // smallest normal float then counts as zero).
+// wpg: waves per workgroup of the scalar-load kernel (1, 2, 4, 8 or 16),
+// each on its own j-split of the same 64 i; they fold their sums in LDS, so
+// force_partials() = ceil(nsplit / wpg) partial splits reach the reduction
+// (the LDS kernel always writes nsplit).
 int force_igroup(int variant);
+int force_partials(int variant, int nsplit, int wpg);
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
                         int jchunk, int tile, const Partials &p, bool xcd_remap, int variant,
-                        bool zskip);
+                        bool zskip, int wpg);
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          const JPred *jpred, int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,

@@ -234,11 +234,21 @@ __device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
         pair<MASK, ZSKIP>(jp[jj], jj, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
 }
 
+// A workgroup holds wpg waves (1..16) on the same 64 i and on consecutive
+// j-splits; before writing, the waves fold their partial sums through LDS in
+// wave order (a fixed order, so results stay deterministic), so the
+// reduction kernel reads nsplit / wpg partials instead of nsplit.  Dynamic
+// LDS: double[7][wpg][64], then float best[wpg][64], float second[wpg][64],
+// int tile[wpg][64].
+constexpr int kMaxWpg = 16;
+size_t force_fold_lds(int wpg) { return wpg > 1 ? (size_t)wpg * 64 * (7 * 8 + 3 * 4) : 0; }
+
 template <bool ZSKIP>
-__global__ __launch_bounds__(64) void k_force(
+__global__ __launch_bounds__(64 * kMaxWpg) void k_force(
     const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
-    int jchunk, int tile, int n_igroups, Partials P, int remap)
+    int jchunk, int tile, int n_igroups, Partials P, int remap, int wpg)
 {
+    extern __shared__ double fold[];
     int b = blockIdx.x;
     if (remap) {
         // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
@@ -247,9 +257,13 @@ __global__ __launch_bounds__(64) void k_force(
         const int per = gridDim.x >> 3;
         b = (b & 7) * per + (b >> 3);
     }
-    const int split = b / n_igroups;
-    const int ig = b - split * n_igroups;
-    const int lane = threadIdx.x;
+    const int sgroup = b / n_igroups;
+    const int ig = b - sgroup * n_igroups;
+    const int lane = threadIdx.x & 63;
+    // uniform within the wave: readfirstlane tells the compiler so, which keeps
+    // the j records on scalar loads into SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int split = sgroup * wpg + wave;
     const int i = ig * 64 + lane;
     const int ic = i < ni ? i : ni - 1;
 
@@ -294,8 +308,46 @@ __global__ __launch_bounds__(64) void k_force(
         }
     }
 
+    if (wpg > 1) {
+        const int stride = wpg * 64;
+        const int slot = wave * 64 + lane;
+        float *fb = reinterpret_cast<float *>(fold + 7 * stride);
+        float *fs = fb + stride;
+        int *ft = reinterpret_cast<int *>(fs + stride);
+        fold[0 * stride + slot] = A0;
+        fold[1 * stride + slot] = A1;
+        fold[2 * stride + slot] = A2;
+        fold[3 * stride + slot] = A3;
+        fold[4 * stride + slot] = A4;
+        fold[5 * stride + slot] = A5;
+        fold[6 * stride + slot] = A6;
+        fb[slot] = rbest;
+        fs[slot] = rsecond;
+        ft[slot] = rtile;
+        __syncthreads();
+        if (wave != 0) return;
+        for (int w = 1; w < wpg; ++w) {
+            const int o = w * 64 + lane;
+            A0 += fold[0 * stride + o];
+            A1 += fold[1 * stride + o];
+            A2 += fold[2 * stride + o];
+            A3 += fold[3 * stride + o];
+            A4 += fold[4 * stride + o];
+            A5 += fold[5 * stride + o];
+            A6 += fold[6 * stride + o];
+            // later splits hold higher addresses: they win only strictly
+            const float bw = fb[o];
+            if (bw < rbest) {
+                rsecond = __builtin_fminf(rbest, fs[o]);
+                rbest = bw;
+                rtile = ft[o];
+            } else {
+                rsecond = __builtin_fminf(rsecond, bw);
+            }
+        }
+    }
     if (i < ni) {
-        const size_t o = (size_t)split * P.ld + i;
+        const size_t o = (size_t)sgroup * P.ld + i;
         P.c[0][o] = A0;
         P.c[1][o] = A1;
         P.c[2][o] = A2;
@@ -512,14 +564,22 @@ __global__ __launch_bounds__(256) void k_force_lds(
 
 int force_igroup(int variant) { return variant == 1 ? kLdsWG * kLdsI : 64; }
 
+int force_partials(int variant, int nsplit, int wpg)
+{
+    if (variant == 1 || wpg <= 1) return nsplit;
+    return (nsplit + wpg - 1) / wpg;
+}
+
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
                         int jchunk, int tile, const Partials &p, bool xcd_remap, int variant,
-                        bool zskip)
+                        bool zskip, int wpg)
 {
     if (ni <= 0) return hipSuccess;
+    if (variant == 1) wpg = 1;
+    if (wpg < 1 || wpg > kMaxWpg || (wpg & (wpg - 1))) return hipErrorInvalidValue;
     const int n_igroups = (ni + force_igroup(variant) - 1) / force_igroup(variant);
-    const int nwg = n_igroups * nsplit;
+    const int nwg = n_igroups * force_partials(variant, nsplit, wpg);
     const int remap = (xcd_remap && nwg % 8 == 0) ? 1 : 0;
     // r^2 >= eps2 > 0 when eps2 is a normal float, so the zero-r^2 rule only
     // needs its per-pair select without softening
@@ -535,11 +595,11 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
             hipLaunchKernelGGL((k_force_lds<false, false>), dim3(nwg), dim3(kLdsWG), 0, s, jpred,
                                nj, ib, ni, eps2, jchunk, n_igroups, p, remap);
     } else if (zskip) {
-        hipLaunchKernelGGL(k_force<true>, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
-                           0.0f, jchunk, tile, n_igroups, p, remap);
+        hipLaunchKernelGGL(k_force<true>, dim3(nwg), dim3(64 * wpg), force_fold_lds(wpg), s,
+                           jpred, nj, ib, ni, 0.0f, jchunk, tile, n_igroups, p, remap, wpg);
     } else {
-        hipLaunchKernelGGL(k_force<false>, dim3(nwg), dim3(64), 0, s, jpred, nj, ib, ni,
-                           eps2, jchunk, tile, n_igroups, p, remap);
+        hipLaunchKernelGGL(k_force<false>, dim3(nwg), dim3(64 * wpg), force_fold_lds(wpg), s,
+                           jpred, nj, ib, ni, eps2, jchunk, tile, n_igroups, p, remap, wpg);
     }
     return hipGetLastError();
 }

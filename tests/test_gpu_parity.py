@@ -329,6 +329,57 @@ def test_reduce_lanes_per_i(g, lanes):
         g.option("reduce_lanes", 0)
 
 
+@pytest.mark.parametrize("wpg", [1, 2, 4, 8, 16])
+def test_waves_per_workgroup_fold(g, wpg):
+    """Every waves-per-workgroup count of the scalar-load kernel (the waves of
+    a workgroup fold their j-splits in LDS before the reduction), on split
+    counts that leave a workgroup partly empty (47) and that divide evenly
+    (64), with index exclusion and exact nearest neighbours."""
+    s = plummer_state(6000, seed=41)
+    load_j(g, s)
+    g.option("force_variant", 0)
+    g.option("wpg", wpg)
+    try:
+        for nsplit in (47, 64):
+            g.option("nsplit", nsplit)
+            isel = np.random.default_rng(wpg + nsplit).choice(6000, 200, replace=False)
+            errs = _abi_force_vs_oracle(g, s, 0.0, 1e-8, isel)
+            assert g.option("last_wpg") == wpg
+            _check(errs, f"wpg={wpg} nsplit={nsplit}")
+    finally:
+        g.option("nsplit", 0)
+        g.option("wpg", 0)
+        g.option("force_variant", -1)
+
+
+def test_fold_is_deterministic_and_matches_unfolded(g):
+    """The same block twice gives identical bits for every wpg, and folding
+    changes results only by FP64 reassociation of the split sums."""
+    s = plummer_state(8000, seed=43)
+    load_j(g, s)
+    xp, vp = _predict(s, 0.0)
+    isel = np.arange(0, 8000, 13)
+    g.option("force_variant", 0)
+    g.option("nsplit", 64)
+    outs = {}
+    try:
+        for wpg in (1, 4, 16):
+            g.option("wpg", wpg)
+            a = g.force(8000, s["idx"][isel], xp[isel], vp[isel], 1e-8)
+            b = g.force(8000, s["idx"][isel], xp[isel], vp[isel], 1e-8)
+            for u, w in zip(a, b):
+                np.testing.assert_array_equal(u, w)
+            outs[wpg] = a
+    finally:
+        g.option("nsplit", 0)
+        g.option("wpg", 0)
+        g.option("force_variant", -1)
+    for wpg in (4, 16):
+        np.testing.assert_allclose(outs[wpg][0], outs[1][0], rtol=0, atol=1e-12 * np.abs(outs[1][0]).max())
+        np.testing.assert_allclose(outs[wpg][2], outs[1][2], rtol=1e-13)
+        np.testing.assert_array_equal(outs[wpg][3], outs[1][3])
+
+
 def test_autotune_table_and_parity(g, tmp_path):
     """The autotuner fills one shape per i-block bucket, forces computed with
     tuned shapes keep parity, and the table survives a save/load round
