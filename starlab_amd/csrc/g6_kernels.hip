@@ -156,10 +156,10 @@ struct Acc32 {
     float best, second;
 };
 
-// FP32 r^2 of a double-single difference: ((dz*dz + dy*dy) + dx*dx) + eps2
-// with the two inner sums fused.  Every force variant and the reduction
-// kernel's rescan use this exact sequence, so all round identically (adding
-// eps2 = 0 is exact, which lets a kernel skip that add).
+// FP32 r^2 of a double-single difference: ((eps2 + dz*dz) + dy*dy) + dx*dx,
+// all three steps fused multiply-adds.  Every force variant and the
+// reduction kernel's rescan use this exact sequence, so all round
+// identically; with eps2 = 0 the first step is exactly dz*dz.
 __device__ __forceinline__ float ds_r2(float xjh, float yjh, float zjh, float xjl,
                                        float yjl, float zjl, float xih, float yih,
                                        float zih, float xil, float yil, float zil,
@@ -169,10 +169,9 @@ __device__ __forceinline__ float ds_r2(float xjh, float yjh, float zjh, float xj
     dx = (xjh - xih) + (xjl - xil);
     dy = (yjh - yih) + (yjl - yil);
     dz = (zjh - zih) + (zjl - zil);
-    float r2 = dz * dz;
+    float r2 = __builtin_fmaf(dz, dz, eps2);
     r2 = __builtin_fmaf(dy, dy, r2);
-    r2 = __builtin_fmaf(dx, dx, r2);
-    return r2 + eps2;
+    return __builtin_fmaf(dx, dx, r2);
 }
 
 template <bool MASK, bool ZSKIP>
@@ -384,9 +383,8 @@ __device__ __forceinline__ void pair_v(float jxh, float jyh, float jzh, float jx
 {
 #pragma clang fp contract(off)
     float dx, dy, dz;
-    float r2 = ds_r2(jxh, jyh, jzh, jxl, jyl, jzl, I.xh, I.yh, I.zh, I.xl, I.yl, I.zl,
-                     0.0f, dx, dy, dz);
-    if (EPS) r2 = r2 + eps2v;
+    const float r2 = ds_r2(jxh, jyh, jzh, jxl, jyl, jzl, I.xh, I.yh, I.zh, I.xl, I.yl, I.zl,
+                           EPS ? eps2v : 0.0f, dx, dy, dz);
     const float dvx = jvx - I.vx;
     const float dvy = jvy - I.vy;
     const float dvz = jvz - I.vz;
