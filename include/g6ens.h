@@ -74,7 +74,12 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * keeps stepping on its own stream while a heavy-lane graph runs; default
  * 64), "fused_steps" (with light_max <= 64 the light lane runs a fused
  * kernel, one wave per system, up to this many steps per launch; 0 uses
- * step graphs there too; default 256).
+ * step graphs there too; default 256), "wg_steps" (> 0: systems with
+ * light_max < n <= wg_max_n run in a workgroup-fused kernel, one workgroup
+ * per system (256 threads up to wg_small_max = 512, else 1024), up to this
+ * many steps per launch, bitwise equal to the step graphs; default 0 = step
+ * graphs, which measured faster on mixed cluster batches), "wg_small_max",
+ * "wg_max_n" (<= 2048; fixed while open); read-only "medium_steps".
  * Returns 0, or -1 for an unknown key. */
 int       g6e_set_option(const char *key, long long value);
 long long g6e_get_option(const char *key);

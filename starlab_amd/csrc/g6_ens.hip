@@ -606,6 +606,184 @@ __global__ __launch_bounds__(64) void k_ens_fused(Pool P, SysHdr *__restrict__ H
 }
 
 // ---------------------------------------------------------------------------
+// Workgroup-fused kernel for medium systems (light_max < n <= wg_max_n): one
+// workgroup of NT threads owns one system and runs whole engine steps in a
+// loop of up to max_iters steps, with workgroup barriers between the phases
+// instead of kernel boundaries:
+//   predict  thread t predicts particles t, t + NT, ... into a 64-B record
+//            per particle in LDS (JPred for precision 32, JPred64 for 64);
+//   force    the same thread sums each of its i over the system's j-splits
+//            and 256-j tiles in the step kernels' order (split totals added
+//            in split order, as k_ens_correct does);
+//   correct  the step kernels' corrector, then a workgroup minimum of the
+//            step candidates and the header update by thread 0.
+// Each step reproduces the step kernels' operations, so a trajectory is the
+// same bit for bit as on the graph path.  Dynamic LDS: n_max records of 64 B,
+// then NT / 64 doubles for the minimum.  Systems whose records do not fit
+// stay on the graph path.
+// ---------------------------------------------------------------------------
+static_assert(sizeof(JPred) == 64 && sizeof(JPred64) == 64, "LDS record size");
+constexpr int kWgLargeThreads = 1024;   // threads per system of the large lane
+
+size_t ens_wg_lds(int nmax, int nt) { return (size_t)nmax * 64 + (size_t)(nt / 64) * 8; }
+
+template <int NT, int PREC>
+__global__ __launch_bounds__(NT) void k_ens_wg(Pool P, SysHdr *__restrict__ H,
+                                               const Chunk *__restrict__ list, int nlist,
+                                               int *__restrict__ phase_out, long long max_steps,
+                                               double dt_min, int max_iters, int nmax)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double wg_lds[];
+    __shared__ SysHdr sh;
+    const int w = blockIdx.x;
+    if (w >= nlist) return;                        // uniform per workgroup
+    const int slot = list[w].slot;
+    const int tid = threadIdx.x;
+    if (tid == 0) sh = H[slot];
+    __syncthreads();
+    if (sh.phase < INIT || sh.phase > FINAL || sh.n > nmax || sh.prec != PREC) return;   // uniform
+    const int n = sh.n;
+    const size_t cap = P.cap;
+    const size_t off = (size_t)sh.off;
+    JPred *rec32 = reinterpret_cast<JPred *>(wg_lds);
+    JPred64 *rec64 = reinterpret_cast<JPred64 *>(wg_lds);
+    double *red = wg_lds + (size_t)nmax * 8;
+    const int rounds = (n + NT - 1) / NT;
+    for (int it = 0; it < max_iters; ++it) {
+        const int phase = sh.phase;
+        const double dt = phase == STEP ? sh.dt : 0.0;
+        constexpr int prec = PREC;
+        // ---- predict ----
+        for (int i = tid; i < n; i += NT) {
+            const size_t p = off + i;
+            double x[3], v[3], a[3], jk[3], xp[3], vp[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                x[k] = P.x[k * cap + p]; v[k] = P.v[k * cap + p];
+                a[k] = P.a[k * cap + p]; jk[k] = P.j[k * cap + p];
+            }
+            ens_predict_one(x, v, a, jk, dt, xp, vp);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                P.xp[k * cap + p] = xp[k];
+                P.vp[k * cap + p] = vp[k];
+            }
+            const double m = P.m[p];
+            if (prec == 64) rec64[i] = make_jpred64(xp, vp, m);
+            else rec32[i] = make_jpred(xp, vp, m);
+        }
+        __syncthreads();
+        // ---- force + correct, NT i at a time ----
+        double cand = __builtin_inf();
+        for (int r = 0; r < rounds; ++r) {
+            const int i = r * NT + tid;
+            const bool act = i < n;
+            const int ic = act ? i : n - 1;
+            const int self = act ? i : -1;
+            double a1[3], j1[3], pot = 0.0;
+            for (int sp = 0; sp < sh.nsplit; ++sp) {
+                const int jbeg = sp * sh.jchunk;
+                const int jend = min(jbeg + sh.jchunk, n);
+                double A[7] = {0, 0, 0, 0, 0, 0, 0};
+                if (prec == 64) {
+                    const JPred64 I = rec64[ic];
+                    DAcc d;
+                    d.ax = d.ay = d.az = d.jx = d.jy = d.jz = d.p = 0.0;
+                    for (int t = jbeg; t < jend; t += kTile) {
+                        const int tn = min(kTile, jend - t);
+                        const unsigned rel = (unsigned)(self - t);
+                        if (__ballot(rel < (unsigned)tn) == 0) {
+                            for (int jj = 0; jj < tn; ++jj)
+                                ens_pair64<false>(rec64[t + jj], t + jj, self, I, sh.eps2, d);
+                        } else {
+                            for (int jj = 0; jj < tn; ++jj)
+                                ens_pair64<true>(rec64[t + jj], t + jj, self, I, sh.eps2, d);
+                        }
+                    }
+                    A[0] = d.ax; A[1] = d.ay; A[2] = d.az;
+                    A[3] = d.jx; A[4] = d.jy; A[5] = d.jz; A[6] = d.p;
+                } else {
+                    const JPred I = rec32[ic];
+                    const float eps2 = (float)sh.eps2;
+                    for (int t = jbeg; t < jend; t += kTile) {
+                        const int tn = min(kTile, jend - t);
+                        EAcc e;
+                        e.ax = e.ay = e.az = e.jx = e.jy = e.jz = e.kx = e.ky = e.kz = e.p = 0.f;
+                        const unsigned rel = (unsigned)(self - t);
+                        if (__ballot(rel < (unsigned)tn) == 0)
+                            ens_tile<false>(rec32 + t, tn, rel, I, eps2, e);
+                        else
+                            ens_tile<true>(rec32 + t, tn, rel, I, eps2, e);
+                        A[0] += (double)e.ax;
+                        A[1] += (double)e.ay;
+                        A[2] += (double)e.az;
+                        A[3] += (double)e.jx - 3.0 * (double)e.kx;
+                        A[4] += (double)e.jy - 3.0 * (double)e.ky;
+                        A[5] += (double)e.jz - 3.0 * (double)e.kz;
+                        A[6] += (double)e.p;
+                    }
+                }
+                // split 0 as is, later splits added in order (k_ens_correct)
+                if (sp == 0) {
+                    a1[0] = A[0]; a1[1] = A[1]; a1[2] = A[2];
+                    j1[0] = A[3]; j1[1] = A[4]; j1[2] = A[5];
+                    pot = A[6];
+                } else {
+                    a1[0] += A[0]; a1[1] += A[1]; a1[2] += A[2];
+                    j1[0] += A[3]; j1[1] += A[4]; j1[2] += A[5];
+                    pot += A[6];
+                }
+            }
+            if (act) {
+                const size_t p = off + i;
+                PState st;
+                double xp[3], vp[3];
+                for (int k = 0; k < 3; ++k) {
+                    st.x[k] = P.x[k * cap + p]; st.v[k] = P.v[k * cap + p];
+                    st.a[k] = P.a[k * cap + p]; st.j[k] = P.j[k * cap + p];
+                    xp[k] = P.xp[k * cap + p]; vp[k] = P.vp[k * cap + p];
+                }
+                st.pot = 0; st.pot0 = 0;
+                const double c = ens_correct_one(st, xp, vp, a1, j1, pot, phase, dt, sh.eta, prec);
+                cand = fmin(cand, c);
+                if (phase == INIT) {
+                    for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = st.a[k]; P.j[k * cap + p] = st.j[k]; }
+                    P.pot[p] = st.pot;
+                    P.pot0[p] = st.pot0;
+                } else if (phase == STEP) {
+                    for (int k = 0; k < 3; ++k) {
+                        P.x[k * cap + p] = st.x[k];
+                        P.v[k * cap + p] = st.v[k];
+                        P.a[k * cap + p] = st.a[k];
+                        P.j[k * cap + p] = st.j[k];
+                    }
+                    P.pot[p] = st.pot;
+                } else {
+                    P.pot[p] = st.pot;
+                }
+            }
+        }
+        // ---- workgroup minimum of the step candidates, header update ----
+        for (int o = 32; o > 0; o >>= 1) cand = fmin(cand, __shfl_xor(cand, o));
+        if ((tid & 63) == 0) red[tid >> 6] = cand;
+        __syncthreads();
+        if (tid == 0) {
+            double nat = red[0];
+            for (int q = 1; q < NT / 64; ++q) nat = fmin(nat, red[q]);
+            ens_finalize_hdr(sh, nat, max_steps, dt_min);
+        }
+        __syncthreads();    // header visible; LDS records free for the next step
+        if (sh.phase == DONE || sh.phase == FAILED) break;
+    }
+    if (tid == 0) {
+        sh.pmin = 0x7FF0000000000000ull;
+        H[slot] = sh;
+        phase_out[slot] = sh.phase;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // admission and retirement
 // staged particle record: m, x[3], v[3], dest index (as double)
 __global__ void k_ens_unpack(Pool P, const double *__restrict__ rec, int n)
@@ -670,6 +848,7 @@ struct SlotInfo {
     int off = 0, n = 0, nsplit = 1, lane = 0;
     std::vector<double> m, x0, v0;
     double eps2 = 0, t_end = 0;
+    int prec = 64;
 };
 
 struct Result {
@@ -685,10 +864,18 @@ struct GraphExec {
     int chunk_grid = 0, item_grid = 0, steps = 0;
 };
 
-// Systems run in two lanes with their own work lists, graphs and streams:
-// lane 1 holds light systems (n <= light_max, e.g. three-body scatterings),
-// lane 0 the rest.  The light lane keeps stepping while a heavy graph runs,
-// so a light system's latency does not grow with the heavy step time.
+// Systems run in lanes with their own work lists and streams:
+//   lane 1  light systems (n <= light_max, e.g. three-body scatterings): one
+//           wave per system in k_ens_fused;
+//   lane 2  light_max < n <= wg_small_max: one 256-thread workgroup per
+//           system in k_ens_wg;
+//   lane 3  wg_small_max < n <= wg_max_n: one kWgLargeThreads workgroup per
+//           system in k_ens_wg;
+//   lane 0  the rest (and every non-light system when wg_steps = 0): the
+//           four step kernels over (system, i-chunk, j-split) work items,
+//           replayed as HIP graphs.
+// The light lane keeps stepping while the other lanes run, so a light
+// system's latency does not grow with the heavy step time.
 struct Lane {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -699,6 +886,8 @@ struct Lane {
     int n_chunks = 0, n_items = 0;
     std::vector<GraphExec> graphs;
     long long steps = 0;
+    int nmax = 0;                 // largest system of a workgroup-fused lane
+    int n32 = 0;                  // its systems of precision 32 (listed first)
 };
 
 struct Engine {
@@ -709,7 +898,7 @@ struct Engine {
     int max_slots = 0;
     SysHdr *d_hdr = nullptr;
     int *d_phase = nullptr, *h_phase = nullptr;
-    Lane lanes[2];
+    Lane lanes[4];
     bool items_dirty = true;
     // staging
     double *h_stage = nullptr, *d_stage = nullptr;
@@ -733,6 +922,10 @@ struct Engine {
     int precision = 64;
     int light_max = 64;
     int fused_steps = 256;            // light lane: steps per fused launch (0: graphs)
+    int wg_steps = 0;                 // medium lanes: steps per workgroup-fused launch (0: graphs;
+                                      // measured slower than the graphs on the 4096-cluster batch)
+    int wg_small_max = 512;           // lane 2: light_max < n <= this, 256 threads per system
+    int wg_max_n = 2048;              // lane 3: wg_small_max < n <= this, 1024 threads per system
     double dt_max = 0.125;
     long long max_steps = 50000000;
     // stats
@@ -768,6 +961,14 @@ size_t bytes_per_particle(int max_split)
 {
     return sizeof(double) * (3 * 6 + 3) + sizeof(JPred) + sizeof(JPred64) +
            sizeof(double) * 7 * (size_t)max_split;
+}
+
+int lane_for(int n)
+{
+    if (n <= X.light_max) return 1;
+    if (X.wg_steps > 0 && n <= X.wg_small_max) return 2;
+    if (X.wg_steps > 0 && n <= X.wg_max_n) return 3;
+    return 0;
 }
 
 int nsplit_for(int n)
@@ -965,7 +1166,8 @@ int admit()
         si.off = off;
         si.n = it->n;
         si.nsplit = nsplit_for(it->n);
-        si.lane = it->n <= X.light_max ? 1 : 0;
+        si.lane = lane_for(it->n);
+        si.prec = it->prec;
         si.m = it->m;
         si.x0 = it->x;
         si.v0 = it->v;
@@ -1029,6 +1231,35 @@ int admit()
 
 int rebuild_items()
 {
+    for (int ln = 2; ln < 4; ++ln) {
+        // one entry per system, largest first (the longest workgroups start first)
+        Lane &L = X.lanes[ln];
+        std::vector<int> sys;
+        for (int s = 0; s < X.max_slots; ++s)
+            if (X.slots[s].busy && X.slots[s].lane == ln) sys.push_back(s);
+        // precision 32 first (one kernel per precision), largest first within each
+        std::stable_sort(sys.begin(), sys.end(), [](int a, int b) {
+            const SlotInfo &p = X.slots[a], &q = X.slots[b];
+            return p.prec != q.prec ? p.prec < q.prec : p.n > q.n;
+        });
+        const int nc = (int)sys.size();
+        if (ensure_items(L, std::max(nc, 1), 1)) return -1;
+        L.nmax = 0;
+        L.n32 = 0;
+        for (int q = 0; q < nc; ++q) {
+            L.h_chunks[q] = Chunk{sys[q], 0};
+            L.nmax = std::max(L.nmax, X.slots[sys[q]].n);
+            if (X.slots[sys[q]].prec == 32) ++L.n32;
+        }
+        L.n_chunks = nc;
+        L.n_items = 0;
+        L.h_counts[0] = nc;
+        L.h_counts[1] = 0;
+        if (nc) ECHK(hipMemcpyAsync(L.d_chunks, L.h_chunks, sizeof(Chunk) * nc,
+                                    hipMemcpyHostToDevice, X.stream));
+        ECHK(hipMemcpyAsync(L.d_counts, L.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice,
+                            X.stream));
+    }
     for (int ln = 0; ln < 2; ++ln) {
         Lane &L = X.lanes[ln];
         int nc = 0, ni = 0;
@@ -1143,6 +1374,19 @@ int g6e_open(long long hbm_budget_bytes, int max_systems)
         ECHK(hipHostMalloc((void **)&L.h_counts, sizeof(int) * 2, hipHostMallocDefault));
         L.steps = 0;
     }
+    // the large workgroup-fused lane needs more than the default dynamic LDS
+    {
+        const int big = (int)ens_wg_lds(std::max(X.wg_max_n, 1), kWgLargeThreads);
+        const int small = (int)ens_wg_lds(std::max(X.wg_small_max, 1), 256);
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wg<kWgLargeThreads, 32>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wg<kWgLargeThreads, 64>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, big));
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wg<256, 32>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, small));
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wg<256, 64>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, small));
+    }
     X.P.max_split = X.max_split;
     const size_t bpp = bytes_per_particle(X.max_split);
     const long long cap = std::max(64LL, hbm_budget_bytes / (long long)bpp);
@@ -1255,6 +1499,7 @@ int g6e_run(int steps)
         if (busy_count() == 0 || done >= steps) break;
         Lane &H = X.lanes[0], &Lt = X.lanes[1];
         const bool heavy = H.n_chunks > 0, light = Lt.n_chunks > 0;
+        const bool med[4] = {false, false, X.lanes[2].n_chunks > 0, X.lanes[3].n_chunks > 0};
         hipGraphExec_t eh = nullptr, el = nullptr;
         if (heavy && get_graph(0, eh)) return -1;
         if (light && !(X.fused_steps > 0 && X.light_max <= 64) && get_graph(1, el)) return -1;
@@ -1264,8 +1509,43 @@ int g6e_run(int steps)
             ECHK(hipGraphLaunch(eh, H.stream));
             ECHK(hipEventRecord(H.ev1, H.stream));
         }
+        for (int ln = 2; ln < 4; ++ln) {
+            if (!med[ln]) continue;
+            Lane &M = X.lanes[ln];
+            ECHK(hipEventRecord(M.ev0, M.stream));
+            const int n32 = M.n32, n64 = M.n_chunks - M.n32;
+            const Chunk *l32 = M.d_chunks, *l64 = M.d_chunks + n32;
+            const int nt = ln == 2 ? 256 : kWgLargeThreads;
+            const size_t lds = ens_wg_lds(M.nmax, nt);
+            if (ln == 2) {
+                if (n32) hipLaunchKernelGGL((k_ens_wg<256, 32>), dim3(n32), dim3(256), lds, M.stream,
+                                            X.P, X.d_hdr, l32, n32, X.d_phase, X.max_steps, 1e-30,
+                                            X.wg_steps, M.nmax);
+                if (n64) hipLaunchKernelGGL((k_ens_wg<256, 64>), dim3(n64), dim3(256), lds, M.stream,
+                                            X.P, X.d_hdr, l64, n64, X.d_phase, X.max_steps, 1e-30,
+                                            X.wg_steps, M.nmax);
+            } else {
+                if (n32) hipLaunchKernelGGL((k_ens_wg<kWgLargeThreads, 32>), dim3(n32),
+                                            dim3(kWgLargeThreads), lds, M.stream, X.P, X.d_hdr,
+                                            l32, n32, X.d_phase, X.max_steps, 1e-30, X.wg_steps,
+                                            M.nmax);
+                if (n64) hipLaunchKernelGGL((k_ens_wg<kWgLargeThreads, 64>), dim3(n64),
+                                            dim3(kWgLargeThreads), lds, M.stream, X.P, X.d_hdr,
+                                            l64, n64, X.d_phase, X.max_steps, 1e-30, X.wg_steps,
+                                            M.nmax);
+            }
+            ECHK(hipGetLastError());
+            ECHK(hipEventRecord(M.ev1, M.stream));
+        }
+        auto others_running = [&]() {
+            if (heavy && hipStreamQuery(H.stream) == hipErrorNotReady) return true;
+            for (int ln = 2; ln < 4; ++ln)
+                if (med[ln] && hipStreamQuery(X.lanes[ln].stream) == hipErrorNotReady) return true;
+            return false;
+        };
+        const bool any_other = heavy || med[2] || med[3];
         if (light) {
-            // keep the light lane stepping until the heavy graph is done
+            // keep the light lane stepping until the other lanes are done
             const bool fused = X.fused_steps > 0 && X.light_max <= 64;
             int replays = 0;
             do {
@@ -1281,24 +1561,36 @@ int g6e_run(int steps)
                 ECHK(hipEventRecord(Lt.ev1, Lt.stream));
                 ECHK(hipStreamSynchronize(Lt.stream));
                 float ms = 0;
-                if (!heavy && hipEventElapsedTime(&ms, Lt.ev0, Lt.ev1) == hipSuccess)
+                if (!any_other && hipEventElapsedTime(&ms, Lt.ev0, Lt.ev1) == hipSuccess)
                     ms_total += ms;
                 Lt.steps += X.steps_per_graph;
                 ++replays;
-            } while (heavy && hipStreamQuery(H.stream) == hipErrorNotReady && replays < 4096);
+            } while (any_other && others_running() && replays < 4096);
         }
+        double ms_other = 0;
         if (heavy) {
             ECHK(hipStreamSynchronize(H.stream));
             float ms = 0;
-            if (hipEventElapsedTime(&ms, H.ev0, H.ev1) == hipSuccess) ms_total += ms;
+            if (hipEventElapsedTime(&ms, H.ev0, H.ev1) == hipSuccess) ms_other = std::max(ms_other, (double)ms);
             H.steps += X.steps_per_graph;
         }
+        for (int ln = 2; ln < 4; ++ln) {
+            if (!med[ln]) continue;
+            Lane &M = X.lanes[ln];
+            ECHK(hipStreamSynchronize(M.stream));
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, M.ev0, M.ev1) == hipSuccess) ms_other = std::max(ms_other, (double)ms);
+            M.steps += X.wg_steps;
+        }
+        ms_total += ms_other;
         ECHK(hipMemcpyAsync(X.h_phase, X.d_phase, sizeof(int) * X.max_slots, hipMemcpyDeviceToHost,
                             X.stream));
         ECHK(hipStreamSynchronize(X.stream));
         X.dev_seconds += ms_total * 1e-3;
-        X.engine_steps += X.steps_per_graph;
-        done += X.steps_per_graph;
+        const int round_steps = heavy ? X.steps_per_graph
+                                      : (med[2] || med[3]) ? X.wg_steps : X.steps_per_graph;
+        X.engine_steps += round_steps;
+        done += round_steps;
     }
     return busy_count() + (int)X.queue.size();
 }
@@ -1353,6 +1645,13 @@ int g6e_set_option(const char *key, long long value)
     else if (k == "max_steps") X.max_steps = std::max(1LL, value);
     else if (k == "light_max") X.light_max = (int)std::max(0LL, value);
     else if (k == "fused_steps") X.fused_steps = (int)std::max(0LL, value);
+    else if (k == "wg_steps") X.wg_steps = (int)std::max(0LL, value);
+    else if (k == "wg_small_max" || k == "wg_max_n") {
+        // records of 64 B per particle must fit the 160 KB of LDS
+        if (X.open) return efail(-1, "%s is fixed while open", k.c_str());
+        const int v = (int)std::max(0LL, std::min(2048LL, value));
+        if (k == "wg_small_max") X.wg_small_max = v; else X.wg_max_n = v;
+    }
     else if (k == "precision") {
         if (value != 32 && value != 64) return efail(-1, "precision must be 32 or 64");
         X.precision = (int)value;
@@ -1371,6 +1670,10 @@ long long g6e_get_option(const char *key)
     if (k == "precision") return X.precision;
     if (k == "light_max") return X.light_max;
     if (k == "fused_steps") return X.fused_steps;
+    if (k == "wg_steps") return X.wg_steps;
+    if (k == "wg_small_max") return X.wg_small_max;
+    if (k == "wg_max_n") return X.wg_max_n;
+    if (k == "medium_steps") return X.lanes[2].steps + X.lanes[3].steps;
     if (k == "light_steps") return X.lanes[1].steps;
     if (k == "heavy_steps") return X.lanes[0].steps;
     if (k == "pool_capacity") return X.P.cap;

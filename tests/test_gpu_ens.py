@@ -38,7 +38,7 @@ def _jobs(seed, sizes, t_end, eta=0.05):
 
 
 DEFAULTS = dict(jsplit_target=512, precision=64, max_steps=50_000_000, steps_per_graph=8,
-                fused_steps=256, light_max=64)
+                fused_steps=256, light_max=64, wg_steps=0)
 PATHS = pytest.mark.parametrize("fused", [256, 0], ids=["fused", "graphs"])
 
 
@@ -163,3 +163,35 @@ def test_fused_kernel_equals_step_graphs_bitwise(prec):
         np.testing.assert_array_equal(f["x"], g["x"])
         np.testing.assert_array_equal(f["v"], g["v"])
         assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("jsplit", [0, 256])
+def test_wg_kernel_equals_step_graphs_bitwise(prec, jsplit):
+    """The workgroup-fused kernel (one workgroup per system of 64 < n <= 2048,
+    whole steps per launch) performs the step kernels' arithmetic in their
+    order, j-splits included: same trajectories, step counts and energies bit
+    for bit, in both size classes (256 and 1024 threads per system) and beside
+    a system too large for it (graph path) and light ones (wave-fused)."""
+    jobs = _jobs(10, [65, 100, 300, 512, 513, 900, 2048, 2100, 3, 40], t_end=1.0 / 16)
+    wg, _ = _run(jobs, precision=prec, jsplit_target=jsplit, wg_steps=32)
+    graphs, _ = _run(jobs, precision=prec, jsplit_target=jsplit, wg_steps=0)
+    for j, f, g in zip(jobs, wg, graphs):
+        assert not f["failed"] and f["t"] == g["t"] == 1.0 / 16, len(j["m"])
+        assert f["steps"] == g["steps"], len(j["m"])
+        np.testing.assert_array_equal(f["x"], g["x"])
+        np.testing.assert_array_equal(f["v"], g["v"])
+        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
+
+
+def test_wg_short_launches_resume_bitwise():
+    """Stopping a workgroup-fused launch every step and resuming from the
+    header gives the same trajectory as 32 steps per launch."""
+    jobs = _jobs(11, [130, 700], t_end=1.0 / 32)
+    a, _ = _run(jobs, precision=32, wg_steps=1)
+    b, _ = _run(jobs, precision=32, wg_steps=32)
+    for f, g in zip(a, b):
+        assert f["steps"] == g["steps"]
+        np.testing.assert_array_equal(f["x"], g["x"])
+        assert f["e1"] == g["e1"]
+
