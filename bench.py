@@ -88,17 +88,23 @@ def cpu_baseline(inp, target_s=10.0):
 
 
 def ensemble_legs(seconds=2.0):
-    """Batched ensemble (4096 Plummer clusters, N ~ U[32, 2048], integrated
-    to t = 1/32) and the served Poisson stream (2000 req/s, 70 % binary-single
-    scatterings to t = 32, 30 % clusters N ~ U[256, 4096] to t = 1/32)."""
+    """Batched ensembles and the served stream.
+
+    batch_fp32_t1: 4096 Plummer clusters, N ~ U[32, 2048], FP32 pairs,
+    integrated to t = 1 (the batched-ensemble configuration as specified);
+    batch_fp64: the same clusters in FP64 to t = 1/32; both with engine shapes
+    from the ensemble autotuner.  serving: a Poisson stream of 2000 req/s,
+    70 % binary-single scatterings (FP64) to t = 32, 30 % clusters
+    N ~ U[256, 4096] (FP32) to t = 1/32."""
     from starlab_amd import serve
+    keys = ("jobs", "wall_s", "systems_per_s", "pairs_per_s", "p50_s", "p99_s", "de_median",
+            "de_max", "failed", "tuned")
     out = {}
-    for prec in (32, 64):
-        batch = serve.cluster_batch(4096, 32, 2048, 4, 1.0 / 32, prec=prec)
-        r = serve.run_batch(batch)
-        out[f"batch_fp{prec}"] = {k: r[k] for k in ("jobs", "wall_s", "systems_per_s",
-                                                    "pairs_per_s", "p50_s", "p99_s",
-                                                    "de_median", "de_max", "failed")}
+    for name, prec, t_end in (("batch_fp32_t1", 32, 1.0), ("batch_fp64", 64, 1.0 / 32)):
+        batch = serve.cluster_batch(4096, 32, 2048, 4, t_end, prec=prec)
+        r = serve.run_batch(batch, tune=True)
+        out[name] = {k: r[k] for k in keys}
+        out[name]["t_end"] = t_end
     jobs = serve.serving_mix(int(2000 * seconds), 5)
     r = serve.run_poisson(jobs, 2000.0)
     out["serving"] = r

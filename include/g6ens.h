@@ -69,7 +69,8 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * "precision" (pair arithmetic of jobs submitted afterwards: 64 = FP64 in the
  * CPU oracle's operation order, the default; 32 = FP32 pairs on
  * double-single positions with a noise-floored step criterion),
- * "max_steps" (per system; beyond it the job fails),
+ * "max_steps" (per system; beyond it the job fails), "dt_max_bits" (largest
+ * block step 2^bits, default -3),
  * "light_max" (systems with n <= light_max run in the light lane, which
  * keeps stepping on its own stream while a heavy-lane graph runs; default
  * 64), "fused_steps" (with light_max <= 64 the light lane runs a fused
@@ -80,6 +81,7 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * many steps per launch, bitwise equal to the step graphs; default 0 = step
  * graphs, which measured faster on mixed cluster batches), "wg_small_max",
  * "wg_max_n" (<= 2048; fixed while open); read-only "medium_steps".
+ * Options are library-wide and keep their values across g6e_close.
  * Returns 0, or -1 for an unknown key. */
 int       g6e_set_option(const char *key, long long value);
 long long g6e_get_option(const char *key);

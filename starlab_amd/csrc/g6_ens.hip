@@ -1666,6 +1666,7 @@ long long g6e_get_option(const char *key)
     if (k == "jsplit_target") return X.jsplit_target;
     if (k == "max_split") return X.max_split;
     if (k == "lookahead") return X.lookahead;
+    if (k == "dt_max_bits") return std::ilogb(X.dt_max);
     if (k == "max_steps") return X.max_steps;
     if (k == "precision") return X.precision;
     if (k == "light_max") return X.light_max;

@@ -31,21 +31,34 @@ def _dp(a):
 
 
 class Ensemble:
+    """One open engine.  Options are library-wide and outlive g6e_close; this
+    wrapper puts every option it changed back to its earlier value on close."""
+
     def __init__(self, hbm_budget_bytes: int, max_systems: int, **options):
         self.L = load()
+        self._saved = {}
         for k, v in options.items():
             self.option(k, v)
         rc = self.L.g6e_open(int(hbm_budget_bytes), int(max_systems))
         if rc:
+            self._restore()
             raise RuntimeError(f"g6e_open failed: {self.last_error()}")
         self._n = {}
 
     def close(self):
         self.L.g6e_close()
+        self._restore()
+
+    def _restore(self):
+        for k, v in self._saved.items():
+            self.L.g6e_set_option(k.encode(), int(v))
+        self._saved = {}
 
     def option(self, key, value=None):
         if value is None:
             return self.L.g6e_get_option(key.encode())
+        if key not in self._saved:
+            self._saved[key] = self.L.g6e_get_option(key.encode())
         if self.L.g6e_set_option(key.encode(), int(value)):
             raise KeyError(key)
         return value
@@ -104,6 +117,61 @@ class Ensemble:
                 for jid in self.poll():
                     out[jid] = self.result(jid)
                 return out
+
+
+# ---------------------------------------------------------------------------
+# shape autotuner
+# ---------------------------------------------------------------------------
+
+# Engine options the tuner chooses among: j per split within a system (the
+# split-j shape of the step kernels), engine steps per graph replay, and
+# whether medium systems run in the workgroup-fused kernel.
+TUNE_SPACE = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32), wg_steps=(0, 32))
+
+
+def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192,
+             space=None, fixed=None):
+    """Choose the ensemble engine's shape options for a workload.
+
+    Takes a size-stratified sample of `jobs` (dicts with m, x, v, eta, eps2
+    and optionally prec), runs copies of it to t = t_probe once per candidate
+    option set (the product of `space`, default TUNE_SPACE) on an idle engine,
+    and returns (best options, {options tuple: seconds per pair interaction},
+    seconds spent).  The jobs themselves are not touched.  `fixed` options
+    are applied to every candidate."""
+    import itertools
+    import time as _time
+    space = TUNE_SPACE if space is None else space
+    order = sorted(range(len(jobs)), key=lambda k: len(jobs[k]["m"]))
+    step = max(1, len(order) // max(1, sample))
+    probe = [jobs[k] for k in order[::step]][:sample]
+    if budget is None:
+        # room for the sample twice over (~1.2 KB of pool per particle): opening
+        # a pool far larger than the probe needs costs more than the probe
+        budget = max(1 << 30, 2 * 1280 * sum(len(j["m"]) for j in probe))
+    keys = sorted(space)
+    timings = {}
+    t_start = _time.perf_counter()
+    best, best_cost = None, float("inf")
+    for values in itertools.product(*(space[k] for k in keys)):
+        opts = dict(zip(keys, values))
+        opts.update(fixed or {})
+        eng = Ensemble(budget, max_systems, **opts)
+        try:
+            for j in probe:
+                eng.option("precision", j.get("prec", 64))
+                eng.submit(j["m"], j["x"], j["v"], t_probe, j["eta"], j["eps2"])
+            t0 = _time.perf_counter()
+            eng.run_until_idle(chunk=64)
+            wall = _time.perf_counter() - t0
+            pairs = eng.stats()["pairs"]
+        finally:
+            eng.close()
+        cost = wall / max(pairs, 1.0)
+        timings[values] = cost
+        if cost < best_cost:
+            best, best_cost = dict(zip(keys, values)), cost
+    return best, timings, _time.perf_counter() - t_start
 
 
 # ---------------------------------------------------------------------------

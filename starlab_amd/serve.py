@@ -12,7 +12,7 @@ import time
 
 import numpy as np
 
-from .ens import Ensemble, binary_single, plummer_cluster
+from .ens import Ensemble, autotune, binary_single, plummer_cluster
 
 GB = 1 << 30
 
@@ -58,9 +58,17 @@ def _pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
 
-def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, **opt):
-    """Submit every job at t = 0 and run to completion."""
-    eng = Ensemble(budget, max_systems, steps_per_graph=chunk, **opt)
+def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, **opt):
+    """Submit every job at t = 0 and run to completion.  With tune=True the
+    engine's shape options come from ens.autotune on a sample of the jobs
+    (its time is reported, not counted in wall_s)."""
+    tuned = None
+    if tune:
+        best, timings, secs = autotune(jobs, max_systems=max_systems)
+        opt = {**opt, **best}
+        chunk = best.get("steps_per_graph", chunk)
+        tuned = dict(options=best, seconds=round(secs, 3), candidates=len(timings))
+    eng = Ensemble(budget, max_systems, **{"steps_per_graph": chunk, **opt})
     try:
         t0 = time.perf_counter()
         ids = [_submit(eng, j) for j in jobs]
@@ -85,7 +93,7 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, **opt):
                 device_s=st["device_seconds"], engine_steps=st["engine_steps"],
                 p50_s=_pct(lat, 50), p99_s=_pct(lat, 99),
                 failed=sum(results[i]["failed"] for i in ids),
-                de_median=_pct(de, 50), de_max=float(np.max(de)))
+                de_median=_pct(de, 50), de_max=float(np.max(de)), tuned=tuned)
 
 
 def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, **opt):

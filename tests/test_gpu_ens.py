@@ -195,3 +195,25 @@ def test_wg_short_launches_resume_bitwise():
         np.testing.assert_array_equal(f["x"], g["x"])
         assert f["e1"] == g["e1"]
 
+
+def test_autotune_chooses_from_its_space():
+    """The ensemble shape tuner times every candidate on copies of a sample
+    and returns one of them; the jobs it was given are untouched."""
+    from starlab_amd.ens import autotune
+    jobs = _jobs(12, [3, 40, 300, 700, 1500], t_end=1.0 / 16)
+    x0 = [j["x"].copy() for j in jobs]
+    space = dict(jsplit_target=(256, 1024), wg_steps=(0, 32))
+    try:
+        best, timings, secs = autotune(jobs, sample=5, t_probe=1.0 / 256, budget=2 * GB,
+                                       max_systems=64, space=space)
+    finally:
+        from starlab_amd import load
+        L = load()
+        for k, v in DEFAULTS.items():            # engine options outlive the engine
+            L.g6e_set_option(k.encode(), int(v))
+    assert best["jsplit_target"] in (256, 1024) and best["wg_steps"] in (0, 32)
+    assert len(timings) == 4 and all(t > 0 for t in timings.values())
+    assert secs > 0
+    for j, x in zip(jobs, x0):
+        np.testing.assert_array_equal(j["x"], x)
+
