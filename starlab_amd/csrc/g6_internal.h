@@ -50,7 +50,8 @@ struct Partials {
     double *c[7];          // ax ay az jx jy jz pot
     float *best;           // smallest FP32 r^2 (+inf: none)
     int *btile;            // first address of the j-tile holding it
-    float *second;         // runner-up FP32 r^2
+    float *second;         // runner-up FP32 r^2 outside the best tile (the reduction
+                           // adds the best tile's own runner-up when it rescans it)
     int ld;                // stride between splits (= IBlock ld)
 };
 

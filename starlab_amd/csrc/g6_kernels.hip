@@ -144,16 +144,17 @@ hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
 // of `tile` (64, 128 or 256) j: FP32 partial sums per tile, folded into FP64
 // at the tile end.
 //
-// Nearest neighbour: per lane, the smallest FP32 r^2 (v_min) and the
-// runner-up (v_med3 of best, runner-up, r^2) in each tile; at the tile end
-// the lane remembers which tile holds its best.  The reduction kernel finds
-// the address inside that tile by recomputing r^2 with the same FP32
-// operations.  Positive floats compare exactly, and ties resolve to the
-// lowest address as in the CPU path.
+// Nearest neighbour: per lane, the smallest FP32 r^2 of each tile (one
+// v_min per pair).  At the tile end the lane keeps the best tile and, as the
+// runner-up, the smallest best of the other tiles.  The reduction kernel
+// recomputes r^2 over the winning tile with the same FP32 operations, which
+// gives the address (first match, so ties resolve to the lowest address as in
+// the CPU path) and the tile's own runner-up.  Positive floats compare
+// exactly.
 // ---------------------------------------------------------------------------
 struct Acc32 {
     float ax, ay, az, jx, jy, jz, kx, ky, kz, p;
-    float best, second;
+    float best;
 };
 
 // FP32 r^2 of a double-single difference: ((eps2 + dz*dz) + dy*dy) + dx*dx,
@@ -210,7 +211,6 @@ __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float
     a.kx = __builtin_fmaf(q, dx, a.kx);
     a.ky = __builtin_fmaf(q, dy, a.ky);
     a.kz = __builtin_fmaf(q, dz, a.kz);
-    a.second = __builtin_amdgcn_fmed3f(a.best, a.second, rnn);
     a.best = __builtin_fminf(a.best, rnn);
 }
 
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
         const int tn = min(tile, jend - t);
         Acc32 a;
         a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.kx = a.ky = a.kz = a.p = 0.f;
-        a.best = a.second = __builtin_inff();
+        a.best = __builtin_inff();
         const unsigned rel = (unsigned)(excl - t);
         const bool mine = rel < (unsigned)tn;
         if (__ballot(mine) == 0)
@@ -298,8 +298,9 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
         A4 += (double)a.jy - 3.0 * (double)a.ky;
         A5 += (double)a.jz - 3.0 * (double)a.kz;
         A6 += (double)a.p;
+        // runner-up: the smallest best among the other tiles
         if (a.best < rbest) {
-            rsecond = __builtin_fminf(rbest, a.second);
+            rsecond = __builtin_fminf(rsecond, rbest);
             rbest = a.best;
             rtile = t;
         } else {
@@ -413,7 +414,6 @@ __device__ __forceinline__ void pair_v(float jxh, float jyh, float jzh, float jx
     a.kx = __builtin_fmaf(q, dx, a.kx);
     a.ky = __builtin_fmaf(q, dy, a.ky);
     a.kz = __builtin_fmaf(q, dz, a.kz);
-    a.second = __builtin_amdgcn_fmed3f(a.best, a.second, rnn);
     a.best = __builtin_fminf(a.best, rnn);
 }
 
@@ -441,7 +441,7 @@ __device__ __forceinline__ void tile_lds(const float4 *__restrict__ sj, int tn,
 __device__ __forceinline__ void acc_reset(Acc32 &a)
 {
     a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.kx = a.ky = a.kz = a.p = 0.f;
-    a.best = a.second = __builtin_inff();
+    a.best = __builtin_inff();
 }
 
 struct Acc64 {
@@ -460,7 +460,7 @@ __device__ __forceinline__ void acc_fold(Acc64 &A, const Acc32 &a, int t)
     A.c[5] += (double)a.jz - 3.0 * (double)a.kz;
     A.c[6] += (double)a.p;
     if (a.best < A.best) {
-        A.second = __builtin_fminf(A.best, a.second);
+        A.second = __builtin_fminf(A.second, A.best);
         A.best = a.best;
         A.tile = t;
     } else {
@@ -715,23 +715,33 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         const uint32_t bb = __float_as_uint(best);
         const int gshift = lane & ~(L - 1);
         int found = INT_MAX;
+        // the whole tile: first address matching best, and the two smallest
+        // r^2 (as a multiset) for the tile's own runner-up
+        float m1 = __builtin_inff(), m2 = __builtin_inff();
         for (int j0 = tile; j0 < tend; j0 += L) {
             const int j = j0 + l;
             bool hit = false;
-            if (j < tend) {
+            if (j < tend && j != excl) {
                 const JPred &J = jpred[j];
                 float dx, dy, dz;
                 const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih, xil,
                                        yil, zil, eps2, dx, dy, dz);
-                hit = j != excl && __float_as_uint(r2) == bb;
+                hit = __float_as_uint(r2) == bb;
+                m2 = __builtin_amdgcn_fmed3f(m1, m2, r2);
+                m1 = __builtin_fminf(m1, r2);
             }
             unsigned long long m = __ballot(hit) >> gshift;
             if (L < 64) m &= (1ull << L) - 1ull;
-            if (m) {
-                found = j0 + __ffsll((long long)m) - 1;
-                break;
-            }
+            if (m && found == INT_MAX) found = j0 + __ffsll((long long)m) - 1;
         }
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) {
+            const float o1 = __shfl_xor(m1, off);
+            const float o2 = __shfl_xor(m2, off);
+            m2 = __builtin_fminf(__builtin_fmaxf(m1, o1), __builtin_fminf(m2, o2));
+            m1 = __builtin_fminf(m1, o1);
+        }
+        second = __builtin_fminf(second, m2);
         if (found == INT_MAX) ambiguous = true;
         else nn = jindex[found];
         if (!(second > best * (1.0f + kNNBand))) ambiguous = true;
