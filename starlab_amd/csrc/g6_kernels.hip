@@ -175,6 +175,15 @@ __device__ __forceinline__ float ds_r2(float xjh, float yjh, float zjh, float xj
     return __builtin_fmaf(dx, dx, r2);
 }
 
+// A wave-uniform float copied into a VGPR, for kernels where VALU
+// instructions with that SGPR operand measured slower (k_pot, k_force_lds).
+__device__ __forceinline__ float to_vgpr(float s)
+{
+    float v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+
 template <bool MASK, bool ZSKIP>
 __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float xih,
                                      float yih, float zih, float xil, float yil,
@@ -270,6 +279,8 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
     const float xil = ib.xl[ic], yil = ib.yl[ic], zil = ib.zl[ic];
     const float vxi = ib.vx[ic], vyi = ib.vy[ic], vzi = ib.vz[ic];
     const int excl = i < ni ? ib.excl[ic] : -1;
+    // eps2 stays in its SGPR here: a copy in a VGPR measured ~0.5% slower
+    // (the j fields, not this constant, set the SGPR-operand cost)
 
     const int jbeg = split * jchunk;
     const int jend = min(jbeg + jchunk, nj);
@@ -497,9 +508,7 @@ __global__ __launch_bounds__(256) void k_force_lds(
     const IPos I1 = load_i(ib, i1 < ni ? i1 : ni - 1);
     const int ex0 = i0 < ni ? ib.excl[i0] : -1;
     const int ex1 = i1 < ni ? ib.excl[i1] : -1;
-    // eps2 in a VGPR: VOP2 arithmetic with an SGPR operand runs at half rate
-    float eps2v;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(eps2v) : "s"(eps2));
+    const float eps2v = to_vgpr(eps2);
 
     const int jbeg = split * jchunk;
     const int jend = min(jbeg + jchunk, nj);
@@ -874,6 +883,7 @@ __global__ __launch_bounds__(64) void k_pot(
     const float xih = ib.xh[ic], yih = ib.yh[ic], zih = ib.zh[ic];
     const float xil = ib.xl[ic], yil = ib.yl[ic], zil = ib.zl[ic];
     const int excl = i < ni ? ib.excl[ic] : -1;
+    const float eps2v = to_vgpr(eps2);
     const int jbeg = split * jchunk;
     const int jend = min(jbeg + jchunk, nj);
     double P = 0.0;
@@ -882,9 +892,9 @@ __global__ __launch_bounds__(64) void k_pot(
         const unsigned rel = (unsigned)(excl - t);
         float p;
         if (__ballot(rel < (unsigned)tn) == 0)
-            p = tile_pot<false>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil, eps2);
+            p = tile_pot<false>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil, eps2v);
         else
-            p = tile_pot<true>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil, eps2);
+            p = tile_pot<true>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil, eps2v);
         P += (double)p;
     }
     if (i < ni) part[(size_t)split * ld + i] = P;
