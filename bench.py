@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -159,6 +160,20 @@ def grape4_leg(inp, xp, vp, npipe=2048):
             "calls": len(blocks), "pipes_per_call": npipe, "distinct_i_per_call": slots}
 
 
+def abi_leg(nj=N, blocks=200, sizes=(1, 16, 256, 2048)):
+    """kira's per-block sequence through the GRAPE-6 ABI (set_j of the previous
+    block, set_ti, firsthalf, lasthalf2) from the compiled C caller
+    starlab_amd/abi_bench: microseconds per block, host-to-host."""
+    exe = os.path.join(ROOT, "starlab_amd", "abi_bench")
+    r = subprocess.run([exe, str(nj), str(blocks), *map(str, sizes)], capture_output=True,
+                       text=True, timeout=300)
+    if r.returncode:
+        raise RuntimeError(f"abi_bench failed ({r.returncode}): {r.stderr.strip()}")
+    keys = ("ni", "us_per_block", "us_firsthalf", "us_lasthalf", "pairs_per_s")
+    return {"nj": nj, "blocks": blocks,
+            "per_block": [{k: json.loads(line)[k] for k in keys} for line in r.stdout.splitlines()]}
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "force_n65536_traffic.json")
     try:
@@ -295,6 +310,8 @@ def main():
     if not a.no_aux:
         log("bench: GRAPE-4 ABI sweep")
         out["aux"]["grape4_abi_sweep"] = grape4_leg(inp, xp, vp)
+        log("bench: GRAPE-6 ABI block latency (C caller)")
+        out["aux"]["grape6_abi_blocks"] = abi_leg()
     if not a.no_ens:
         log("bench: ensemble and serving legs")
         ens = ensemble_legs()

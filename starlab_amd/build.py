@@ -5,6 +5,8 @@
 libg6.so  exports the GRAPE-6 ABI (include/grape6.h), its extensions
           (include/g6x.h) and the ensemble engine (include/g6ens.h);
 libg4x.so exports the GRAPE-4 ABI (include/grape4.h) and include/g4x.h.
+abi_bench is a plain C program (csrc/abi_bench.c) linked against libg6.so
+          that times kira's per-block call sequence.
 Both link the same force kernels and engine core; each has its own engine
 state.  Symbols other than the declared C entry points are hidden.
 
@@ -27,6 +29,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libg6.so")
 LIBG4 = os.path.join(HERE, "libg4x.so")
+ABI_BENCH = os.path.join(HERE, "abi_bench")
 
 LIBS = {
     LIB: ["g6_kernels.hip", "g6_core.hip", "g6_engine.hip", "g6_ens.hip"],
@@ -76,6 +79,14 @@ def build(verbose: bool = False) -> str:
         _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
               *[obj(s) for s in parts]], verbose)
         os.replace(tmp, lib)
+    # the per-block ABI timing program: a plain C caller of libg6.so
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        raise RuntimeError("no C compiler (gcc) for abi_bench")
+    tmp = ABI_BENCH + ".tmp"
+    _run([cc, "-O2", "-std=c99", inc, os.path.join(CSRC, "abi_bench.c"), f"-L{HERE}",
+          "-Wl,-rpath,$ORIGIN", "-lg6", "-lm", "-o", tmp], verbose)
+    os.replace(tmp, ABI_BENCH)
     return LIB
 
 

@@ -96,8 +96,8 @@ struct Engine {
     int *d_jindex = nullptr;
 
     int scap = 0;               // scatter staging capacity (records)
-    char *h_stage = nullptr;    // pinned: n records, n addresses, n index fields
-    char *d_stage = nullptr;
+    char *h_stage = nullptr;    // pinned: n records, n addresses, n index fields;
+    const char *h_stage_dev = nullptr;   // its device address (k_scatter reads it)
 
     bool pred_valid = false;
     double pred_ti = 0;
@@ -117,7 +117,14 @@ struct Engine {
     // ---- results ----------------------------------------------------------
     int rcap = 0;
     char *d_res = nullptr;      // acc[3n] jerk[3n] pot[n] nn[n] flags[n]
-    char *h_res = nullptr;      // pinned
+    char *h_res = nullptr;      // pinned; the ABI path's reduction writes it directly
+    char *h_res_dev = nullptr;  // its device address
+
+    // Pinned buffers read (h_stage, h_ibuf) or written (h_res) by queued
+    // device work: set when such work is queued, cleared once the stream has
+    // drained past it.  Host code that rewrites or frees one waits first.
+    bool host_busy = false;
+    unsigned long long busy_seq = 0, done_seq = 0;
 
     // ---- neighbour lists --------------------------------------------------
     size_t nbcap_entries = 0;

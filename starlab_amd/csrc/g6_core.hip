@@ -188,16 +188,36 @@ int ensure_device_j(int need)
     return 0;
 }
 
+// Wait until no queued device work reads or writes the pinned staging
+// buffers.  In kira's call order lasthalf has already waited, so this costs
+// nothing.
+int wait_host_buffers()
+{
+    if (!E.host_busy) return 0;
+    HIPCHK(hipStreamSynchronize(E.stream));
+    E.host_busy = false;
+    return 0;
+}
+
+void mark_host_busy()
+{
+    E.host_busy = true;
+    ++E.busy_seq;
+}
+
 int ensure_stage(int n)
 {
     if (n <= E.scap) return 0;
     int cap = std::max(n, std::max(256, E.scap * 2));
+    if (wait_host_buffers()) return -1;
     hfree(E.h_stage);
-    dfree(E.d_stage);
+    E.h_stage_dev = nullptr;
     E.scap = 0;
     const size_t bytes = (sizeof(JRaw) + 2 * sizeof(int)) * (size_t)cap;
     HIPCHK(hipHostMalloc((void **)&E.h_stage, bytes, hipHostMallocDefault));
-    HIPCHK(hipMalloc((void **)&E.d_stage, bytes));
+    void *dev = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&dev, E.h_stage, 0));
+    E.h_stage_dev = static_cast<const char *>(dev);
     E.scap = cap;
     return 0;
 }
@@ -220,10 +240,10 @@ int flush_j(int nj)
         HIPCHK(hipMemcpyAsync(E.d_jindex, E.h_jindex, sizeof(int) * (size_t)E.n_addr,
                               hipMemcpyHostToDevice, E.stream));
     } else {
-        if (ensure_stage(nd)) return -1;
-        // the staging buffers may still be read by an earlier copy
-        HIPCHK(hipStreamSynchronize(E.stream));
-        // one packed upload: records, then addresses, then index fields
+        // the staging buffer may still be read by an earlier scatter
+        if (wait_host_buffers() || ensure_stage(nd)) return -1;
+        // one packed list, read by k_scatter straight from pinned memory:
+        // records, then addresses, then index fields
         JRaw *rec = reinterpret_cast<JRaw *>(E.h_stage);
         int *addr = reinterpret_cast<int *>(rec + nd);
         for (int k = 0; k < nd; ++k) {
@@ -232,12 +252,11 @@ int flush_j(int nj)
             addr[k] = a;
             addr[nd + k] = E.h_jindex[a];
         }
-        HIPCHK(hipMemcpyAsync(E.d_stage, E.h_stage, (sizeof(JRaw) + 2 * sizeof(int)) * (size_t)nd,
-                              hipMemcpyHostToDevice, E.stream));
         KTimer t(K_SCATTER);
         HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
-                              reinterpret_cast<const JRaw *>(E.d_stage), nd));
+                              reinterpret_cast<const JRaw *>(E.h_stage_dev), nd));
         t.stop();
+        mark_host_busy();
     }
     for (int a : E.dirty) E.dirty_flag[a] = 0;
     E.dirty.clear();
@@ -272,8 +291,10 @@ int ensure_i(int ni)
 {
     if (ni <= E.icap) return 0;
     int cap = std::max((ni + 63) & ~63, 256);
+    if (wait_host_buffers()) return -1;
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
     hfree(E.h_res); dfree(E.d_res);
+    E.h_res_dev = nullptr;
     E.icap = 0;
     HIPCHK(hipHostMalloc((void **)&E.h_ibuf, ibuf_bytes(cap), hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E.d_ibuf, ibuf_bytes(cap)));
@@ -282,6 +303,9 @@ int ensure_i(int ni)
     const size_t rb = results_bytes(cap);
     HIPCHK(hipHostMalloc((void **)&E.h_res, rb, hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E.d_res, rb));
+    void *dev = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&dev, E.h_res, 0));
+    E.h_res_dev = static_cast<char *>(dev);
     E.icap = cap;
     return 0;
 }
@@ -491,11 +515,13 @@ int do_close()
     (void)hipStreamSynchronize(E.stream);
     harvest_timing();
     free_device_j();
-    dfree(E.d_stage);
     hfree(E.h_stage);
+    E.h_stage_dev = nullptr;
     E.scap = 0;
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
     hfree(E.h_res); dfree(E.d_res);
+    E.h_res_dev = nullptr;
+    E.host_busy = false;
     E.icap = 0;
     dfree(E.d_part); E.pbytes = 0;
     dfree(E.d_nbcount); dfree(E.d_nblist); hfree(E.h_nbcount); hfree(E.h_nblist);
@@ -576,10 +602,9 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     if (covered != ni) return fail(-1, "%s: i-groups cover %d of %d slots", who, covered, ni);
     if (flush_j(nj)) return -1;
     if (predict(nj, false)) return -1;
-    if (ensure_i(ni)) return -1;
+    // the pinned i buffer may still be read by the previous upload
+    if (wait_host_buffers() || ensure_i(ni)) return -1;
     const int ld = (ni + 63) & ~63;
-    // The pinned i buffer may still be read by the previous upload.
-    HIPCHK(hipStreamSynchronize(E.stream));
     float *f = reinterpret_cast<float *>(E.h_ibuf);
     int *ex = reinterpret_cast<int *>(f + 9 * (size_t)ld);
     double *d = reinterpret_cast<double *>(E.h_ibuf + (size_t)ld * 40);
@@ -602,7 +627,8 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     }
     HIPCHK(hipMemcpyAsync(E.d_ibuf, E.h_ibuf, ibuf_bytes(ld), hipMemcpyHostToDevice, E.stream));
     const IBlock ib = iblock_at(E.d_ibuf, ld);
-    const Results r = results_at(E.d_res, ni);
+    // the reduction writes the results straight into pinned host memory
+    const Results r = results_at(E.h_res_dev, ni);
     for (int g = 0; g < ngroups; ++g) {
         // a group is the block shifted by its first slot (same stride); the
         // partial-sum buffer is reused in stream order
@@ -616,8 +642,9 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         gr.nn += o; gr.flags += o;
         if (enqueue_force(groups[g].n, nj, groups[g].eps2, gb, gr, groups[g].zskip)) return -1;
     }
-    HIPCHK(hipMemcpyAsync(E.h_res, E.d_res, results_bytes(ni), hipMemcpyDeviceToHost, E.stream));
+    mark_host_busy();
     HIPCHK(hipEventRecord(E.ev_done, E.stream));
+    E.done_seq = E.busy_seq;
     E.last_ni = ni;
     E.last_nj = nj;
     E.last_include_self = include_self;
@@ -642,6 +669,8 @@ const Results *calc_wait(const char *who)
         fail(-1, "%s: %s", who, hipGetErrorString(e));
         return nullptr;
     }
+    // nothing touching the pinned buffers was queued after ev_done
+    if (E.done_seq == E.busy_seq) E.host_busy = false;
     harvest_timing();
     h = results_at(E.h_res, E.last_ni);
     E.last_refinements = count_refined(h, E.last_ni);

@@ -136,3 +136,16 @@ def test_plain_c_caller_links_and_runs(tmp_path):
     assert r.returncode == 0
     lines = r.stdout.split()
     assert lines[0] == "2048"
+
+
+def test_abi_bench_is_built_and_fails_loudly_without_device():
+    """The compiled per-block ABI timing program links libg6.so and reports a
+    failed g6_open_ (exit status 1) when no device is present."""
+    exe = os.path.join(ROOT, "starlab_amd", "abi_bench")
+    assert os.access(exe, os.X_OK), "build() makes starlab_amd/abi_bench"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([exe, "1024", "2", "16"], capture_output=True, text=True, timeout=120)
+    if r.returncode == 0:
+        pytest.skip("a device is present")
+    assert r.returncode == 1 and "g6_open_ failed" in r.stderr
