@@ -120,11 +120,14 @@ struct Engine {
     char *h_res = nullptr;      // pinned; the ABI path's reduction writes it directly
     char *h_res_dev = nullptr;  // its device address
 
-    // Pinned buffers read (h_stage, h_ibuf) or written (h_res) by queued
-    // device work: set when such work is queued, cleared once the stream has
-    // drained past it.  Host code that rewrites or frees one waits first.
-    bool host_busy = false;
-    unsigned long long busy_seq = 0, done_seq = 0;
+    // Pinned buffers used by queued device work: h_stage (read by k_scatter),
+    // h_ibuf (read by the i upload), h_res (written by the reduction).  Each
+    // such enqueue takes the next op_seq; the buffer remembers it.  synced_seq
+    // is the last op_seq known complete (lasthalf's wait on ev_done, which
+    // follows done_seq).  Host code that rewrites or frees a buffer waits only
+    // if that buffer's op is newer than synced_seq.
+    unsigned long long op_seq = 0, synced_seq = 0, done_seq = 0;
+    unsigned long long stage_seq = 0, ibuf_seq = 0, res_seq = 0;
 
     // ---- neighbour lists --------------------------------------------------
     size_t nbcap_entries = 0;

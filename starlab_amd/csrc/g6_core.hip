@@ -188,28 +188,24 @@ int ensure_device_j(int need)
     return 0;
 }
 
-// Wait until no queued device work reads or writes the pinned staging
-// buffers.  In kira's call order lasthalf has already waited, so this costs
-// nothing.
-int wait_host_buffers()
+// Wait until the queued op numbered seq (one that reads or writes a pinned
+// buffer) has completed.  In kira's call order lasthalf has already waited
+// for it, so this costs nothing.
+int wait_queued(unsigned long long seq)
 {
-    if (!E.host_busy) return 0;
+    if (seq <= E.synced_seq) return 0;
     HIPCHK(hipStreamSynchronize(E.stream));
-    E.host_busy = false;
+    E.synced_seq = E.op_seq;
     return 0;
 }
 
-void mark_host_busy()
-{
-    E.host_busy = true;
-    ++E.busy_seq;
-}
+unsigned long long mark_queued() { return ++E.op_seq; }
 
 int ensure_stage(int n)
 {
     if (n <= E.scap) return 0;
     int cap = std::max(n, std::max(256, E.scap * 2));
-    if (wait_host_buffers()) return -1;
+    if (wait_queued(E.stage_seq)) return -1;
     hfree(E.h_stage);
     E.h_stage_dev = nullptr;
     E.scap = 0;
@@ -241,7 +237,7 @@ int flush_j(int nj)
                               hipMemcpyHostToDevice, E.stream));
     } else {
         // the staging buffer may still be read by an earlier scatter
-        if (wait_host_buffers() || ensure_stage(nd)) return -1;
+        if (wait_queued(E.stage_seq) || ensure_stage(nd)) return -1;
         // one packed list, read by k_scatter straight from pinned memory:
         // records, then addresses, then index fields
         JRaw *rec = reinterpret_cast<JRaw *>(E.h_stage);
@@ -256,7 +252,7 @@ int flush_j(int nj)
         HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
                               reinterpret_cast<const JRaw *>(E.h_stage_dev), nd));
         t.stop();
-        mark_host_busy();
+        E.stage_seq = mark_queued();
     }
     for (int a : E.dirty) E.dirty_flag[a] = 0;
     E.dirty.clear();
@@ -291,7 +287,7 @@ int ensure_i(int ni)
 {
     if (ni <= E.icap) return 0;
     int cap = std::max((ni + 63) & ~63, 256);
-    if (wait_host_buffers()) return -1;
+    if (wait_queued(std::max(E.ibuf_seq, E.res_seq))) return -1;
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
     hfree(E.h_res); dfree(E.d_res);
     E.h_res_dev = nullptr;
@@ -388,6 +384,16 @@ struct Geometry {
 
 bool valid_wpg(int w) { return w == 1 || w == 2 || w == 4 || w == 8 || w == 16; }
 
+// waves of the small-block force shape: one 16-wave workgroup per CU; used
+// up to 4 i-groups (ni <= 256), beyond which it measured slower
+constexpr int kSmallWaves = 4096;
+constexpr int kSmallMaxGroups = 4;
+
+// j per FP32 tile of the scalar-load kernel: 16 and 32 shorten each wave's
+// chain of dependent scalar loads for tiny i-blocks, 256 amortises the FP64
+// folds of big ones
+bool valid_tile(int t) { return t == 16 || t == 32 || t == 64 || t == 128 || t == kTile; }
+
 // Force geometry: kernel variant, j-tile length (FP32 partial-sum span),
 // j-split and the reduction's lanes per i.  Precedence: the autotuner's
 // candidate while it measures, explicit options, the tuned table for this
@@ -403,7 +409,7 @@ Geometry choose_geometry(int ni, int nj)
         if (t.waves > 0) sh = t;
         if (E.opt.force_variant == 0 || E.opt.force_variant == 1) sh.variant = E.opt.force_variant;
         else if (t.waves == 0) sh.variant = ni >= E.opt.lds_min_ni ? 1 : 0;
-        if (E.opt.tile == 64 || E.opt.tile == 128 || E.opt.tile == kTile) sh.tile = E.opt.tile;
+        if (valid_tile(E.opt.tile)) sh.tile = E.opt.tile;
         if (E.opt.reduce_lanes > 0) sh.lanes = E.opt.reduce_lanes;
         if (valid_wpg(E.opt.wpg)) sh.wpg = E.opt.wpg;
         if (t.waves == 0 || E.opt.waves_target != Options().waves_target)
@@ -414,23 +420,44 @@ Geometry choose_geometry(int ni, int nj)
     const int per_wave = force_igroup(g.variant) / (g.variant == 1 ? 4 : 1);
     const int n_iw = std::max(1, (ni + per_wave - 1) / per_wave);
     const int target = std::max(1, sh.waves);
-    if (g.variant == 1) g.tile = kTile;
-    else if (sh.tile == 64 || sh.tile == 128 || sh.tile == kTile) g.tile = sh.tile;
-    else g.tile = (long long)n_iw * ((nj + kTile - 1) / kTile) >= target ? kTile : 64;
+    const bool nsplit_set = !E.forcing_shape && E.opt.nsplit > 0;
+    // Small i-blocks (at most 4 i-groups): one 16-wave workgroup per CU,
+    // kSmallWaves waves in all, each on a short chain of j (16..256 per tile).
+    // Measured through the ABI at ni <= 256: 2-14% less time per block than
+    // 64-j tiles with 8-wave workgroups.
+    bool small = false;
+    if (g.variant == 1) {
+        g.tile = kTile;
+    } else if (valid_tile(sh.tile)) {
+        g.tile = sh.tile;
+    } else if ((long long)n_iw * ((nj + kTile - 1) / kTile) >= target) {
+        g.tile = kTile;
+    } else if (!nsplit_set && n_iw <= kSmallMaxGroups) {
+        small = true;
+        const int want = std::max(1, kSmallWaves / n_iw);
+        g.tile = 16;
+        while (g.tile < kTile && (long long)g.tile * want < nj) g.tile *= 2;
+    } else {
+        g.tile = 64;
+    }
     const int ntiles = std::max(1, (nj + g.tile - 1) / g.tile);
-    int sp = (!E.forcing_shape && E.opt.nsplit > 0) ? E.opt.nsplit
-                                                    : std::max(1, (target + n_iw - 1) / n_iw);
+    int sp = nsplit_set ? E.opt.nsplit
+                        : std::max(1, ((small ? kSmallWaves : target) + n_iw - 1) / n_iw);
     sp = std::min(sp, ntiles);
     const int tiles_per = (ntiles + sp - 1) / sp;
     g.jchunk = tiles_per * g.tile;
     g.nsplit = std::max(1, (nj + g.jchunk - 1) / g.jchunk);
     // waves per workgroup: the splits of a workgroup fold their sums in LDS;
-    // built-in rule 8 for many splits (small i-blocks), else 4, at most nsplit
-    // (measured at N = 65536: 16 waves per workgroup is slower at every size)
+    // built-in rule 16 for the small-block shape, else 8 for many splits and
+    // 4 otherwise, at most nsplit (measured at N = 65536: 16 waves per
+    // workgroup is slower for large blocks)
     if (g.variant == 1) {
         g.wpg = 1;
     } else if (valid_wpg(sh.wpg)) {
         g.wpg = sh.wpg;
+    } else if (small) {
+        g.wpg = 1;
+        while (2 * g.wpg <= std::min(16, g.nsplit)) g.wpg *= 2;
     } else {
         const int cap = g.nsplit >= 64 ? 8 : 4;
         g.wpg = 1;
@@ -521,7 +548,7 @@ int do_close()
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
     hfree(E.h_res); dfree(E.d_res);
     E.h_res_dev = nullptr;
-    E.host_busy = false;
+    E.synced_seq = E.op_seq;
     E.icap = 0;
     dfree(E.d_part); E.pbytes = 0;
     dfree(E.d_nbcount); dfree(E.d_nblist); hfree(E.h_nbcount); hfree(E.h_nblist);
@@ -571,7 +598,7 @@ int set_j(int address, int index, double tj, double dtj, double mass,
 void reset_state()
 {
     if (!E.open) return;
-    (void)hipStreamSynchronize(E.stream);
+    if (hipStreamSynchronize(E.stream) == hipSuccess) E.synced_seq = E.op_seq;
     E.calc_pending = false;
     E.nb_valid = false;
     E.pred_valid = false;
@@ -603,7 +630,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     if (flush_j(nj)) return -1;
     if (predict(nj, false)) return -1;
     // the pinned i buffer may still be read by the previous upload
-    if (wait_host_buffers() || ensure_i(ni)) return -1;
+    if (wait_queued(E.ibuf_seq) || ensure_i(ni)) return -1;
     const int ld = (ni + 63) & ~63;
     float *f = reinterpret_cast<float *>(E.h_ibuf);
     int *ex = reinterpret_cast<int *>(f + 9 * (size_t)ld);
@@ -626,6 +653,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         ee[k] = (k < ni && e2) ? e2[k] : 0.0;
     }
     HIPCHK(hipMemcpyAsync(E.d_ibuf, E.h_ibuf, ibuf_bytes(ld), hipMemcpyHostToDevice, E.stream));
+    E.ibuf_seq = mark_queued();
     const IBlock ib = iblock_at(E.d_ibuf, ld);
     // the reduction writes the results straight into pinned host memory
     const Results r = results_at(E.h_res_dev, ni);
@@ -642,9 +670,9 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         gr.nn += o; gr.flags += o;
         if (enqueue_force(groups[g].n, nj, groups[g].eps2, gb, gr, groups[g].zskip)) return -1;
     }
-    mark_host_busy();
+    E.res_seq = mark_queued();
     HIPCHK(hipEventRecord(E.ev_done, E.stream));
-    E.done_seq = E.busy_seq;
+    E.done_seq = E.op_seq;
     E.last_ni = ni;
     E.last_nj = nj;
     E.last_include_self = include_self;
@@ -669,8 +697,7 @@ const Results *calc_wait(const char *who)
         fail(-1, "%s: %s", who, hipGetErrorString(e));
         return nullptr;
     }
-    // nothing touching the pinned buffers was queued after ev_done
-    if (E.done_seq == E.busy_seq) E.host_busy = false;
+    E.synced_seq = std::max(E.synced_seq, E.done_seq);
     harvest_timing();
     h = results_at(E.h_res, E.last_ni);
     E.last_refinements = count_refined(h, E.last_ni);
@@ -956,8 +983,9 @@ int autotune(int nj, int ni_max, double eps2)
         const int waves[] = {2048, 4096, 8192, 16384, 32768};
         for (int variant = 0; variant < 2; ++variant) {
             if (variant == 1 && ni < 512) continue;
-            for (int tile : {64, 128, kTile}) {
+            for (int tile : {16, 32, 64, 128, kTile}) {
                 if (variant == 1 && tile != kTile) continue;
+                if (tile < 64 && ni > 64) continue;   // short tiles only for one i-group
                 for (int w : waves) {
                     for (int wpg : {1, 4, 16}) {
                         if (variant == 1 && wpg != 1) continue;
@@ -1024,7 +1052,7 @@ int load_tuning(const char *path)
         const int got = sscanf(line, "%d %d %d %d %d %d", &b, &v, &t, &w, &l, &q);
         if (got != 5 && got != 6) continue;
         if (b < 0 || b >= kBuckets || (v != 0 && v != 1) || w <= 0 ||
-            (t != 64 && t != 128 && t != kTile) || (v == 1 && t != kTile) ||
+            !valid_tile(t) || (v == 1 && t != kTile) ||
             (l != 0 && l != 4 && l != 8 && l != 16 && l != 32 && l != 64) ||
             (q != 0 && !valid_wpg(q)))
             continue;

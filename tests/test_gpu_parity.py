@@ -311,6 +311,27 @@ def test_small_blocks_many_splits(g, nsplit):
         g.option("nsplit", 0)
 
 
+@pytest.mark.parametrize("tile", [16, 32])
+def test_short_tiles(g, tile):
+    """Short j-tiles (the small-block shape's 16 and 32 j per FP32 tile) on a
+    j count that is not a multiple of the tile, for blocks of one i-group and
+    of several (self-exclusion, exact nearest neighbours, softened and not)."""
+    s = plummer_state(9001, seed=43)
+    load_j(g, s)
+    g.option("force_variant", 0)
+    g.option("tile", tile)
+    try:
+        rng = np.random.default_rng(tile)
+        for ni, eps2 in ((1, 0.0), (40, 1e-8), (300, 0.0)):
+            isel = rng.choice(9001, ni, replace=False)
+            errs = _abi_force_vs_oracle(g, s, 0.0, eps2, isel)
+            assert g.option("last_tile") == tile
+            _check(errs, f"tile={tile} ni={ni}")
+    finally:
+        g.option("tile", 0)
+        g.option("force_variant", -1)
+
+
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
 def test_reduce_lanes_per_i(g, lanes):
     """Every lanes-per-i shape of the split reduction, on a split count (47)
@@ -391,7 +412,7 @@ def test_autotune_table_and_parity(g, tmp_path):
         tab = g.tuning()
         assert n == len(tab) == 11                      # buckets 1, 2, 4, ..., 1024 (ni <= 700)
         for b, (v, tile, waves, lanes) in tab.items():
-            assert v in (0, 1) and tile in (64, 128, 256) and waves > 0
+            assert v in (0, 1) and tile in (16, 32, 64, 128, 256) and waves > 0
             assert lanes in (0, 4, 8, 16, 32, 64)         # 0: the split-count rule won
         rng = np.random.default_rng(9)
         for ni in (1, 5, 64, 300, 700):
