@@ -77,16 +77,28 @@ class G6Backend:
         self.g.close()
 
 
+def dot(a, b):
+    """Row-wise dot product of (n, 3) arrays as Starlab's vec operator*
+    (include/starlab_vector.h): x*x + y*y + z*z, left to right."""
+    return a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1] + a[:, 2] * b[:, 2]
+
+
+def pow2_floor(d):
+    """2^floor(log2(d)) for d > 0, exactly (from the binary exponent)."""
+    _, e = np.frexp(d)
+    return np.ldexp(1.0, e - 1)
+
+
 def first_step(eta, a, j, pot, dt_max):
     """set_first_timestep (hdyn_ev.C:377-410), vectorized."""
     eta_init = eta / 16.0
-    a2 = (a * a).sum(1)
-    j2 = (j * j).sum(1)
+    a2 = dot(a, a)
+    j2 = dot(j, j)
     dt_adot = np.where(j2 > 0, a2 / np.where(j2 > 0, j2, 1.0), 1e300)
     dt_ff = np.where((pot < 0) & (a2 > 0), -pot / np.where(a2 > 0, a2, 1.0), 1e300)
     dt = eta_init * np.sqrt(np.minimum(dt_adot, dt_ff))
     dt = np.minimum(np.minimum(dt, eta_init), dt_max)
-    return 2.0 ** np.floor(np.log2(dt))
+    return pow2_floor(dt)
 
 
 class BlockHermite:
@@ -190,10 +202,10 @@ class BlockHermite:
         dt2 = d * d
         dt4 = dt2 * dt2
         dt6 = dt4 * dt2
-        l2 = 36 * (at3 * at3).sum(1) / dt6
-        k2 = 4 * (bt2 * bt2).sum(1) / dt4
-        jj = (j1 * j1).sum(1)
-        aa = (a1 * a1).sum(1)
+        l2 = 36 * dot(at3, at3) / dt6
+        k2 = 4 * dot(bt2, bt2) / dt4
+        jj = dot(j1, j1)
+        aa = dot(a1, a1)
         with np.errstate(invalid="ignore", divide="ignore"):
             nat = self.eta * np.sqrt((np.sqrt(aa * k2) + jj) / (np.sqrt(jj * l2) + k2))
         nat = np.where(nat > 0, nat, np.inf)
@@ -210,3 +222,107 @@ class BlockHermite:
         """Total energy with a caller-supplied exact pair-sum (synchronised state)."""
         ep, ek = energies_fn(self.m, self.x, self.v, self.eps2)
         return ep + ek
+
+
+# ---------------------------------------------------------------------------
+# native driver (libkira_blockstep.so, include/kira_blockstep.h)
+# ---------------------------------------------------------------------------
+import ctypes as _C
+import os as _os
+
+_DP = _C.POINTER(_C.c_double)
+_IP = _C.POINTER(_C.c_int)
+
+
+class _JMem(_C.Structure):
+    _fields_ = [("n", _C.c_int), ("t", _DP), ("m", _DP), ("x", _DP), ("v", _DP),
+                ("acc", _DP), ("jerk", _DP), ("k18", _DP)]
+
+
+_OBSERVER = _C.CFUNCTYPE(_C.c_int, _C.c_void_p, _C.POINTER(_JMem), _C.c_double, _C.c_int,
+                         _IP, _DP, _DP, _DP, _DP, _DP, _IP)
+LIBKB = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "libkira_blockstep.so")
+_kb = None
+
+
+def _load_kb():
+    global _kb
+    if _kb is None:
+        if not _os.path.exists(LIBKB):
+            from . import LibraryMissing
+            raise LibraryMissing(f"{LIBKB} is not built; run `python -m starlab_amd.build`")
+        L = _C.CDLL(LIBKB)
+        L.bs_run.restype = _C.c_int
+        L.bs_run.argtypes = [_C.c_char_p, _C.c_int, _DP, _DP, _DP, _C.c_double, _C.c_double,
+                             _C.c_double, _C.c_double, _C.c_int, _DP, _DP, _DP, _DP, _DP,
+                             _OBSERVER, _C.c_void_p]
+        L.bs_last_error.restype = _C.c_char_p
+        _kb = L
+    return _kb
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class NativeBlockHermite:
+    """BlockHermite in native code: kira's per-block ABI sequence driven by
+    libkira_blockstep.so against the GRAPE-6 library at `grape_lib` (default
+    libg6.so).  Same rules and operation order as BlockHermite, so the two
+    runs are bitwise equal on the same library.
+
+    observer(jmem, t, index, xi, vi, acc, jerk, pot, nn) is called after each
+    force call (numpy views valid only during the call; jmem is a dict of the
+    j-memory arrays t, m, x, v, acc, jerk, k18); a true return stops the run."""
+
+    def __init__(self, m, x, v, eta=0.1, eps2=0.0, dt_max=0.125):
+        self.n = len(m)
+        self.m = np.ascontiguousarray(m, dtype=np.float64).copy()
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3).copy()
+        self.v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3).copy()
+        self.a = np.zeros_like(self.x)
+        self.j = np.zeros_like(self.x)
+        self.pot = np.zeros(self.n)
+        self.dt = np.zeros(self.n)
+        self.t = np.zeros(self.n)
+        self.eta, self.eps2, self.dt_max = eta, eps2, dt_max
+        self.blocks = self.particle_steps = 0
+        self.abi_seconds = self.seconds = 0.0
+
+    def run(self, t_end, grape_lib=None, bulk=False, observer=None):
+        L = _load_kb()
+        if grape_lib is None:
+            from . import LIBG6
+            grape_lib = LIBG6
+        cb = _OBSERVER()
+        if observer is not None:
+            def _cb(_user, jm, t, ni, idx, xi, vi, acc, jerk, pot, nn):
+                J = jm.contents
+                n = J.n
+                view = lambda p, *shape: np.ctypeslib.as_array(p, shape)
+                jmem = dict(t=view(J.t, n), m=view(J.m, n), x=view(J.x, n, 3), v=view(J.v, n, 3),
+                            acc=view(J.acc, n, 3), jerk=view(J.jerk, n, 3),
+                            k18=view(J.k18, n, 3))
+                try:
+                    stop = observer(jmem, t, view(idx, ni), view(xi, ni, 3), view(vi, ni, 3),
+                                    view(acc, ni, 3), view(jerk, ni, 3), view(pot, ni),
+                                    view(nn, ni))
+                except Exception as e:          # surface it after the run
+                    self._observer_error = e
+                    return 1
+                return 1 if stop else 0
+            cb = _OBSERVER(_cb)
+        self._observer_error = None
+        stats = np.zeros(4)
+        rc = L.bs_run(grape_lib.encode(), self.n, _dp(self.m), _dp(self.x), _dp(self.v),
+                      float(t_end), float(self.eta), float(self.eps2), float(self.dt_max),
+                      int(bool(bulk)), _dp(self.a), _dp(self.j), _dp(self.pot), _dp(self.dt),
+                      _dp(stats), cb, None)
+        if self._observer_error is not None:
+            raise self._observer_error
+        if rc:
+            raise RuntimeError((L.bs_last_error() or b"").decode())
+        self.t[:] = t_end
+        self.blocks, self.particle_steps = int(stats[0]), int(stats[1])
+        self.abi_seconds, self.seconds = float(stats[2]), float(stats[3])
+        return self

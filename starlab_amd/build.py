@@ -7,6 +7,9 @@ libg6.so  exports the GRAPE-6 ABI (include/grape6.h), its extensions
 libg4x.so exports the GRAPE-4 ABI (include/grape4.h) and include/g4x.h.
 abi_bench is a plain C program (csrc/abi_bench.c) linked against libg6.so
           that times kira's per-block call sequence.
+libkira_blockstep.so (csrc/kira_blockstep.cc, include/kira_blockstep.h) is
+          kira's block-step integration as a native caller of any GRAPE-6 ABI
+          library, opened by path.
 Both link the same force kernels and engine core; each has its own engine
 state.  Symbols other than the declared C entry points are hidden.
 
@@ -30,6 +33,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libg6.so")
 LIBG4 = os.path.join(HERE, "libg4x.so")
 ABI_BENCH = os.path.join(HERE, "abi_bench")
+LIBKB = os.path.join(HERE, "libkira_blockstep.so")
 
 LIBS = {
     LIB: ["g6_kernels.hip", "g6_core.hip", "g6_engine.hip", "g6_ens.hip"],
@@ -87,6 +91,14 @@ def build(verbose: bool = False) -> str:
     _run([cc, "-O2", "-std=c99", inc, os.path.join(CSRC, "abi_bench.c"), f"-L{HERE}",
           "-Wl,-rpath,$ORIGIN", "-lg6", "-lm", "-o", tmp], verbose)
     os.replace(tmp, ABI_BENCH)
+    # kira's block-step caller in native code; it opens a GRAPE-6 library by path
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("no C++ compiler (g++) for libkira_blockstep.so")
+    tmp = LIBKB + ".tmp"
+    _run([cxx, "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", inc,
+          os.path.join(CSRC, "kira_blockstep.cc"), "-ldl", "-o", tmp], verbose)
+    os.replace(tmp, LIBKB)
     return LIB
 
 

@@ -39,6 +39,34 @@ class HostBackend:
         pass
 
 
+class OracleAbiBackend:
+    """The arithmetic of tests/oracle_g6abi.c in Python: j-memory holds what
+    g6_set_j_particle_ receives (a/2, j/6, k/18), every j is predicted with
+    the oracle's GRAPE-form predictor (4th order) at t_i, and the FP64 oracle
+    sums the forces.  The Python driver on this backend and the native driver
+    on that library must agree bit for bit."""
+
+    def open(self, n):
+        self.n = n
+        self.mir = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), a2=np.zeros((n, 3)),
+                        j6=np.zeros((n, 3)), k18=np.zeros((n, 3)), t=np.zeros(n), m=np.zeros(n))
+        self.idx = np.arange(n, dtype=np.int32)
+
+    def send_j(self, idx, tj, dtj, m, x, v, a, j, k18):
+        M = self.mir
+        M["t"][idx] = tj; M["m"][idx] = m; M["x"][idx] = x; M["v"][idx] = v
+        M["a2"][idx] = a / 2.0; M["j6"][idx] = j / 6.0; M["k18"][idx] = k18
+
+    def force(self, nj, t, idx, xi, vi, eps2):
+        M = self.mir
+        xp, vp = oracle.predict(M["x"], M["v"], M["a2"], M["j6"], M["k18"], M["t"], t, 4)
+        acc, jerk, pot, nn, _ = oracle.force(xi, vi, idx, xp, vp, M["m"], self.idx, eps2)
+        return acc, jerk, pot, nn
+
+    def close(self):
+        pass
+
+
 class CheckedG6(G6Backend):
     """every: check every `every`-th force call; max_i: at most this many i
     of a checked call (a seeded sample) go to the oracle."""
