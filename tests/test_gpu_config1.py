@@ -59,3 +59,29 @@ def test_config1_plummer_1024_to_t1():
     assert be.worst["jerk_norm"] <= 2e-6
     assert be.worst["pot_rel"] <= 1e-6
     assert de_gpu <= 10 * de_cpu + 1e-8
+
+
+def test_config1_native_driver_equals_python_driver():
+    """The native block-step driver (libkira_blockstep.so) on libg6 reproduces
+    the Python driver on libg6 bit for bit -- per-particle j updates as kira
+    makes them, and the bulk j update -- for config 1 to t = 1/4; the
+    observer's oracle checks hold on every block."""
+    from starlab_amd.blockstep import G6Backend, NativeBlockHermite
+    from starlab_amd.g6 import GRAPE6
+    from tests.kira_backends import NativeChecker
+    m, x, v = oracle.makeplummer(1024, 42)
+    py = BlockHermite(m, x, v, eta=0.1, eps2=0.0)
+    py.run(G6Backend(GRAPE6()), 0.25)
+    chk = NativeChecker(0.0)
+    nat = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(0.25, observer=chk)
+    bulk = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(0.25, bulk=True)
+    print(f"python {py.blocks} blocks; native {nat.blocks} blocks {nat.seconds:.2f}s "
+          f"(ABI {nat.abi_seconds:.2f}s); bulk {bulk.seconds:.2f}s", flush=True)
+    for other in (nat, bulk):
+        assert other.blocks == py.blocks and other.particle_steps == py.particle_steps
+        for name in ("x", "v", "a", "j", "pot", "dt"):
+            np.testing.assert_array_equal(getattr(other, name), getattr(py, name), err_msg=name)
+    assert chk.checked_blocks == nat.blocks and chk.nn_mismatch == 0
+    assert np.percentile(np.concatenate(chk.acc_errs), 99.9) <= 1e-6
+    assert chk.worst["acc_norm"] <= 2e-6 and chk.worst["jerk_norm"] <= 2e-6
+    assert chk.worst["pot_rel"] <= 1e-6

@@ -2,7 +2,8 @@
 (slope -2.35, mass ratio 100), 5 % primordial binaries (log-uniform a,
 thermal e) as their centres of mass -- what kira keeps in GRAPE j-memory --
 integrated with kira's individual power-of-two block steps from t = 0 to
-t = 0.25, forces from libg6 through the GRAPE-6 ABI (eps = 0, eta = 0.1).
+t = 0.25 by the native block-step driver (libkira_blockstep.so), forces from
+libg6 through the GRAPE-6 ABI (eps = 0, eta = 0.1).
 
 * a seeded sample of block steps (every 97th force call, at most 16 i each)
   is recomputed by the FP64 oracle on the same inputs: acc normwise error
@@ -19,8 +20,8 @@ import pytest
 
 import oracle
 from starlab_amd import ics
-from starlab_amd.blockstep import BlockHermite
-from tests.kira_backends import CheckedG6
+from starlab_amd.blockstep import NativeBlockHermite
+from tests.kira_backends import NativeChecker
 
 pytestmark = pytest.mark.gpu
 
@@ -48,29 +49,35 @@ def test_config3_king_salpeter_binaries_262144_to_quarter():
           f"m_max/m_min {m.max() / m.min():.1f}", flush=True)
     e0 = sum(oracle.energies(m, x, v, 0.0))
 
-    b = BlockHermite(m, x, v, eta=0.1, eps2=0.0)
-    be = CheckedG6(GRAPE6(), bulk=True, every=97, max_i=16, seed=3)
+    chk = NativeChecker(0.0, every=97, max_i=16, seed=3)
     sizes, times = [], []
+    obs_s = [0.0]
     last = [time.perf_counter()]
 
-    def on_block(s, t, blk, *r):
-        sizes.append(len(blk))
+    def observer(jm, t, idx, xi, vi, acc, jerk, pot, nn):
+        t0 = time.perf_counter()
+        sizes.append(len(idx))
         times.append(t)
+        chk(jm, t, idx, xi, vi, acc, jerk, pot, nn)
         now = time.perf_counter()
+        obs_s[0] += now - t0
         if now - last[0] > 10:
             last[0] = now
-            print(f"  t={t:.5f} blocks {s.blocks} steps {s.particle_steps}", flush=True)
+            print(f"  t={t:.5f} blocks {len(sizes)}", flush=True)
+        return False
 
-    t0 = time.perf_counter()
-    b.run(be, T_END, on_block=on_block)
-    wall = time.perf_counter() - t0
+    # kira's per-block sequence in native code (libkira_blockstep.so) on libg6
+    b = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(T_END, observer=observer)
+    wall = b.seconds - obs_s[0]
+    be = chk
     e1 = sum(oracle.energies(b.m, b.x, b.v, 0.0))
     de = abs((e1 - e0) / e0)
     errs = np.concatenate(be.acc_errs)
     sizes = np.array(sizes)
     hist = {int(1 << k): int(((sizes >= (1 << k)) & (sizes < (2 << k))).sum())
             for k in range(0, 19)}
-    print(f"blocks {b.blocks} particle steps {b.particle_steps} wall {wall:.1f}s "
+    print(f"blocks {b.blocks} particle steps {b.particle_steps} wall {wall:.1f}s without the "
+          f"checks (in ABI calls {b.abi_seconds:.1f}s, checks {obs_s[0]:.1f}s) "
           f"pair interactions/s {b.particle_steps * (N - 1) / wall:.3e} dE/E {de:.3e}", flush=True)
     print(f"block sizes (lower edge: count) {hist}", flush=True)
     print(f"checked {be.checked_blocks} calls / {be.checked_i} particle steps; "
