@@ -174,6 +174,47 @@ def abi_leg(nj=N, blocks=200, sizes=(1, 16, 256, 2048)):
             "per_block": [{k: json.loads(line)[k] for k in keys} for line in r.stdout.splitlines()]}
 
 
+def kira_leg(n=262144, t_end=0.25):
+    """kira end to end on libg6 (config 3's workload): King W0 = 6, N = 262144
+    top-level nodes, Salpeter 1-100, 5 % binaries as centres of mass, eta =
+    0.1, eps = 0, t = 0 -> 1/4, integrated by the native block-step driver
+    (libkira_blockstep.so) with one g6_set_j_particle_ per particle as kira
+    makes them.  Energies from g6x_potential (FP32 pairs, FP64 sums)."""
+    from starlab_amd import ics
+    from starlab_amd.blockstep import NativeBlockHermite
+    from starlab_amd.g6 import GRAPE6
+    g = GRAPE6()
+    if g.open() != 0:
+        raise RuntimeError(g.last_error())
+    idx = np.arange(n, dtype=np.int32)
+    z = np.zeros((n, 3))
+
+    def pot_energy(m, x):
+        g.set_j_particles(idx, idx, np.zeros(n), np.ones(n), m, z, z, z, z, x)
+        return 0.5 * float((m * g.potential(n, 0, n, 0.0, 0.0)).sum())
+
+    def energy(m, x, v):
+        return pot_energy(m, x) + 0.5 * float((m * (v * v).sum(1)).sum())
+
+    c = ics.king_cluster(n, 42, potential_fn=pot_energy)
+    m, x, v = c["m"], c["x"], c["v"]
+    e0 = energy(m, x, v)
+    g.close()
+    b = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(t_end)
+    g = GRAPE6()
+    if g.open() != 0:
+        raise RuntimeError(g.last_error())
+    e1 = energy(b.m, b.x, b.v)
+    g.close()
+    return {"workload": f"King W0=6 N={n}, Salpeter 1-100, 5% binaries (centres of mass), "
+                        f"eta=0.1, eps=0, t=0->{t_end}; native kira block-step driver, "
+                        "per-particle g6_set_j_particle_",
+            "blocks": b.blocks, "particle_steps": b.particle_steps,
+            "seconds": round(b.seconds, 3), "seconds_in_abi": round(b.abi_seconds, 3),
+            "pairs_per_s": b.particle_steps * (n - 1) / b.seconds,
+            "de_over_e": abs((e1 - e0) / e0)}
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "force_n65536_traffic.json")
     try:
@@ -191,6 +232,8 @@ def main():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ens", action="store_true", help="skip the ensemble / serving legs")
+    ap.add_argument("--no-kira", action="store_true",
+                    help="skip the kira end-to-end leg (native block-step driver)")
     ap.add_argument("--no-tune", action="store_true", help="skip the force-shape autotuner")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the energy/density fast-path and GRAPE-4 legs")
@@ -312,6 +355,9 @@ def main():
         out["aux"]["grape4_abi_sweep"] = grape4_leg(inp, xp, vp)
         log("bench: GRAPE-6 ABI block latency (C caller)")
         out["aux"]["grape6_abi_blocks"] = abi_leg()
+    if not a.no_kira:
+        log("bench: kira end to end (native block-step driver, N=262144 to t=1/4)")
+        out["kira_end_to_end"] = kira_leg()
     if not a.no_ens:
         log("bench: ensemble and serving legs")
         ens = ensemble_legs()

@@ -86,6 +86,9 @@ struct Engine {
     int n_addr = 0;             // 1 + highest address ever set
     std::vector<int> dirty;
     std::vector<uint8_t> dirty_flag;
+    // index field -> address of the j that holds it (self-exclusion): a flat
+    // array for indices in [0, kDenseIndex), a hash map for any other
+    std::vector<int> addr_by_index;
     std::unordered_map<int, int> addr_of_index;
 
     int dcap = 0;               // device capacity

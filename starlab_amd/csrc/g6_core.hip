@@ -565,6 +565,8 @@ int do_close()
     return 0;
 }
 
+void index_map_set(int index, int address);
+
 int set_j(int address, int index, double tj, double dtj, double mass,
           const double *k18, const double *j6, const double *a2,
           const double *v, const double *x)
@@ -581,11 +583,12 @@ int set_j(int address, int index, double tj, double dtj, double mass,
     r.pad[0] = r.pad[1] = 0;
     const int old = E.h_jindex[address];
     if (old != index) {
-        auto it = E.addr_of_index.find(old);
-        if (it != E.addr_of_index.end() && it->second == address) E.addr_of_index.erase(it);
+        if (excl_address(old) == address) index_map_set(old, -1);
         E.h_jindex[address] = index;
+        index_map_set(index, address);
+    } else if (excl_address(index) != address) {
+        index_map_set(index, address);   // the index was taken by another address since
     }
-    E.addr_of_index[index] = address;
     if (address + 1 > E.n_addr) E.n_addr = address + 1;
     if (!E.dirty_flag[address]) {
         E.dirty_flag[address] = 1;
@@ -604,10 +607,33 @@ void reset_state()
     E.pred_valid = false;
 }
 
+// Index fields of kira's nodes are small non-negative integers: they map
+// through a flat array; anything else through a hash map.
+constexpr int kDenseIndex = 1 << 24;
+
 int excl_address(int index)
 {
+    if (index >= 0 && index < kDenseIndex)
+        return (size_t)index < E.addr_by_index.size() ? E.addr_by_index[index] : -1;
     auto it = E.addr_of_index.find(index);
     return it == E.addr_of_index.end() ? -1 : it->second;
+}
+
+// address < 0 removes the entry
+void index_map_set(int index, int address)
+{
+    if (index >= 0 && index < kDenseIndex) {
+        if ((size_t)index >= E.addr_by_index.size()) {
+            if (address < 0) return;
+            E.addr_by_index.resize(std::max<size_t>((size_t)index + 1,
+                                                    2 * E.addr_by_index.size()), -1);
+        }
+        E.addr_by_index[index] = address;
+    } else if (address < 0) {
+        E.addr_of_index.erase(index);
+    } else {
+        E.addr_of_index[index] = address;
+    }
 }
 
 int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const double (*vi)[3],
