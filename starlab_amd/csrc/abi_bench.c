@@ -14,6 +14,11 @@
  *   3. g6calc_firsthalf_ / g6calc_lasthalf2_ on NI i-particles (NI <=
  *      g6_npipes_()), positions predicted on the host as kira does.
  *
+ * With ABI_BENCH_H2 set in the environment, every i gets that neighbour
+ * radius squared and each block also calls g6_read_neighbour_list_ and one
+ * g6_get_neighbour_list_ per i, as kira does for nodes with h2 > 0
+ * (hdyn_grape6.C:2062-2066, 2302-2306); their time is reported separately.
+ *
  * Prints one JSON line per NI: median and mean microseconds per block and per
  * phase, and pair interactions per second.  Exit status 0 on success.
  */
@@ -101,6 +106,13 @@ int main(int argc, char **argv)
     double *pot = malloc(sizeof(double) * npipes);
     double *h2 = calloc(npipes, sizeof(double));
     int *nn = malloc(sizeof(int) * npipes);
+    const char *h2env = getenv("ABI_BENCH_H2");
+    const double h2val = h2env ? atof(h2env) : 0.0;
+    for (int k = 0; k < npipes; ++k) h2[k] = h2val;
+    int maxlen = 4096;
+    int *nbl = malloc(sizeof(int) * maxlen);
+    double *tn = malloc(sizeof(double) * nblocks);
+    long long nb_total = 0;
     double *tb = malloc(sizeof(double) * nblocks), *ts = malloc(sizeof(double) * nblocks);
     double *tf = malloc(sizeof(double) * nblocks), *tl = malloc(sizeof(double) * nblocks);
     double ti = 0;
@@ -144,13 +156,23 @@ int main(int argc, char **argv)
             if (g6calc_lasthalf2_(&cl, &njj, &nii, index, xi, vi, &eps2, h2, acc, jerk, pot, nn))
                 status = 1;
             const double tb3 = now_us();
+            if (h2val > 0) {   /* kira's list retrieval for every i with h2 > 0 */
+                if (g6_read_neighbour_list_(&cl) != 0) status = 1;
+                for (int k = 0; k < ni; ++k) {
+                    int pipe = k, len = 0;
+                    if (g6_get_neighbour_list_(&cl, &pipe, &maxlen, &len, nbl) != 0) status = 1;
+                    if (b >= 0) nb_total += len;
+                }
+            }
+            const double tb4 = now_us();
             for (int k = 0; k < ni; ++k)
                 if (!(pot[k] < 0) || nn[k] < 0 || nn[k] >= nj) status = 1;
             if (b >= 0) {
-                tb[b] = tb3 - tb0;
+                tb[b] = tb4 - tb0;
                 ts[b] = tb1 - tb0;
                 tf[b] = tb2 - tb1;
                 tl[b] = tb3 - tb2;
+                tn[b] = tb4 - tb3;
             }
             prev = first;
         }
@@ -164,10 +186,15 @@ int main(int argc, char **argv)
         printf("{\"nj\": %d, \"ni\": %d, \"blocks\": %d, \"us_per_block\": %.2f, "
                "\"us_per_block_mean\": %.2f, \"us_set_j_and_predict\": %.2f, "
                "\"us_firsthalf\": %.2f, \"us_lasthalf\": %.2f, \"pairs_per_s\": %.4e, "
-               "\"load_us_per_j\": %.3f}\n",
+               "\"load_us_per_j\": %.3f",
                nj, ni, nblocks, median(tb, nblocks), mean, median(ts, nblocks),
                median(tf, nblocks), median(tl, nblocks), (double)ni * (nj - 1) / (mean * 1e-6),
                load_us / nj);
+        if (h2val > 0)
+            printf(", \"h2\": %g, \"us_lists\": %.2f, \"mean_list_length\": %.2f",
+                   h2val, median(tn, nblocks), (double)nb_total / ((double)nblocks * ni));
+        printf("}\n");
+        nb_total = 0;
         fflush(stdout);
     }
     g6_close_(&cl);

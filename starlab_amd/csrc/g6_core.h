@@ -111,7 +111,7 @@ struct Engine {
     int icap = 0;               // slots (multiple of 64)
     char *h_ibuf = nullptr;     // pinned, layout of ibuf_bytes()
     char *d_ibuf = nullptr;
-    int *d_slots = nullptr; int *h_slots = nullptr;
+    int *h_slots = nullptr;     // pinned: i-slots with neighbour lists
 
     // ---- partial sums -----------------------------------------------------
     size_t pbytes = 0;
@@ -135,8 +135,14 @@ struct Engine {
     // ---- neighbour lists --------------------------------------------------
     size_t nbcap_entries = 0;
     int nbcount_cap = 0;
-    int *d_nbcount = nullptr, *d_nblist = nullptr;
-    int *h_nbcount = nullptr, *h_nblist = nullptr;
+    int *d_nblist = nullptr;     // lists too long for the pinned buffer
+    int *d_nlscratch = nullptr;  // per (slot, split) counts, list bases, packed total
+    size_t nlscratch_cap = 0;
+    // pinned buffers the list kernels use directly (device addresses *_dev)
+    int *h_slots_dev = nullptr, *h_nbcount_dev = nullptr, *h_nblist_dev = nullptr;
+    int *h_nlinfo = nullptr, *h_nlinfo_dev = nullptr;   // list starts + packed total
+    int nlinfo_cap = 0;
+    int *h_nbcount = nullptr, *h_nblist = nullptr;   // pinned
     size_t h_nblist_cap = 0;
     std::vector<size_t> nb_offset;
     bool nb_valid = false;

@@ -13,6 +13,7 @@
 #include "g6_core.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -288,14 +289,14 @@ int ensure_i(int ni)
     if (ni <= E.icap) return 0;
     int cap = std::max((ni + 63) & ~63, 256);
     if (wait_queued(std::max(E.ibuf_seq, E.res_seq))) return -1;
-    hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
+    hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots);
     hfree(E.h_res); dfree(E.d_res);
     E.h_res_dev = nullptr;
     E.icap = 0;
     HIPCHK(hipHostMalloc((void **)&E.h_ibuf, ibuf_bytes(cap), hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E.d_ibuf, ibuf_bytes(cap)));
     HIPCHK(hipHostMalloc((void **)&E.h_slots, sizeof(int) * (size_t)cap, hipHostMallocDefault));
-    HIPCHK(hipMalloc((void **)&E.d_slots, sizeof(int) * (size_t)cap));
+    HIPCHK(hipHostGetDevicePointer((void **)&E.h_slots_dev, E.h_slots, 0));
     const size_t rb = results_bytes(cap);
     HIPCHK(hipHostMalloc((void **)&E.h_res, rb, hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E.d_res, rb));
@@ -545,13 +546,15 @@ int do_close()
     hfree(E.h_stage);
     E.h_stage_dev = nullptr;
     E.scap = 0;
-    hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots); dfree(E.d_slots);
+    hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots);
     hfree(E.h_res); dfree(E.d_res);
     E.h_res_dev = nullptr;
     E.synced_seq = E.op_seq;
     E.icap = 0;
     dfree(E.d_part); E.pbytes = 0;
-    dfree(E.d_nbcount); dfree(E.d_nblist); hfree(E.h_nbcount); hfree(E.h_nblist);
+    dfree(E.d_nblist); hfree(E.h_nbcount); hfree(E.h_nblist);
+    dfree(E.d_nlscratch); E.nlscratch_cap = 0;
+    hfree(E.h_nlinfo); E.nlinfo_cap = 0;
     E.nbcap_entries = 0; E.h_nblist_cap = 0; E.nbcount_cap = 0;
     for (hipEvent_t ev : E.ev_pool) (void)hipEventDestroy(ev);
     E.ev_pool.clear();
@@ -740,11 +743,11 @@ int ensure_nb(int ni)
         E.nbcap_entries = need;
     }
     if (ni > E.nbcount_cap) {
-        dfree(E.d_nbcount); hfree(E.h_nbcount);
+        hfree(E.h_nbcount);
         E.nbcount_cap = 0;
-        HIPCHK(hipMalloc((void **)&E.d_nbcount, sizeof(int) * (size_t)ni));
         HIPCHK(hipHostMalloc((void **)&E.h_nbcount, sizeof(int) * (size_t)ni,
                              hipHostMallocDefault));
+        HIPCHK(hipHostGetDevicePointer((void **)&E.h_nbcount_dev, E.h_nbcount, 0));
         E.nbcount_cap = ni;
     }
     return 0;
@@ -770,47 +773,60 @@ int read_nl()
     }
     const int ld = (ni + 63) & ~63;
     IBlock ib = iblock_at(E.d_ibuf, ld);
-    HIPCHK(hipMemsetAsync(E.d_nbcount, 0, sizeof(int) * (size_t)ni, E.stream));
-    HIPCHK(hipMemcpyAsync(E.d_slots, E.h_slots, sizeof(int) * (size_t)nslots,
-                          hipMemcpyHostToDevice, E.stream));
+    const size_t need = nl_scratch_ints(nslots, E.last_nj);
+    if (need > E.nlscratch_cap) {
+        dfree(E.d_nlscratch);
+        E.nlscratch_cap = 0;
+        HIPCHK(hipMalloc((void **)&E.d_nlscratch, sizeof(int) * need));
+        E.nlscratch_cap = need;
+    }
+    if (nslots + 1 > E.nlinfo_cap) {
+        hfree(E.h_nlinfo);
+        E.nlinfo_cap = 0;
+        const int c = std::max(nslots + 1, 1024);
+        HIPCHK(hipHostMalloc((void **)&E.h_nlinfo, sizeof(int) * (size_t)c, hipHostMallocDefault));
+        HIPCHK(hipHostGetDevicePointer((void **)&E.h_nlinfo_dev, E.h_nlinfo, 0));
+        E.nlinfo_cap = c;
+    }
+    if (E.h_nblist_cap == 0) {   // typical lists land here directly
+        const size_t c = (size_t)1 << 20;
+        HIPCHK(hipHostMalloc((void **)&E.h_nblist, sizeof(int) * c, hipHostMallocDefault));
+        HIPCHK(hipHostGetDevicePointer((void **)&E.h_nblist_dev, E.h_nblist, 0));
+        E.h_nblist_cap = c;
+    }
+    // counts of unlisted slots stay 0; the kernels read slots and write
+    // counts, list starts and (when they fit) the lists in pinned memory
+    memset(E.h_nbcount, 0, sizeof(int) * (size_t)ni);
     {
         KTimer t(K_NBLIST);
         HIPCHK(launch_nblist(E.stream, E.d_xp64, E.dcap, E.d_jindex, E.last_nj, ib,
-                             h2_at(E.d_ibuf, ld), e2_at(E.d_ibuf, ld), E.d_slots, nslots,
-                             E.last_include_self,
-                             E.opt.nb_cap, E.d_nbcount, E.d_nblist));
+                             h2_at(E.d_ibuf, ld), e2_at(E.d_ibuf, ld), E.h_slots_dev, nslots,
+                             E.last_include_self, E.opt.nb_cap, E.h_nbcount_dev, E.d_nlscratch,
+                             E.d_nblist, E.h_nlinfo_dev, E.h_nblist_dev,
+                             (int)std::min<size_t>(E.h_nblist_cap, INT_MAX)));
         t.stop();
     }
-    HIPCHK(hipMemcpyAsync(E.h_nbcount, E.d_nbcount, sizeof(int) * (size_t)ni,
-                          hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
+    E.synced_seq = E.op_seq;
     harvest_timing();
-    // fetch the stored part of every non-empty list into one host array
-    size_t total = 0;
+    const size_t total = (size_t)E.h_nlinfo[nslots];
     int overflow = 0;
     for (int s = 0; s < nslots; ++s) {
         const int k = E.h_slots[s];
-        const int c = E.h_nbcount[k];
-        if (c > E.opt.nb_cap) overflow = 1;
-        E.nb_offset[k] = total;
-        total += (size_t)std::min(c, E.opt.nb_cap);
+        if (E.h_nbcount[k] > E.opt.nb_cap) overflow = 1;
+        E.nb_offset[k] = (size_t)E.h_nlinfo[s];
     }
     if (total > E.h_nblist_cap) {
+        // too long for the pinned buffer: the kernel wrote them on the device
         hfree(E.h_nblist);
         E.h_nblist_cap = 0;
-        HIPCHK(hipHostMalloc((void **)&E.h_nblist, sizeof(int) * std::max<size_t>(total, 1024),
-                             hipHostMallocDefault));
-        E.h_nblist_cap = std::max<size_t>(total, 1024);
+        HIPCHK(hipHostMalloc((void **)&E.h_nblist, sizeof(int) * total, hipHostMallocDefault));
+        HIPCHK(hipHostGetDevicePointer((void **)&E.h_nblist_dev, E.h_nblist, 0));
+        E.h_nblist_cap = total;
+        HIPCHK(hipMemcpyAsync(E.h_nblist, E.d_nblist, sizeof(int) * total,
+                              hipMemcpyDeviceToHost, E.stream));
+        HIPCHK(hipStreamSynchronize(E.stream));
     }
-    for (int s = 0; s < nslots; ++s) {
-        const int k = E.h_slots[s];
-        const int c = std::min(E.h_nbcount[k], E.opt.nb_cap);
-        if (c > 0)
-            HIPCHK(hipMemcpyAsync(E.h_nblist + E.nb_offset[k],
-                                  E.d_nblist + (size_t)k * E.opt.nb_cap, sizeof(int) * (size_t)c,
-                                  hipMemcpyDeviceToHost, E.stream));
-    }
-    HIPCHK(hipStreamSynchronize(E.stream));
     return overflow;
 }
 

@@ -111,9 +111,17 @@ hipError_t launch_knn(hipStream_t s, const double *xp64, int cap, int nj, int i_
                       const JRaw *jraw, double *density, double *dk2);
 // lanes per i the reduction uses by default for nsplit splits (4..64)
 int reduce_lanes(int nsplit);
-hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,
-                         const int *jindex, int nj, const IBlock &ib,
-                         const double *h2, const double *e2, const int *slots, int nslots,
-                         int include_self, int nb_cap, int *counts, int *lists);
+// Neighbour lists of the nslots i-slots listed in slots[]: counts[slot] =
+// true count; the lists packed back to back in ascending slot order (at most
+// nb_cap entries each), into host_packed when the packed total is at most
+// host_cap entries, else into packed (device); host_info[0, nslots) = each
+// list's start, host_info[nslots] = the packed total.  scratch holds
+// nl_scratch_ints() ints.  slots, counts, host_info and host_packed may be
+// pinned host memory (device addresses).
+size_t nl_scratch_ints(int nslots, int nj);
+hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap, const int *jindex, int nj,
+                         const IBlock &ib, const double *h2, const double *e2, const int *slots,
+                         int nslots, int include_self, int nb_cap, int *counts, int *scratch,
+                         int *packed, int *host_info, int *host_packed, int host_cap);
 
 }  // namespace g6

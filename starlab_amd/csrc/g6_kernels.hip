@@ -1080,60 +1080,206 @@ hipError_t launch_knn(hipStream_t s, const double *xp64, int cap, int nj, int i_
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour lists (FP64, exact): one wave per listed i-slot; lanes sweep 64
-// consecutive j, ballot + prefix count keeps the list in ascending address.
-// counts[i] is the true count; at most nb_cap entries are stored.
-// The test is ((e2[i] + dx*dx) + dy*dy) + dz*dz < h2[i]: with e2 = 0 that is
-// the unsoftened d^2 < h2 of the GRAPE-6 front end (0 + a = a exactly); the
-// GRAPE-4 front end passes e2 = 2 eps2 and h2 = 2 h2, the operation order of
-// the reference's IEEE GRAPE-4 simulator (harp3fastsim.c:289-303).
+// Neighbour lists (FP64, exact).  The test is
+//   ((e2[i] + dx*dx) + dy*dy) + dz*dz < h2[i]
+// with e2 = 0 for the GRAPE-6 front end (0 + a = a exactly: the unsoftened
+// d^2 < h2) and e2 = 2 eps2, h2 = 2 h2 for the GRAPE-4 front end, the
+// operation order of the reference's IEEE GRAPE-4 simulator
+// (harp3fastsim.c:289-303).  Lists are in ascending address; counts are the
+// true counts, at most nb_cap entries per list are stored.
+//
+// Three passes over (64 listed i-slots x j-split) waves.  Lanes are slots and
+// j is wave-uniform (scalar loads of the FP64 predicted positions), so each j
+// is read once per 64 slots, and the j range is split over enough waves to
+// fill the chip even for a single slot:
+//   k_nl<false>        hits per (slot, split)                     -> cnt
+//   k_nl_scan_splits   one wave per slot: exclusive offsets over its splits,
+//                      the slot's true count and its clipped length
+//   k_nl_scan_slots    one workgroup: packed bases of the clipped lists
+//   k_nl<true>         the same sweep, writing each hit at its place in one
+//                      packed array (slot s's list at base[s], nb_cap at most)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_nblist(
-    const double *__restrict__ xp64, int cap, const int *__restrict__ jindex,
-    int nj, IBlock ib, const double *__restrict__ h2, const double *__restrict__ e2,
-    const int *__restrict__ slots, int nslots, int include_self, int nb_cap,
-    int *__restrict__ counts, int *__restrict__ lists)
+template <bool WRITE>
+__global__ __launch_bounds__(64) void k_nl(
+    const double *__restrict__ xp64, int cap, const int *__restrict__ jindex, int nj,
+    IBlock ib, const double *__restrict__ h2, const double *__restrict__ e2,
+    const int *__restrict__ slots, int nslots, int include_self, int jchunk, int nsplit,
+    int *__restrict__ cnt, const int *__restrict__ base, int nb_cap, int *__restrict__ packed,
+    const int *__restrict__ total, int host_cap, int *__restrict__ host_packed)
 {
 #pragma clang fp contract(off)
-    const int w = blockIdx.x;
-    if (w >= nslots) return;
-    const int i = slots[w];
-    const int lane = threadIdx.x;
-    const double hh = h2[i];
-    const double ee = e2[i];
+    const int n_sgroups = (nslots + 63) / 64;
+    const int b = blockIdx.x;
+    const int split = b / n_sgroups;
+    const int sg = b - split * n_sgroups;
+    const int s = sg * 64 + (int)threadIdx.x;
+    const int sc = s < nslots ? s : nslots - 1;
+    const int i = slots[sc];
+    const double hh = h2[i], ee = e2[i];
     const double xi = ib.x64[i], yi = ib.y64[i], zi = ib.z64[i];
-    const int excl = ib.excl[i];
-    int cnt = 0;
-    int *out = lists + (size_t)i * nb_cap;
-    for (int j0 = 0; j0 < nj; j0 += 64) {
-        const int j = j0 + lane;
-        bool in = false;
-        if (j < nj) {
-            const double dx = xp64[j] - xi;
-            const double dy = xp64[(size_t)cap + j] - yi;
-            const double dz = xp64[2 * (size_t)cap + j] - zi;
-            const double d2 = ((ee + dx * dx) + dy * dy) + dz * dz;
-            in = d2 < hh && (include_self || j != excl);
-        }
-        const unsigned long long m = __ballot(in);
-        if (in) {
-            const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (pos < nb_cap) out[pos] = jindex[j];
-        }
-        cnt += __popcll(m);
+    const int excl = include_self ? -1 : ib.excl[i];
+    const int jbeg = split * jchunk;
+    const int jend = min(jbeg + jchunk, nj);
+    const size_t c_at = (size_t)sc * nsplit + split;
+    int pos = 0, lim = 0, out0 = 0;
+    int *dst = packed;
+    if (WRITE) {
+        pos = cnt[c_at];   // offset of this split inside the slot's list
+        lim = nb_cap;
+        out0 = base[sc];
+        // lists that fit the caller's pinned buffer go straight to the host
+        if (*total <= host_cap) dst = host_packed;
     }
-    if (lane == 0) counts[i] = cnt;
+    const double *__restrict__ yp = xp64 + cap;
+    const double *__restrict__ zp = xp64 + 2 * (size_t)cap;
+    int c = 0;
+    auto test = [&](int j, double xj, double yj, double zj) {
+        const double dx = xj - xi;
+        const double dy = yj - yi;
+        const double dz = zj - zi;
+        const double d2 = ((ee + dx * dx) + dy * dy) + dz * dz;
+        const bool hit = d2 < hh && j != excl;
+        if (WRITE) {
+            if (hit && s < nslots) {
+                if (pos < lim) dst[out0 + pos] = jindex[j];
+                ++pos;
+            }
+        } else {
+            c += hit ? 1 : 0;
+        }
+    };
+    int j = jbeg;
+    // 8 j per step: their coordinates arrive as wide scalar loads issued
+    // together, so the load latency is paid once per 8 j, not per j
+    for (; j + 8 <= jend; j += 8) {
+        double X[8], Y[8], Z[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            X[u] = xp64[j + u];
+            Y[u] = yp[j + u];
+            Z[u] = zp[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) test(j + u, X[u], Y[u], Z[u]);
+    }
+    for (; j < jend; ++j) test(j, xp64[j], yp[j], zp[j]);
+    if (!WRITE && s < nslots) cnt[c_at] = c;
 }
 
-hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap,
-                         const int *jindex, int nj, const IBlock &ib,
-                         const double *h2, const double *e2, const int *slots, int nslots,
-                         int include_self, int nb_cap, int *counts, int *lists)
+// In-place exclusive scan of a[0, n) by one workgroup of kScanThreads;
+// returns the total (to every thread).
+constexpr int kScanThreads = 1024;
+
+__device__ int block_exclusive_scan(int *__restrict__ a, int n, int *part)
+{
+    const int t = threadIdx.x;
+    const int per = (n + kScanThreads - 1) / kScanThreads;
+    const int b0 = min(t * per, n), b1 = min(b0 + per, n);
+    int local = 0;
+    for (int k = b0; k < b1; ++k) local += a[k];
+    part[t] = local;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int k = 0; k < kScanThreads; ++k) {
+            const int v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        part[kScanThreads] = run;
+    }
+    __syncthreads();
+    int run = part[t];
+    for (int k = b0; k < b1; ++k) {
+        const int v = a[k];
+        a[k] = run;
+        run += v;
+    }
+    const int total = part[kScanThreads];
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(64) void k_nl_scan_splits(
+    int *__restrict__ cnt, int nsplit, const int *__restrict__ slots, int nb_cap,
+    int *__restrict__ counts, int *__restrict__ clip)
+{
+    // cnt is slot-major: [slot][split]
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    int *row = cnt + (size_t)s * nsplit;
+    int carry = 0;
+    for (int p0 = 0; p0 < nsplit; p0 += 64) {
+        const int p = p0 + lane;
+        const int v = p < nsplit ? row[p] : 0;
+        int x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (p < nsplit) row[p] = carry + x - v;
+        carry += __shfl(x, 63);
+    }
+    if (lane == 0) {
+        counts[slots[s]] = carry;
+        clip[s] = min(carry, nb_cap);
+    }
+}
+
+// clip -> base (in place), total; both also to the host (host_info[nslots + 1])
+__global__ __launch_bounds__(kScanThreads) void k_nl_scan_slots(int *__restrict__ clip,
+                                                                int nslots,
+                                                                int *__restrict__ total_out,
+                                                                int *__restrict__ host_info)
+{
+    __shared__ int part[kScanThreads + 1];
+    const int stored = block_exclusive_scan(clip, nslots, part);
+    for (int s = threadIdx.x; s < nslots; s += kScanThreads) host_info[s] = clip[s];
+    if (threadIdx.x == 0) {
+        *total_out = stored;
+        host_info[nslots] = stored;
+    }
+}
+
+// j-split of the list passes: about 4096 waves in all, at least 16 j per wave
+void nl_geometry(int nslots, int nj, int &nsplit, int &jchunk)
+{
+    const int n_sgroups = (nslots + 63) / 64;
+    nsplit = std::max(1, std::min(4096 / n_sgroups, (nj + 15) / 16));
+    jchunk = (nj + nsplit - 1) / nsplit;
+    nsplit = (nj + jchunk - 1) / jchunk;
+}
+
+size_t nl_scratch_ints(int nslots, int nj)
+{
+    int nsplit, jchunk;
+    nl_geometry(nslots, nj, nsplit, jchunk);
+    return (size_t)nslots * nsplit + nslots + 1;
+}
+
+hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap, const int *jindex, int nj,
+                         const IBlock &ib, const double *h2, const double *e2, const int *slots,
+                         int nslots, int include_self, int nb_cap, int *counts, int *scratch,
+                         int *packed, int *host_info, int *host_packed, int host_cap)
 {
     if (nslots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nblist, dim3(nslots), dim3(64), 0, s, xp64, cap, jindex,
-                       nj, ib, h2, e2, slots, nslots, include_self, nb_cap, counts,
-                       lists);
+    int nsplit, jchunk;
+    nl_geometry(nslots, nj, nsplit, jchunk);
+    int *cnt = scratch;
+    int *base = cnt + (size_t)nslots * nsplit;
+    int *total = base + nslots;
+    const dim3 grid(((nslots + 63) / 64) * nsplit);
+    hipLaunchKernelGGL(k_nl<false>, grid, dim3(64), 0, s, xp64, cap, jindex, nj, ib, h2, e2,
+                       slots, nslots, include_self, jchunk, nsplit, cnt, base, nb_cap, packed,
+                       total, host_cap, host_packed);
+    hipLaunchKernelGGL(k_nl_scan_splits, dim3(nslots), dim3(64), 0, s, cnt, nsplit, slots,
+                       nb_cap, counts, base);
+    hipLaunchKernelGGL(k_nl_scan_slots, dim3(1), dim3(kScanThreads), 0, s, base, nslots, total,
+                       host_info);
+    hipLaunchKernelGGL(k_nl<true>, grid, dim3(64), 0, s, xp64, cap, jindex, nj, ib, h2, e2,
+                       slots, nslots, include_self, jchunk, nsplit, cnt, base, nb_cap, packed,
+                       total, host_cap, host_packed);
     return hipGetLastError();
 }
 
