@@ -160,17 +160,24 @@ def grape4_leg(inp, xp, vp, npipe=2048):
             "calls": len(blocks), "pipes_per_call": npipe, "distinct_i_per_call": slots}
 
 
-def abi_leg(nj=N, blocks=200, sizes=(1, 16, 256, 2048)):
+def abi_leg(nj=N, blocks=200, sizes=(1, 16, 256, 2048), h2=0.0):
     """kira's per-block sequence through the GRAPE-6 ABI (set_j of the previous
     block, set_ti, firsthalf, lasthalf2) from the compiled C caller
-    starlab_amd/abi_bench: microseconds per block, host-to-host."""
+    starlab_amd/abi_bench: microseconds per block, host-to-host.  With h2 > 0
+    every i asks for its neighbour list (radius^2 h2) and each block also
+    runs g6_read_neighbour_list_ and g6_get_neighbour_list_ per i."""
     exe = os.path.join(ROOT, "starlab_amd", "abi_bench")
+    env = dict(os.environ)
+    if h2 > 0:
+        env["ABI_BENCH_H2"] = repr(h2)
     r = subprocess.run([exe, str(nj), str(blocks), *map(str, sizes)], capture_output=True,
-                       text=True, timeout=300)
+                       text=True, timeout=300, env=env)
     if r.returncode:
         raise RuntimeError(f"abi_bench failed ({r.returncode}): {r.stderr.strip()}")
-    keys = ("ni", "us_per_block", "us_firsthalf", "us_lasthalf", "pairs_per_s")
-    return {"nj": nj, "blocks": blocks,
+    keys = ["ni", "us_per_block", "us_firsthalf", "us_lasthalf", "pairs_per_s"]
+    if h2 > 0:
+        keys += ["us_lists", "mean_list_length"]
+    return {"nj": nj, "blocks": blocks, "h2": h2,
             "per_block": [{k: json.loads(line)[k] for k in keys} for line in r.stdout.splitlines()]}
 
 
@@ -355,6 +362,9 @@ def main():
         out["aux"]["grape4_abi_sweep"] = grape4_leg(inp, xp, vp)
         log("bench: GRAPE-6 ABI block latency (C caller)")
         out["aux"]["grape6_abi_blocks"] = abi_leg()
+        # about 30 neighbours per i in the uniform sphere of abi_bench
+        out["aux"]["grape6_abi_blocks_with_lists"] = abi_leg(blocks=100, sizes=(1, 256, 2048),
+                                                             h2=0.006)
     if not a.no_kira:
         log("bench: kira end to end (native block-step driver, N=262144 to t=1/4)")
         out["kira_end_to_end"] = kira_leg()
