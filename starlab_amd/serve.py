@@ -142,3 +142,41 @@ def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, *
                 p50_3body_s=_pct(lat["3body"], 50), p99_3body_s=_pct(lat["3body"], 99),
                 p50_cluster_s=_pct(lat["cluster"], 50), p99_cluster_s=_pct(lat["cluster"], 99),
                 failed=sum(r["failed"] for r in results.values()))
+
+
+def main(argv=None):
+    """Command line: run the batched-cluster workload or the served Poisson
+    stream on the ensemble engine and print one JSON line.
+
+        python -m starlab_amd.serve batch [--count 4096 --nmin 32 --nmax 2048
+                                           --t-end 0.03125 --prec 32 --tune]
+        python -m starlab_amd.serve poisson [--rate 2000 --seconds 2]
+    """
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m starlab_amd.serve")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    b = sub.add_parser("batch", help="Plummer clusters, N ~ U[nmin, nmax], all at t = 0")
+    b.add_argument("--count", type=int, default=4096)
+    b.add_argument("--nmin", type=int, default=32)
+    b.add_argument("--nmax", type=int, default=2048)
+    b.add_argument("--t-end", type=float, default=1.0 / 32)
+    b.add_argument("--prec", type=int, choices=(32, 64), default=32)
+    b.add_argument("--seed", type=int, default=4)
+    b.add_argument("--tune", action="store_true", help="engine shapes from ens.autotune")
+    p = sub.add_parser("poisson", help="70 %% three-body scatterings, 30 %% clusters")
+    p.add_argument("--rate", type=float, default=2000.0)
+    p.add_argument("--seconds", type=float, default=2.0)
+    p.add_argument("--seed", type=int, default=5)
+    a = ap.parse_args(argv)
+    if a.cmd == "batch":
+        jobs = cluster_batch(a.count, a.nmin, a.nmax, a.seed, a.t_end, prec=a.prec)
+        r = run_batch(jobs, tune=a.tune)
+    else:
+        r = run_poisson(serving_mix(int(a.rate * a.seconds), a.seed), a.rate)
+    print(json.dumps(r), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
