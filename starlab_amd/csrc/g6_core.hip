@@ -942,6 +942,10 @@ int potential_sweep(int nj, int i_first, int ni, double ti, double eps2, double 
     return 0;
 }
 
+// waves of the k-nearest-neighbour sweep: 8 per CU (measured at N = 65536:
+// 4.3 ms with 2 splits of 32768 j against 11.7 ms with 32 splits)
+constexpr int kKnnWaves = 2048;
+
 int density_sweep(int nj, int i_first, int ni, double ti, int k, double *density, double *dk2)
 {
     if (k < 2 || k > 32) return fail(-1, "g6x_densities: k = %d outside [2, 32]", k);
@@ -949,8 +953,11 @@ int density_sweep(int nj, int i_first, int ni, double ti, int k, double *density
     if (sweep_begin("g6x_densities", nj, i_first, ni, ti)) return -1;
     const int ld = (ni + 63) & ~63;
     const int n_iw = (ni + 63) / 64;
-    // j-splits of at least 256 j, at most 64 of them
-    int nsplit = std::max(1, (std::max(1, E.opt.waves_target) + n_iw - 1) / n_iw);
+    // j-splits: each split refills its k-list from empty, and while the list
+    // fills, some lane of the wave inserts at nearly every j; long splits pay
+    // that once.  About kKnnWaves waves in all, splits of at least 256 j, at
+    // most 64 of them ("nsplit" overrides).
+    int nsplit = E.opt.nsplit > 0 ? E.opt.nsplit : std::max(1, (kKnnWaves + n_iw - 1) / n_iw);
     nsplit = std::min({nsplit, std::max(1, nj / 256), 64});
     const int jchunk = (nj + nsplit - 1) / nsplit;
     nsplit = (nj + jchunk - 1) / jchunk;

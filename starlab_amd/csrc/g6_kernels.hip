@@ -1003,13 +1003,27 @@ __global__ __launch_bounds__(64) void k_knn(
     klist_init(L);
     const int jbeg = split * jchunk;
     const int jend = min(jbeg + jchunk, nj);
-    for (int j = jbeg; j < jend; ++j) {
-        const double dx = X[j] - xi;
-        const double dy = Y[j] - yi;
-        const double dz = Z[j] - zi;
+    auto test = [&](int j, double xj, double yj, double zj) {
+        const double dx = xj - xi;
+        const double dy = yj - yi;
+        const double dz = zj - zi;
         const double d2 = (dx * dx + dy * dy) + dz * dz;
         if (d2 < L.kth && j != self) klist_insert(L, k, d2, j);
+    };
+    int j = jbeg;
+    // 8 j per step through wide scalar loads (one load latency per 8 j)
+    for (; j + 8 <= jend; j += 8) {
+        double XX[8], YY[8], ZZ[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            XX[u] = X[j + u];
+            YY[u] = Y[j + u];
+            ZZ[u] = Z[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) test(j + u, XX[u], YY[u], ZZ[u]);
     }
+    for (; j < jend; ++j) test(j, X[j], Y[j], Z[j]);
     if (i < ni) {
 #pragma unroll
         for (int q = 0; q < KMAX; ++q) {
