@@ -72,7 +72,8 @@ int g6x_sync(int cluster_id);
  *   "force_variant"  -1 auto, 0 scalar-load, 1 LDS-staged     G6_FORCE_VARIANT
  *   "lds_min_ni"     auto picks the LDS kernel from this ni  G6_LDS_MIN_NI
  *   "xcd_remap"      1 = keep each j-split on one XCD         G6_XCD_REMAP
- *   "tile"           j per FP32 tile: 0 auto, 64, 128, 256    G6_TILE
+ *   "tile"           j per FP32 tile: 0 auto, 16, 32, 64, 128, 256   G6_TILE
+ *                    (16 and 32 suit blocks of one or a few i-groups)
  *   "last_tile"      (read only) tile of the last force launch
  *   "reduce_lanes"   lanes per i in the split reduction: 0 auto, 4..64  G6_REDUCE_LANES
  *   "wpg"            waves per force workgroup, each on its own j-split, folded
@@ -86,7 +87,8 @@ long long g6x_get_option(const char *key);
  * "timing" = 1).  kernel: 0 predictor, 1 force, 2 reduce (includes the
  * FP64 nearest-neighbour rescan of ambiguous i), 3 unused (0 launches),
  * 4 neighbour list, 5 j scatter, 6 potential (g6x_potential), 7 k-nearest
- * neighbours (g6x_densities).  Returns launches; *ms = total ms. */
+ * neighbours (g6x_densities).  Returns launches; *ms = total ms.  Kernel 4
+ * covers the four launches of one neighbour-list build. */
 int  g6x_kernel_time(int kernel, double *ms);
 void g6x_reset_timing(void);
 
@@ -99,9 +101,9 @@ int  g6x_last_refinements(void);
 
 /* Autotuner.  For each i-block size bucket (bucket b: 2^(b-1) < ni <= 2^b)
  * up to ni_max, times force + reduction for every candidate shape (kernel
- * variant, j-tile length 64/128/256, waves per launch 2048..32768, then the
- * reduction's lanes per i) against the first nj j-particles as loaded, and
- * keeps the fastest.  Later force calls of that size use the tuned shape
+ * variant, j-tile length 64/128/256 and, for ni <= 64, 16/32, waves per
+ * launch 2048..32768, waves per workgroup, then the reduction's lanes per i)
+ * against the first nj j-particles as loaded, and keeps the fastest.  Later force calls of that size use the tuned shape
  * unless an explicit option overrides it.  Takes ~1 s at nj = 65536.
  * Returns the number of buckets tuned, or -1. */
 int  g6x_autotune(int cluster_id, int nj, int ni_max, double eps2);
