@@ -67,7 +67,8 @@ int g6x_sync(int cluster_id);
  *   "npipes"         value returned by g6_npipes_            G6_NPIPES
  *   "nb_cap"         neighbour entries stored per i          G6_NB_CAP
  *   "waves_target"   waves the force grid aims for           G6_WAVES
- *   "nsplit"         force j-split (0 = automatic)           G6_NSPLIT
+ *   "nsplit"         force j-split (0 = automatic); also the j-split of the
+ *                    k-nearest-neighbour sweep of g6x_densities   G6_NSPLIT
  *   "timing"         1 = time each kernel with HIP events    G6_TIMING
  *   "force_variant"  -1 auto, 0 scalar-load, 1 LDS-staged     G6_FORCE_VARIANT
  *   "lds_min_ni"     auto picks the LDS kernel from this ni  G6_LDS_MIN_NI
