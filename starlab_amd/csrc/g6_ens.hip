@@ -36,6 +36,7 @@
 #include "../../include/g6ens.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -829,6 +830,14 @@ __global__ void k_ens_pack(Pool P, const int *__restrict__ src, double *__restri
     o[7] = P.pot0[p];
 }
 
+// headers of the listed slots, gathered for one download
+__global__ void k_ens_gather_headers(const SysHdr *__restrict__ H, const int *__restrict__ slots,
+                                     SysHdr *__restrict__ out, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = H[slots[k]];
+}
+
 // ===========================================================================
 // host engine
 // ===========================================================================
@@ -931,6 +940,9 @@ struct Engine {
     // stats
     long long engine_steps = 0, n_finished = 0;
     double pairs = 0, dev_seconds = 0;
+    // host time of g6e_run by phase (seconds): retire, admit, rebuild, graphs,
+    // waiting for the lanes, the phase download
+    double host_s[6] = {0, 0, 0, 0, 0, 0};
 };
 
 Engine X;
@@ -1080,12 +1092,17 @@ int retire()
     ECHK(hipGetLastError());
     ECHK(hipMemcpyAsync(X.h_out, X.d_out, sizeof(double) * 8 * total, hipMemcpyDeviceToHost,
                         X.stream));
-    // final headers of the retired slots
-    std::vector<SysHdr> hdr(done.size());
-    for (size_t q = 0; q < done.size(); ++q)
-        ECHK(hipMemcpyAsync(&hdr[q], X.d_hdr + done[q], sizeof(SysHdr), hipMemcpyDeviceToHost,
-                            X.stream));
+    // final headers of the retired slots: gathered on the device, one download
+    for (size_t q = 0; q < done.size(); ++q) X.h_hslots[q] = done[q];
+    ECHK(hipMemcpyAsync(X.d_hslots, X.h_hslots, sizeof(int) * done.size(), hipMemcpyHostToDevice,
+                        X.stream));
+    hipLaunchKernelGGL(k_ens_gather_headers, dim3((unsigned)((done.size() + 255) / 256)),
+                       dim3(256), 0, X.stream, X.d_hdr, X.d_hslots, X.d_hstage, (int)done.size());
+    ECHK(hipGetLastError());
+    ECHK(hipMemcpyAsync(X.h_hstage, X.d_hstage, sizeof(SysHdr) * done.size(),
+                        hipMemcpyDeviceToHost, X.stream));
     ECHK(hipStreamSynchronize(X.stream));
+    std::vector<SysHdr> hdr(X.h_hstage, X.h_hstage + done.size());
     k = 0;
     for (size_t q = 0; q < done.size(); ++q) {
         SlotInfo &si = X.slots[done[q]];
@@ -1434,6 +1451,7 @@ int g6e_open(long long hbm_budget_bytes, int max_systems)
     X.n_finished = 0;
     X.pairs = 0;
     X.dev_seconds = 0;
+    for (double &h : X.host_s) h = 0;
     X.open = true;
     if (rebuild_items()) return -1;
     return 0;
@@ -1492,17 +1510,30 @@ int g6e_run(int steps)
 {
     if (!X.open) return efail(-1, "g6e_run: engine not open");
     int done = 0;
+    using clk = std::chrono::steady_clock;
+    auto since = [](clk::time_point t0) {
+        return std::chrono::duration<double>(clk::now() - t0).count();
+    };
     while (true) {
+        auto t0 = clk::now();
         if (retire()) return -1;
+        X.host_s[0] += since(t0);
+        t0 = clk::now();
         if (admit()) return -1;
+        X.host_s[1] += since(t0);
+        t0 = clk::now();
         if (X.items_dirty && rebuild_items()) return -1;
+        X.host_s[2] += since(t0);
         if (busy_count() == 0 || done >= steps) break;
+        t0 = clk::now();
         Lane &H = X.lanes[0], &Lt = X.lanes[1];
         const bool heavy = H.n_chunks > 0, light = Lt.n_chunks > 0;
         const bool med[4] = {false, false, X.lanes[2].n_chunks > 0, X.lanes[3].n_chunks > 0};
         hipGraphExec_t eh = nullptr, el = nullptr;
         if (heavy && get_graph(0, eh)) return -1;
         if (light && !(X.fused_steps > 0 && X.light_max <= 64) && get_graph(1, el)) return -1;
+        X.host_s[3] += since(t0);
+        t0 = clk::now();
         double ms_total = 0;
         if (heavy) {
             ECHK(hipEventRecord(H.ev0, H.stream));
@@ -1583,9 +1614,12 @@ int g6e_run(int steps)
             M.steps += X.wg_steps;
         }
         ms_total += ms_other;
+        X.host_s[4] += since(t0);
+        t0 = clk::now();
         ECHK(hipMemcpyAsync(X.h_phase, X.d_phase, sizeof(int) * X.max_slots, hipMemcpyDeviceToHost,
                             X.stream));
         ECHK(hipStreamSynchronize(X.stream));
+        X.host_s[5] += since(t0);
         X.dev_seconds += ms_total * 1e-3;
         const int round_steps = heavy ? X.steps_per_graph
                                       : (med[2] || med[3]) ? X.wg_steps : X.steps_per_graph;
@@ -1675,6 +1709,10 @@ long long g6e_get_option(const char *key)
     if (k == "wg_small_max") return X.wg_small_max;
     if (k == "wg_max_n") return X.wg_max_n;
     if (k == "medium_steps") return X.lanes[2].steps + X.lanes[3].steps;
+    static const char *phases[6] = {"host_us_retire", "host_us_admit", "host_us_rebuild",
+                                    "host_us_graphs", "host_us_lanes", "host_us_phase"};
+    for (int q = 0; q < 6; ++q)
+        if (k == phases[q]) return (long long)(X.host_s[q] * 1e6);
     if (k == "light_steps") return X.lanes[1].steps;
     if (k == "heavy_steps") return X.lanes[0].steps;
     if (k == "pool_capacity") return X.P.cap;
