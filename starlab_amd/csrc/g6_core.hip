@@ -563,7 +563,11 @@ int do_close()
     if (E.stream) (void)hipStreamDestroy(E.stream);
     E.stream = nullptr;
     E.open = false;
+    // no calculation survives a close: lasthalf and the neighbour-list calls
+    // report an error until the next firsthalf
     E.calc_pending = false;
+    E.calc_failed = false;
+    E.last_ni = 0;
     E.nb_valid = false;
     return 0;
 }
