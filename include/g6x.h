@@ -52,7 +52,9 @@ int g6x_potential(int cluster_id, int nj, int i_first, int ni, double ti, double
 int g6x_densities(int cluster_id, int nj, int i_first, int ni, double ti, int k,
                   double *density, double *dk2);
 
-/* Copy the results of the last g6x_self_sweep (ni entries) to the host. */
+/* Copy the results of the last g6x_self_sweep (ni entries) to the host.
+ * Returns -1 if the last calculation was anything else (an ABI block,
+ * g6x_potential, g6x_densities) or a close came after it. */
 int g6x_read_results(int cluster_id, int ni, double *acc, double *jerk,
                      double *pot, int *nnbindex);
 

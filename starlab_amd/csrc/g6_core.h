@@ -150,6 +150,7 @@ struct Engine {
     // ---- state of the last calculation -------------------------------------
     bool calc_pending = false;
     bool calc_failed = false;
+    bool sweep_results = false;   // d_res holds the last calculation (g6x_self_sweep)
     int last_ni = 0, last_nj = 0;
     int last_refinements = 0;
     std::vector<double> last_h2;

@@ -568,6 +568,7 @@ int do_close()
     E.calc_pending = false;
     E.calc_failed = false;
     E.last_ni = 0;
+    E.sweep_results = false;
     E.nb_valid = false;
     return 0;
 }
@@ -649,6 +650,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
 {
     E.calc_pending = false;
     E.calc_failed = true;
+    E.sweep_results = false;
     E.nb_valid = false;
     if (ensure_open(who)) return -1;
     if (ni <= 0) return fail(-1, "%s: ni = %d", who, ni);
@@ -912,6 +914,7 @@ int sweep_begin(const char *who, int nj, int i_first, int ni, double ti)
     E.calc_pending = false;
     E.nb_valid = false;
     E.calc_failed = true;   // a sweep leaves no i-block for neighbour lists
+    E.sweep_results = false;   // potential / densities reuse the results buffer
     E.ti = ti;
     if (flush_j(nj)) return -1;
     if (predict(nj, false)) return -1;

@@ -179,6 +179,7 @@ int g6x_self_sweep(int, int nj, int i_first, int ni, double ti, double eps2)
     E.last_ni = ni;
     E.last_nj = nj;
     E.calc_failed = true;   // no neighbour lists for sweeps
+    E.sweep_results = true;
     return 0;
 }
 
@@ -197,6 +198,8 @@ int g6x_densities(int, int nj, int i_first, int ni, double ti, int k, double *de
 int g6x_read_results(int, int ni, double *acc, double *jerk, double *pot, int *nn)
 {
     if (ensure_open("g6x_read_results")) return -1;
+    if (!E.sweep_results)
+        return fail(-1, "g6x_read_results: the last calculation was not a g6x_self_sweep");
     if (ni > E.last_ni) return fail(-1, "g6x_read_results: ni %d > last %d", ni, E.last_ni);
     HIPCHK(hipMemcpyAsync(E.h_res, E.d_res, results_bytes(E.last_ni), hipMemcpyDeviceToHost,
                           E.stream));

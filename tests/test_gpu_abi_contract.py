@@ -100,3 +100,19 @@ def test_unused_entry_points_are_harmless(g):
     load_j(g, s)
     errs = _block(g, s, np.arange(50))
     assert errs["nn_mismatch"] == 0 and errs["acc_norm"] < 1e-6
+
+
+def test_read_results_only_after_a_self_sweep(g):
+    """g6x_read_results returns the last g6x_self_sweep's results, and an
+    error once an ABI block or another sweep has used the device since."""
+    s = plummer_state(2048, seed=59)
+    load_j(g, s)
+    assert g.self_sweep(2048, 0, 2048, 0.0, 1e-8) == 0
+    acc, _, pot, _ = g.read_results(2048)
+    assert np.all(pot < 0) and np.isfinite(acc).all()
+    _block(g, s, np.arange(32))
+    with pytest.raises(RuntimeError, match="self_sweep"):
+        g.read_results(2048)
+    g.potential(2048, 0, 2048, 0.0, 1e-8)
+    with pytest.raises(RuntimeError, match="self_sweep"):
+        g.read_results(2048)
