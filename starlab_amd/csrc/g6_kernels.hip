@@ -1191,19 +1191,28 @@ __device__ int block_exclusive_scan(int *__restrict__ a, int n, int *part)
     const int b0 = min(t * per, n), b1 = min(b0 + per, n);
     int local = 0;
     for (int k = b0; k < b1; ++k) local += a[k];
-    part[t] = local;
+    // exclusive scan of the per-thread sums: inclusive scan inside each wave
+    // (shuffles), then of the kScanThreads / 64 wave totals
+    const int lane = t & 63, wave = t >> 6;
+    int x = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) part[wave] = x;
     __syncthreads();
     if (t == 0) {
         int run = 0;
-        for (int k = 0; k < kScanThreads; ++k) {
-            const int v = part[k];
-            part[k] = run;
+        for (int w = 0; w < kScanThreads / 64; ++w) {
+            const int v = part[w];
+            part[w] = run;
             run += v;
         }
         part[kScanThreads] = run;
     }
     __syncthreads();
-    int run = part[t];
+    int run = part[wave] + x - local;
     for (int k = b0; k < b1; ++k) {
         const int v = a[k];
         a[k] = run;
