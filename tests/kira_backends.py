@@ -7,6 +7,10 @@ CheckedG6     -- libg6 through the ABI, plus a host mirror of j-memory; after
                  each force call the oracle recomputes the same block from the
                  same inputs (GRAPE-form prediction, bitwise the device's) and
                  the per-block errors are recorded.
+OracleAbiBackend -- the arithmetic of tests/oracle_g6abi.c (a GRAPE-6 ABI
+                 library on the oracle) in Python, for bitwise comparisons of
+                 the Python and native drivers without a GPU.
+NativeChecker -- the CheckedG6 checks as an observer of the native driver.
 """
 import numpy as np
 
@@ -67,12 +71,12 @@ class OracleAbiBackend:
         pass
 
 
-class CheckedG6(G6Backend):
-    """every: check every `every`-th force call; max_i: at most this many i
-    of a checked call (a seeded sample) go to the oracle."""
+class _ErrorBook:
+    """Per-block errors of library results against the FP64 oracle on the
+    same inputs: normwise acc error |da| / sum_j |a_ij|, jerk likewise
+    (|dj| / sum_j |j_ij| bound), pot relative, nearest-neighbour mismatches."""
 
-    def __init__(self, g, bulk=False, every=1, max_i=None, seed=0):
-        super().__init__(g, bulk)
+    def _book_init(self, every, max_i, seed):
         self.worst = dict(acc_norm=0.0, jerk_norm=0.0, pot_rel=0.0)
         self.acc_errs = []
         self.nn_mismatch = 0
@@ -82,6 +86,48 @@ class CheckedG6(G6Backend):
         self.every = every
         self.max_i = max_i
         self.rng = np.random.default_rng(seed)
+
+    def _book(self, xp, vp, mass, jidx, t, idx, xi, vi, acc, jerk, pot, nn, eps2):
+        """Check one force call (if it is an `every`-th one); xp, vp = the
+        oracle's prediction of all j at t, or a callable producing them."""
+        self.calls += 1
+        if (self.calls - 1) % self.every:
+            return
+        sel = np.arange(len(idx))
+        if self.max_i is not None and len(idx) > self.max_i:
+            sel = np.sort(self.rng.choice(len(idx), self.max_i, replace=False))
+        acc, jerk, pot, nn = acc[sel], jerk[sel], pot[sel], nn[sel]
+        idx, xi, vi = idx[sel], xi[sel], vi[sel]
+        if callable(xp):
+            xp, vp = xp()
+        racc, rjerk, rpot, rnn, _ = oracle.force(xi, vi, idx, xp, vp, mass, jidx, eps2)
+        sa = np.zeros(len(idx)); sj = np.zeros(len(idx))
+        for k, i in enumerate(idx):
+            d = xp - xi[k]; w = vp - vi[k]
+            r2 = (d * d).sum(1) + eps2
+            r2[i] = np.inf
+            sa[k] = (mass / r2).sum()
+            sj[k] = (mass * 4.0 * np.sqrt((w * w).sum(1)) / (r2 * np.sqrt(r2))).sum()
+        ea = np.linalg.norm(acc - racc, axis=1) / sa
+        self.acc_errs.append(ea)
+        self.worst["acc_norm"] = max(self.worst["acc_norm"], float(ea.max()))
+        self.worst["jerk_norm"] = max(self.worst["jerk_norm"],
+                                      float((np.linalg.norm(jerk - rjerk, axis=1) / sj).max()))
+        self.worst["pot_rel"] = max(self.worst["pot_rel"],
+                                    float((np.abs(pot - rpot) / np.abs(rpot)).max()))
+        self.nn_mismatch += int((nn != rnn).sum())
+        self.checked_blocks += 1
+        self.checked_i += len(idx)
+
+
+class CheckedG6(G6Backend, _ErrorBook):
+    """libg6 through the ABI with a host mirror of j-memory.  every: check
+    every `every`-th force call; max_i: at most this many i of a checked call
+    (a seeded sample) go to the oracle."""
+
+    def __init__(self, g, bulk=False, every=1, max_i=None, seed=0):
+        super().__init__(g, bulk)
+        self._book_init(every, max_i, seed)
 
     def open(self, n):
         super().open(n)
@@ -96,33 +142,25 @@ class CheckedG6(G6Backend):
         M["a2"][idx] = a / 2.0; M["j6"][idx] = j / 6.0; M["k18"][idx] = k18
 
     def force(self, nj, t, idx, xi, vi, eps2):
-        acc_all, jerk_all, pot_all, nn_all = super().force(nj, t, idx, xi, vi, eps2)
-        self.calls += 1
-        if (self.calls - 1) % self.every:
-            return acc_all, jerk_all, pot_all, nn_all
-        sel = np.arange(len(idx))
-        if self.max_i is not None and len(idx) > self.max_i:
-            sel = np.sort(self.rng.choice(len(idx), self.max_i, replace=False))
-        acc, jerk, pot, nn = acc_all[sel], jerk_all[sel], pot_all[sel], nn_all[sel]
-        idx, xi, vi = idx[sel], xi[sel], vi[sel]
+        acc, jerk, pot, nn = super().force(nj, t, idx, xi, vi, eps2)
         M = self.mir
-        xp, vp = oracle.predict(M["x"], M["v"], M["a2"], M["j6"], M["k18"], M["t"], t, 4)
-        racc, rjerk, rpot, rnn, _ = oracle.force(xi, vi, idx, xp, vp, M["m"], self.idx, eps2)
-        sa = np.zeros(len(idx)); sj = np.zeros(len(idx))
-        for k, i in enumerate(idx):
-            d = xp - xi[k]; w = vp - vi[k]
-            r2 = (d * d).sum(1) + eps2
-            r2[i] = np.inf
-            sa[k] = (M["m"] / r2).sum()
-            sj[k] = (M["m"] * 4.0 * np.sqrt((w * w).sum(1)) / (r2 * np.sqrt(r2))).sum()
-        ea = np.linalg.norm(acc - racc, axis=1) / sa
-        self.acc_errs.append(ea)
-        self.worst["acc_norm"] = max(self.worst["acc_norm"], float(ea.max()))
-        self.worst["jerk_norm"] = max(self.worst["jerk_norm"],
-                                      float((np.linalg.norm(jerk - rjerk, axis=1) / sj).max()))
-        self.worst["pot_rel"] = max(self.worst["pot_rel"],
-                                    float((np.abs(pot - rpot) / np.abs(rpot)).max()))
-        self.nn_mismatch += int((nn != rnn).sum())
-        self.checked_blocks += 1
-        self.checked_i += len(idx)
-        return acc_all, jerk_all, pot_all, nn_all
+        pred = lambda: oracle.predict(M["x"], M["v"], M["a2"], M["j6"], M["k18"], M["t"], t, 4)
+        self._book(pred, None, M["m"], self.idx, t, idx, xi, vi, acc, jerk, pot, nn, eps2)
+        return acc, jerk, pot, nn
+
+
+class NativeChecker(_ErrorBook):
+    """Observer for NativeBlockHermite: the same oracle checks as CheckedG6,
+    from the j-memory state the native driver reports at each force call."""
+
+    def __init__(self, eps2, every=1, max_i=None, seed=0):
+        self.eps2 = eps2
+        self._book_init(every, max_i, seed)
+
+    def __call__(self, jm, t, idx, xi, vi, acc, jerk, pot, nn):
+        pred = lambda: oracle.predict(jm["x"], jm["v"], jm["acc"] / 2.0, jm["jerk"] / 6.0,
+                                      jm["k18"], jm["t"], t, 4)
+        jidx = np.arange(len(jm["m"]), dtype=np.int32)
+        self._book(pred, None, jm["m"].copy(), jidx, t, idx.copy(), xi.copy(), vi.copy(),
+                   acc.copy(), jerk.copy(), pot.copy(), nn.copy(), self.eps2)
+        return False
