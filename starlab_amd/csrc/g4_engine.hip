@@ -166,6 +166,11 @@ int firsthalf(int nj, int ni, const double (*xi)[3], const double (*vi)[3],
         E.calc_failed = true;
         return fail(-1, "h3calc_firsthalf: ni = %d outside [1, npipes = %d]", ni, E.opt.npipes);
     }
+    if (!xi || !vi || !eps2 || !h2) {
+        E.calc_pending = false;
+        E.calc_failed = true;
+        return fail(-1, "h3calc_firsthalf: null position, velocity, eps2 or h2 array");
+    }
     // distinct i-particles in pipe order
     std::vector<int> first_pipe;       // distinct slot -> its first pipe
     G.slot_of_pipe.assign(ni, 0);

@@ -353,9 +353,10 @@ int count_refined(const Results &h, int ni)
 int ensure_partials(size_t bytes)
 {
     if (bytes <= E.pbytes) return 0;
+    // grow geometrically, so blocks of slowly rising size reallocate rarely
+    const size_t b = std::max(bytes, E.pbytes * 2);
     dfree(E.d_part);
     E.pbytes = 0;
-    size_t b = std::max(bytes, E.pbytes * 2);
     HIPCHK(hipMalloc((void **)&E.d_part, b));
     E.pbytes = b;
     return 0;
@@ -672,7 +673,9 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     double *d = reinterpret_cast<double *>(E.h_ibuf + (size_t)ld * 40);
     double *hh = d + 3 * (size_t)ld;
     double *ee = d + 4 * (size_t)ld;
-    E.last_h2.assign(h2, h2 + ni);
+    // no h2 array: no neighbour spheres
+    if (h2) E.last_h2.assign(h2, h2 + ni);
+    else E.last_h2.assign(ni, 0.0);
     for (int k = 0; k < ld; ++k) {
         const int s = k < ni ? k : ni - 1;
         for (int c = 0; c < 3; ++c) {
@@ -684,7 +687,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
             d[c * (size_t)ld + k] = x;
         }
         ex[k] = (k < ni && excl) ? excl[k] : -1;
-        hh[k] = k < ni ? h2[k] : 0.0;
+        hh[k] = (k < ni && h2) ? h2[k] : 0.0;
         ee[k] = (k < ni && e2) ? e2[k] : 0.0;
     }
     HIPCHK(hipMemcpyAsync(E.d_ibuf, E.h_ibuf, ibuf_bytes(ld), hipMemcpyHostToDevice, E.stream));
@@ -1022,6 +1025,12 @@ int autotune(int nj, int ni_max, double eps2)
     if (E.calc_pending) return fail(-1, "g6x_autotune: a calculation is in flight");
     if (nj <= 0 || nj > E.n_addr) return fail(-1, "g6x_autotune: nj = %d outside j-memory", nj);
     ni_max = std::min(ni_max, nj);
+    // the candidates overwrite the i-block and the results buffer: the last
+    // block's neighbour lists and a self-sweep's results are gone
+    E.last_ni = 0;
+    E.calc_failed = true;
+    E.nb_valid = false;
+    E.sweep_results = false;
     if (flush_j(nj)) return -1;
     if (predict(nj, true)) return -1;
     const int saved_timing = E.opt.timing;

@@ -34,6 +34,11 @@ int firsthalf(int nj, int ni, const int *index, const double (*xi)[3],
         E.calc_failed = true;
         return fail(-1, "g6calc_firsthalf: ni = %d outside [1, npipes = %d]", ni, E.opt.npipes);
     }
+    if (!index || !xi || !vi) {
+        E.calc_pending = false;
+        E.calc_failed = true;
+        return fail(-1, "g6calc_firsthalf: null index, position or velocity array");
+    }
     std::vector<int> excl(ni);
     for (int k = 0; k < ni; ++k) excl[k] = excl_address(index[k]);
     const IGroup g{0, ni, (float)eps2, false};

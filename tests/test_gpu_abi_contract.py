@@ -116,3 +116,83 @@ def test_read_results_only_after_a_self_sweep(g):
     g.potential(2048, 0, 2048, 0.0, 1e-8)
     with pytest.raises(RuntimeError, match="self_sweep"):
         g.read_results(2048)
+
+
+def _raw_block(g, nj, index, xi, vi, h2, eps2=1e-8):
+    """g6calc_firsthalf_ / g6calc_lasthalf2_ through the raw C entry points,
+    so that NULL arrays reach the library as a C caller would pass them."""
+    import ctypes as C
+    ni = next(len(a) for a in (index, xi, vi, h2) if a is not None)
+    cid, cnj, cni = C.c_int(0), C.c_int(nj), C.c_int(ni)
+    e = C.c_double(eps2)
+    z3 = np.zeros((ni, 3))
+    zp = np.zeros(ni)
+    acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)
+    nn = np.zeros(ni, np.int32)
+    ptr = lambda a: None if a is None else a.ctypes.data   # noqa: E731
+    args = (ptr(index), ptr(xi), ptr(vi))
+    g.L.g6calc_firsthalf_(C.byref(cid), C.byref(cnj), C.byref(cni), *args, z3.ctypes.data,
+                          z3.ctypes.data, zp.ctypes.data, C.byref(e), ptr(h2))
+    first_error = g.last_error()
+    rc = g.L.g6calc_lasthalf2_(C.byref(cid), C.byref(cnj), C.byref(cni), *args, C.byref(e),
+                               ptr(h2), acc.ctypes.data, jerk.ctypes.data, pot.ctypes.data,
+                               nn.ctypes.data)
+    return rc, acc, jerk, pot, nn, first_error
+
+
+def test_null_arrays_from_a_c_caller(g):
+    """A NULL h2 means no neighbour spheres (forces as usual, empty lists);
+    NULL indices, positions or velocities are an error, not a crash."""
+    n, ni = 1500, 32
+    s = plummer_state(n, seed=61)
+    load_j(g, s)
+    g.set_ti(0.0)
+    xp, vp = oracle.predict(s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["tj"], 0.0, 4)
+    isel = np.arange(ni)
+    idx = np.ascontiguousarray(s["idx"][isel])
+    xi = np.ascontiguousarray(xp[isel])
+    vi = np.ascontiguousarray(vp[isel])
+    rc, acc, jerk, pot, nn, _ = _raw_block(g, n, idx, xi, vi, None)
+    assert rc == 0, g.last_error()
+    ref = oracle.force(xi, vi, idx, xp, vp, s["m"], s["idx"], 1e-8)
+    errs = compare(acc, jerk, pot, nn, ref, xi, vi, s["m"], xp, vp, 1e-8, isel)
+    assert errs["nn_mismatch"] == 0 and errs["acc_norm"] < 1e-6
+    assert g.read_neighbour_list() == 0
+    for pipe in (0, ni - 1):
+        rc, lst = g.get_neighbour_list(pipe, 16)
+        assert rc == 0 and len(lst) == 0
+    for missing in range(3):
+        arrs = [idx, xi, vi]
+        arrs[missing] = None
+        rc, *_, first_error = _raw_block(g, n, *arrs, np.zeros(ni))
+        assert rc != 0 and "null" in first_error
+    errs = _block(g, s, np.arange(64))
+    assert errs["nn_mismatch"] == 0 and errs["acc_norm"] < 1e-6
+
+
+def test_autotune_drops_lists_and_sweep_results(g):
+    """g6x_autotune runs its candidates on the i-block and results buffers:
+    afterwards the last block's neighbour lists and the last self-sweep's
+    results report errors instead of returning the tuner's data."""
+    n = 2048
+    s = plummer_state(n, seed=63)
+    load_j(g, s)
+    try:
+        assert g.self_sweep(n, 0, n, 0.0, 1e-8) == 0
+        g.autotune(n, 64, 1e-8)
+        with pytest.raises(RuntimeError, match="self_sweep"):
+            g.read_results(n)
+        g.set_ti(0.0)
+        xp, vp = oracle.predict(s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["tj"], 0.0, 4)
+        idx = np.arange(8, dtype=np.int32)
+        g.calc_firsthalf(n, idx, xp[:8], vp[:8], 1e-8, np.full(8, 0.01))
+        rc, *_ = g.calc_lasthalf2(n, 8)
+        assert rc == 0
+        g.autotune(n, 64, 1e-8)
+        assert g.read_neighbour_list() < 0
+        rc, lst = g.get_neighbour_list(0, 16)
+        assert rc != 0 and len(lst) == 0
+    finally:
+        g.clear_tuning()
+    errs = _block(g, s, np.arange(64))
+    assert errs["nn_mismatch"] == 0 and errs["acc_norm"] < 1e-6
