@@ -16,7 +16,8 @@ extern "C" {
 #endif
 
 /* Bulk form of g6_set_j_particle_: n particles, address[k] / index[k],
- * tj[k], dtj[k], mass[k], and k18, j6, a2, v, x as double[n][3]. */
+ * tj[k], dtj[k], mass[k], and k18, j6, a2, v, x as double[n][3].  dtj may
+ * be NULL (1.0 each); any other NULL array is an error. */
 int g6x_set_j_particles(int cluster_id, int n, const int *address,
                         const int *index, const double *tj, const double *dtj,
                         const double *mass, const double *k18, const double *j6,
@@ -54,7 +55,8 @@ int g6x_densities(int cluster_id, int nj, int i_first, int ni, double ti, int k,
 
 /* Copy the results of the last g6x_self_sweep (ni entries) to the host.
  * Returns -1 if the last calculation was anything else (an ABI block,
- * g6x_potential, g6x_densities) or a close came after it. */
+ * g6x_potential, g6x_densities, g6x_autotune), if that sweep failed, or if a
+ * close came after it. */
 int g6x_read_results(int cluster_id, int ni, double *acc, double *jerk,
                      double *pot, int *nnbindex);
 

@@ -55,7 +55,8 @@ typedef int (*bs_block_fn)(void *user, const bs_jmem *jmem, double t, int ni,
  * with g6x_set_j_particles when the library has it (the same records in one
  * call), else one g6_set_j_particle_ per particle as kira does.
  *
- * In: m[n]; x, v as double[n][3] (overwritten with the state at t_end).
+ * In: m[n]; x, v as double[n][3] (overwritten with the state at t_end);
+ * t_end, eta, dt_max > 0 and eps2 >= 0, all finite (else an error).
  * Out (any may be NULL): acc, jerk as double[n][3]; pot[n]; dt[n] (the last
  * step of each particle); stats[4] = blocks, particle steps, seconds spent
  * inside the ABI calls, seconds in total.

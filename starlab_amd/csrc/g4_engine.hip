@@ -109,6 +109,11 @@ void dma_copy(int nj, const int *hostindex, const double (*xj)[3], const double 
               const double *tj, int bufno)
 {
     if (nj < 0) { fail(-1, "h3mjpdma_indirect: nj = %d", nj); fatal(); return; }
+    if (nj > 0 && (!hostindex || !xj || !vj || !aj || !jj || !mj || !tj)) {
+        fail(-1, "h3mjpdma_indirect: null array");
+        fatal();
+        return;
+    }
     if (bufno < 0 || bufno >= kMaxDmaBuffers) {
         fail(-1, "h3mjpdma_indirect: buffer %d outside [0, %d)", bufno, kMaxDmaBuffers);
         fatal();

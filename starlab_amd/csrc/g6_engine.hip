@@ -158,6 +158,9 @@ int g6x_set_j_particles(int, int n, const int *address, const int *index,
                         const double *k18, const double *j6, const double *a2,
                         const double *v, const double *x)
 {
+    if (n < 0) return fail(-1, "g6x_set_j_particles: n = %d", n);
+    if (n > 0 && (!address || !index || !tj || !mass || !k18 || !j6 || !a2 || !v || !x))
+        return fail(-1, "g6x_set_j_particles: null array (only dtj may be NULL)");
     for (int k = 0; k < n; ++k) {
         int rc = set_j(address[k], index[k], tj[k], dtj ? dtj[k] : 1.0, mass[k],
                        k18 + 3 * (size_t)k, j6 + 3 * (size_t)k, a2 + 3 * (size_t)k,
@@ -170,6 +173,7 @@ int g6x_set_j_particles(int, int n, const int *address, const int *index,
 int g6x_self_sweep(int, int nj, int i_first, int ni, double ti, double eps2)
 {
     if (ensure_open("g6x_self_sweep")) return -1;
+    E.sweep_results = false;   // until this sweep is queued
     if (ni <= 0 || i_first < 0 || i_first + ni > nj || nj <= 0)
         return fail(-1, "g6x_self_sweep: bad range i_first=%d ni=%d nj=%d", i_first, ni, nj);
     E.calc_pending = false;
@@ -205,7 +209,8 @@ int g6x_read_results(int, int ni, double *acc, double *jerk, double *pot, int *n
     if (ensure_open("g6x_read_results")) return -1;
     if (!E.sweep_results)
         return fail(-1, "g6x_read_results: the last calculation was not a g6x_self_sweep");
-    if (ni > E.last_ni) return fail(-1, "g6x_read_results: ni %d > last %d", ni, E.last_ni);
+    if (ni < 0 || ni > E.last_ni)
+        return fail(-1, "g6x_read_results: ni %d outside [0, %d]", ni, E.last_ni);
     HIPCHK(hipMemcpyAsync(E.h_res, E.d_res, results_bytes(E.last_ni), hipMemcpyDeviceToHost,
                           E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
@@ -222,6 +227,7 @@ int g6x_read_results(int, int ni, double *acc, double *jerk, double *pot, int *n
 int g6x_read_predicted(int, int nj, double *xp, double *vp)
 {
     if (ensure_open("g6x_read_predicted")) return -1;
+    if (nj < 0) return fail(-1, "g6x_read_predicted: nj = %d", nj);
     if (!E.pred_valid || nj > E.pred_nj) return fail(-1, "g6x_read_predicted: no prediction");
     std::vector<double> buf(3 * (size_t)nj);
     for (int pass = 0; pass < 2; ++pass) {

@@ -1371,12 +1371,37 @@ int busy_count()
 using namespace g6;
 using namespace g6::ens;
 
-#pragma GCC visibility push(default)
-extern "C" {
+namespace {
 
-int g6e_open(long long hbm_budget_bytes, int max_systems)
+// Free everything the engine holds (null-safe: also after a failed open).
+void release_all()
 {
-    if (X.open) return efail(-1, "g6e_open: already open");
+    if (X.stream) (void)hipStreamSynchronize(X.stream);
+    for (Lane &L : X.lanes) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        destroy_graphs(L);
+        hfree(L.h_chunks); dfree(L.d_chunks); hfree(L.h_items); dfree(L.d_items);
+        dfree(L.d_counts); hfree(L.h_counts);
+        L.chunk_cap = L.item_cap = 0;
+        if (L.ev0) (void)hipEventDestroy(L.ev0);
+        if (L.ev1) (void)hipEventDestroy(L.ev1);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+        L = Lane();
+    }
+    dfree(X.P.x); dfree(X.P.v); dfree(X.P.a); dfree(X.P.j); dfree(X.P.xp); dfree(X.P.vp);
+    dfree(X.P.m); dfree(X.P.pot); dfree(X.P.pot0); dfree(X.P.jp); dfree(X.P.jp64); dfree(X.P.part);
+    dfree(X.d_hdr); dfree(X.d_phase); hfree(X.h_phase);
+    hfree(X.h_hstage); dfree(X.d_hstage); hfree(X.h_hslots); dfree(X.d_hslots);
+    hfree(X.h_stage); dfree(X.d_stage); X.stage_cap = 0;
+    hfree(X.h_out); dfree(X.d_out); hfree(X.h_src); dfree(X.d_src); X.out_cap = 0;
+    if (X.stream) (void)hipStreamDestroy(X.stream);
+    X.stream = nullptr;
+    X.slots.clear();
+    X.open = false;
+}
+
+int open_engine(long long hbm_budget_bytes, int max_systems)
+{
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return efail(-1, "g6e_open: no HIP device available");
@@ -1457,31 +1482,27 @@ int g6e_open(long long hbm_budget_bytes, int max_systems)
     return 0;
 }
 
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+int g6e_open(long long hbm_budget_bytes, int max_systems)
+{
+    if (X.open) return efail(-1, "g6e_open: already open");
+    if (open_engine(hbm_budget_bytes, max_systems)) {
+        const std::string err = X.err;
+        release_all();   // nothing of a failed open stays allocated
+        X.err = err;
+        return -1;
+    }
+    return 0;
+}
+
 int g6e_close(void)
 {
     if (!X.open) return 0;
-    (void)hipStreamSynchronize(X.stream);
-    for (Lane &L : X.lanes) {
-        if (L.stream) (void)hipStreamSynchronize(L.stream);
-        destroy_graphs(L);
-        hfree(L.h_chunks); dfree(L.d_chunks); hfree(L.h_items); dfree(L.d_items);
-        dfree(L.d_counts); hfree(L.h_counts);
-        L.chunk_cap = L.item_cap = 0;
-        if (L.ev0) (void)hipEventDestroy(L.ev0);
-        if (L.ev1) (void)hipEventDestroy(L.ev1);
-        if (L.stream) (void)hipStreamDestroy(L.stream);
-        L = Lane();
-    }
-    dfree(X.P.x); dfree(X.P.v); dfree(X.P.a); dfree(X.P.j); dfree(X.P.xp); dfree(X.P.vp);
-    dfree(X.P.m); dfree(X.P.pot); dfree(X.P.pot0); dfree(X.P.jp); dfree(X.P.jp64); dfree(X.P.part);
-    dfree(X.d_hdr); dfree(X.d_phase); hfree(X.h_phase);
-    hfree(X.h_hstage); dfree(X.d_hstage); hfree(X.h_hslots); dfree(X.d_hslots);
-    hfree(X.h_stage); dfree(X.d_stage); X.stage_cap = 0;
-    hfree(X.h_out); dfree(X.d_out); hfree(X.h_src); dfree(X.d_src); X.out_cap = 0;
-    (void)hipStreamDestroy(X.stream);
-    X.stream = nullptr;
-    X.slots.clear();
-    X.open = false;
+    release_all();
     return 0;
 }
 
@@ -1490,6 +1511,7 @@ long long g6e_submit(int n, const double *m, const double *x, const double *v, d
 {
     if (!X.open) return efail(-1, "g6e_submit: engine not open");
     if (n < 1 || n > X.P.cap) return efail(-1, "g6e_submit: n = %d outside [1, %d]", n, X.P.cap);
+    if (!m || !x || !v) return efail(-1, "g6e_submit: null m, x or v");
     if (!(t_end > 0) || !(eta > 0) || !(eps2 >= 0))
         return efail(-1, "g6e_submit: bad t_end/eta/eps2");
     Job j;
@@ -1631,6 +1653,7 @@ int g6e_run(int steps)
 
 int g6e_poll(int max, long long *job_ids)
 {
+    if (max > 0 && !job_ids) return efail(-1, "g6e_poll: null job_ids");
     int k = 0;
     while (k < max && !X.finished.empty()) {
         job_ids[k++] = X.finished.front();

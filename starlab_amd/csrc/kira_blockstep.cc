@@ -196,6 +196,10 @@ int bs_run(const char *grape_lib, int n, const double *m, double *x, double *v, 
 {
     g_err.clear();
     if (n < 1 || !m || !x || !v || !grape_lib) return bs_fail(-1, "bs_run: bad arguments");
+    // zero or non-finite steps would never reach t_end
+    if (!(t_end > 0) || !(eta > 0) || !(eps2 >= 0) || !(dt_max > 0) || !std::isfinite(t_end) ||
+        !std::isfinite(eta) || !std::isfinite(eps2) || !std::isfinite(dt_max))
+        return bs_fail(-1, "bs_run: need t_end > 0, eta > 0, eps2 >= 0, dt_max > 0 (finite)");
     const double t_start = now_s();
     Run R;
     if (R.abi.load(grape_lib)) return -1;

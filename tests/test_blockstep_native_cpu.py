@@ -74,3 +74,14 @@ def test_observer_can_stop_and_errors_surface(shim, tmp_path):
         NativeBlockHermite(m, x, v).run(0.125, grape_lib=shim, observer=lambda *a: 1 / 0)
     with pytest.raises(RuntimeError, match="dlopen failed"):
         NativeBlockHermite(m, x, v).run(0.125, grape_lib=str(tmp_path / "missing.so"))
+
+
+@pytest.mark.parametrize("kw,t_end", [(dict(eta=0.0), 0.25), (dict(eps2=-1.0), 0.25),
+                                      (dict(dt_max=0.0), 0.25), ({}, 0.0),
+                                      (dict(eta=float("nan")), 0.25), ({}, float("inf"))])
+def test_bad_scalars_are_errors(shim, kw, t_end):
+    """Arguments that would give zero or non-finite steps (a run that never
+    ends) are rejected before the library is opened."""
+    m, x, v = oracle.makeplummer(16, 5)
+    with pytest.raises(RuntimeError, match="t_end > 0"):
+        NativeBlockHermite(m, x, v, **{"eta": 0.1, "eps2": 1e-4, **kw}).run(t_end, grape_lib=shim)

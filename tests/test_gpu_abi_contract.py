@@ -196,3 +196,25 @@ def test_autotune_drops_lists_and_sweep_results(g):
         g.clear_tuning()
     errs = _block(g, s, np.arange(64))
     assert errs["nn_mismatch"] == 0 and errs["acc_norm"] < 1e-6
+
+
+def test_extension_argument_errors(g):
+    """Negative counts and NULL arrays of the extension entry points are
+    errors; a failed self-sweep leaves no results to read."""
+    import ctypes as C
+    L = g.L
+    s = plummer_state(600, seed=65)
+    load_j(g, s)
+    assert L.g6x_set_j_particles(0, -1, None, None, None, None, None, None, None, None,
+                                 None, None) != 0
+    assert L.g6x_set_j_particles(0, 1, None, None, None, None, None, None, None, None,
+                                 None, None) != 0
+    assert L.g6x_set_j_particles(0, 0, None, None, None, None, None, None, None, None,
+                                 None, None) == 0
+    assert g.self_sweep(600, 0, 600, 0.0, 1e-8) == 0
+    buf = np.zeros(3 * 600)
+    assert L.g6x_read_results(0, -1, buf.ctypes.data, None, None, None) != 0
+    assert L.g6x_read_predicted(0, -5, buf.ctypes.data, None) != 0
+    assert g.self_sweep(600, 0, 601, 0.0, 1e-8) != 0      # range outside j-memory
+    with pytest.raises(RuntimeError, match="self_sweep"):
+        g.read_results(600)

@@ -217,3 +217,25 @@ def test_autotune_chooses_from_its_space():
     for j, x in zip(jobs, x0):
         np.testing.assert_array_equal(j["x"], x)
 
+
+
+def test_failed_open_releases_and_bad_arguments_are_errors():
+    """A g6e_open that fails after creating its streams (a budget beyond the
+    pool's index range) frees them: the next open works.  NULL arrays to
+    g6e_submit and g6e_poll are errors, and the engine carries on."""
+    from starlab_amd import load
+    L = load()
+    assert L.g6e_open(1 << 62, 16) != 0
+    assert b"too large" in L.g6e_last_error()
+    eng = Ensemble(GB // 4, 16)
+    try:
+        assert eng.L.g6e_submit(3, None, None, None, 1.0, 0.1, 0.0) < 0
+        assert b"null" in eng.L.g6e_last_error()
+        assert eng.L.g6e_poll(4, None) < 0
+        (job,) = _jobs(67, [3], 1.0 / 16)
+        jid = eng.submit(job["m"], job["x"], job["v"], job["t_end"], job["eta"], job["eps2"])
+        res = eng.run_until_idle()
+        assert set(res) == {jid}
+        assert res[jid]["t"] == job["t_end"] and not res[jid]["failed"]
+    finally:
+        eng.close()
