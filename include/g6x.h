@@ -84,12 +84,17 @@ int g6x_sync(int cluster_id);
  *   "wpg"            waves per force workgroup, each on its own j-split, folded
  *                    in LDS before the reduction: 0 auto, 1, 2, 4, 8, 16  G6_WPG
  *   "last_wpg", "last_lanes"  (read only) of the last force launch
+ *   "fused_small"    1 (default) = ABI blocks of at most 64 i with one
+ *                    softening predict j inside the force kernel (no
+ *                    predictor launch, no JPred round trip; bitwise the
+ *                    same results); 0 = predictor first   G6_FUSED_SMALL
  * Returns 0, or -1 for an unknown key. */
 int       g6x_set_option(const char *key, long long value);
 long long g6x_get_option(const char *key);
 
 /* Per-kernel event timing, accumulated since the last reset (needs
- * "timing" = 1).  kernel: 0 predictor, 1 force, 2 reduce (includes the
+ * "timing" = 1).  kernel: 0 predictor, 1 force (with the prediction for
+ * fused small blocks), 2 reduce (includes the
  * FP64 nearest-neighbour rescan of ambiguous i), 3 unused (0 launches),
  * 4 neighbour list, 5 j scatter, 6 potential (g6x_potential), 7 k-nearest
  * neighbours (g6x_densities).  Returns launches; *ms = total ms.  Kernel 4

@@ -406,6 +406,7 @@ int g4x_set_option(const char *key, long long value)
     else if (k == "exit_on_error") G.exit_on_error = (int)value;
     else if (k == "dedupe") G.dedupe = (int)value;
     else if (k == "timing") E.opt.timing = (int)value;
+    else if (k == "fused_small") E.opt.fused_small = value != 0 ? 1 : 0;
     else return -1;
     return 0;
 }
@@ -420,6 +421,7 @@ long long g4x_get_option(const char *key)
     if (k == "exit_on_error") return G.exit_on_error;
     if (k == "dedupe") return G.dedupe;
     if (k == "timing") return E.opt.timing;
+    if (k == "fused_small") return E.opt.fused_small;
     if (k == "last_slots") return G.n_slots;
     if (k == "last_groups") return (long long)G.groups.size();
     return -1;

@@ -43,6 +43,7 @@ struct Options {
     int tile = 0;             // j per FP32 tile of the scalar-load kernel: 0 auto, 64/128/256
     int reduce_lanes = 0;     // lanes per i in the split reduction: 0 auto, 4..64
     int wpg = 0;              // waves per force workgroup (j-splits folded in LDS): 0 auto, 1..16
+    int fused_small = 1;      // ABI blocks of one i-group predict j inside the force kernel
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every
                               // size so far, so off unless asked for)
@@ -102,9 +103,12 @@ struct Engine {
     char *h_stage = nullptr;    // pinned: n records, n addresses, n index fields;
     const char *h_stage_dev = nullptr;   // its device address (k_scatter reads it)
 
-    bool pred_valid = false;
+    bool pred_valid = false;   // JPred, xp64, vp64 hold the prediction at pred_ti
     double pred_ti = 0;
     int pred_nj = 0, pred_order = 0;
+    bool xp_valid = false;     // xp64, vp64 hold the prediction at xp_ti (the predictor
+    double xp_ti = 0;          // or a fused small block wrote them last)
+    int xp_nj = 0;
     double ti = 0;
 
     // ---- i-block ----------------------------------------------------------
@@ -196,8 +200,10 @@ size_t results_bytes(int ni);
 IBlock iblock_at(char *base, int ld);
 Results results_at(char *base, int ni);
 int count_refined(const Results &h, int ni);
+// fused (optional): predict j inside the force kernel instead of reading the
+// JPred array (one i-group; the caller has not run the predictor)
 int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r,
-                  bool zskip = false);
+                  bool zskip = false, const FusedPredict *fused = nullptr);
 
 // One i-block through the force path.  Slot k has position x[k], velocity
 // v[k], excluded j address excl[k] (-1: none; nullptr: none for all), and

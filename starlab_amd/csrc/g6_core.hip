@@ -72,6 +72,7 @@ void read_env()
     o.tile = (int)env_ll("TILE", o.tile);
     o.reduce_lanes = (int)env_ll("REDUCE_LANES", o.reduce_lanes);
     o.wpg = (int)env_ll("WPG", o.wpg);
+    o.fused_small = (int)env_ll("FUSED_SMALL", o.fused_small);
     if (o.npipes < 1) o.npipes = 1;
     if (o.nb_cap < 1) o.nb_cap = 1;
 }
@@ -157,6 +158,7 @@ void free_device_j()
     dfree(E.d_jraw); dfree(E.d_jpred); dfree(E.d_xp64); dfree(E.d_vp64); dfree(E.d_jindex);
     E.dcap = 0;
     E.pred_valid = false;
+    E.xp_valid = false;
 }
 
 // Device j capacity >= need; a (re)allocation uploads the whole mirror.
@@ -274,6 +276,9 @@ int predict(int nj, bool force)
     E.pred_ti = E.ti;
     E.pred_nj = nj;
     E.pred_order = E.opt.predict_order;
+    E.xp_valid = true;
+    E.xp_ti = E.ti;
+    E.xp_nj = nj;
     return 0;
 }
 
@@ -472,7 +477,8 @@ Geometry choose_geometry(int ni, int nj)
 
 // Enqueue force + reduction (with the nearest-neighbour refinement) for an
 // i-block already in d_ibuf.
-int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r, bool zskip)
+int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r, bool zskip,
+                  const FusedPredict *fused)
 {
     if (zskip && !(eps2 >= __FLT_MIN__)) eps2 = 0.0f;   // as launch_force treats it
     const Geometry g = choose_geometry(ni, nj);
@@ -489,12 +495,12 @@ int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r
     {
         KTimer t(K_FORCE);
         HIPCHK(launch_force(E.stream, E.d_jpred, nj, ib, ni, eps2, g.nsplit, g.jchunk, g.tile,
-                            p, E.opt.xcd_remap != 0, g.variant, zskip, g.wpg));
+                            p, E.opt.xcd_remap != 0, g.variant, zskip, g.wpg, fused));
         t.stop();
     }
     {
         KTimer t(K_REDUCE);
-        HIPCHK(launch_reduce(E.stream, p, g.npart, ni, E.d_jpred, nj, ib, eps2, g.tile,
+        HIPCHK(launch_reduce(E.stream, p, g.npart, ni, nj, ib, eps2, g.tile,
                              E.d_xp64, E.dcap, E.d_jindex, r, g.lanes));
         t.stop();
     }
@@ -664,7 +670,12 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     }
     if (covered != ni) return fail(-1, "%s: i-groups cover %d of %d slots", who, covered, ni);
     if (flush_j(nj)) return -1;
-    if (predict(nj, false)) return -1;
+    // A block of one i-group with one softening predicts j inside the force
+    // kernel (the predictor launch and the JPred round trip dominate such
+    // blocks); any other block runs the predictor first.
+    const bool fused = E.opt.fused_small && ngroups == 1 && ni <= 64 &&
+                       choose_geometry(ni, nj).variant == 0;
+    if (!fused && predict(nj, false)) return -1;
     // the pinned i buffer may still be read by the previous upload
     if (wait_queued(E.ibuf_seq) || ensure_i(ni)) return -1;
     const int ld = (ni + 63) & ~63;
@@ -706,7 +717,17 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         Results gr = r;
         gr.acc += 3 * (size_t)o; gr.jerk += 3 * (size_t)o; gr.pot += o;
         gr.nn += o; gr.flags += o;
-        if (enqueue_force(groups[g].n, nj, groups[g].eps2, gb, gr, groups[g].zskip)) return -1;
+        const FusedPredict fp{E.d_jraw, E.ti, E.opt.predict_order, E.d_xp64, E.d_vp64, E.dcap};
+        if (fused) {
+            // xp64 / vp64 now hold this block's prediction, JPred an older one
+            E.pred_valid = false;
+            E.xp_valid = true;
+            E.xp_ti = E.ti;
+            E.xp_nj = nj;
+        }
+        if (enqueue_force(groups[g].n, nj, groups[g].eps2, gb, gr, groups[g].zskip,
+                          fused ? &fp : nullptr))
+            return -1;
     }
     E.res_seq = mark_queued();
     HIPCHK(hipEventRecord(E.ev_done, E.stream));

@@ -228,7 +228,7 @@ int g6x_read_predicted(int, int nj, double *xp, double *vp)
 {
     if (ensure_open("g6x_read_predicted")) return -1;
     if (nj < 0) return fail(-1, "g6x_read_predicted: nj = %d", nj);
-    if (!E.pred_valid || nj > E.pred_nj) return fail(-1, "g6x_read_predicted: no prediction");
+    if (!E.xp_valid || nj > E.xp_nj) return fail(-1, "g6x_read_predicted: no prediction");
     std::vector<double> buf(3 * (size_t)nj);
     for (int pass = 0; pass < 2; ++pass) {
         const double *src = pass ? E.d_vp64 : E.d_xp64;
@@ -269,6 +269,7 @@ int g6x_set_option(const char *key, long long value)
     else if (k == "force_variant") o.force_variant = (int)value;
     else if (k == "lds_min_ni") o.lds_min_ni = (int)value;
     else if (k == "tile") o.tile = (int)value;
+    else if (k == "fused_small") o.fused_small = value != 0 ? 1 : 0;
     else if (k == "wpg") {
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
             return -1;
@@ -299,6 +300,7 @@ long long g6x_get_option(const char *key)
     if (k == "force_variant") return o.force_variant;
     if (k == "lds_min_ni") return o.lds_min_ni;
     if (k == "tile") return o.tile;
+    if (k == "fused_small") return o.fused_small;
     if (k == "last_tile") return E.geom_tile;
     if (k == "reduce_lanes") return o.reduce_lanes;
     if (k == "wpg") return o.wpg;

@@ -87,14 +87,27 @@ hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
 // each on its own j-split of the same 64 i; they fold their sums in LDS, so
 // force_partials() = ceil(nsplit / wpg) partial splits reach the reduction
 // (the LDS kernel always writes nsplit).
+// fused: predict j inside the force kernel (variant 0, one i-group only):
+// the raw j records and the prediction time replace the JPred array, and
+// the FP64 prediction is written to xp64 / vp64 ([3][cap]) as the predictor
+// kernel writes it.
+struct FusedPredict {
+    const JRaw *jraw;
+    double ti;
+    int order;
+    double *xp64, *vp64;
+    int cap;
+};
 int force_igroup(int variant);
 int force_partials(int variant, int nsplit, int wpg);
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
                         int jchunk, int tile, const Partials &p, bool xcd_remap, int variant,
-                        bool zskip, int wpg);
+                        bool zskip, int wpg, const FusedPredict *fused);
+// The reduction's nearest-neighbour rescan reads the FP64 prediction
+// (xp64) and splits it into the double-single words the force kernel used.
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
-                         const JPred *jpred, int nj, const IBlock &ib, float eps2,
+                         int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
                          const Results &r, int lanes);
 // Potential only: pot[i] = -sum_j m_j / r_ij for the i-block (self excluded

@@ -55,18 +55,14 @@ __device__ __forceinline__ void split_ds(double x, float &hi, float &lo)
     lo = (float)(x - (double)hi);
 }
 
-__global__ __launch_bounds__(256) void k_predict(
-    const JRaw *__restrict__ jraw, int nj, double ti, int order,
-    JPred *__restrict__ jpred, double *__restrict__ xp64,
-    double *__restrict__ vp64, int cap)
+// One j-particle predicted to ti (the predictor kernel and the fused
+// small-block force kernel share it, so both round identically).
+__device__ __forceinline__ void predict_one(const JRaw &r, double ti, int order, double xp[3],
+                                            double vp[3])
 {
 #pragma clang fp contract(off)
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nj) return;
-    const JRaw &r = jraw[j];
     const double s = ti - r.tj;
     const double s1 = 1.5 * s, s2 = 2.0 * s;
-    double xp[3], vp[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         double cx = r.j6[k], cv = r.j6[k];
@@ -77,6 +73,18 @@ __global__ __launch_bounds__(256) void k_predict(
         xp[k] = ((cx * s + r.a2[k]) * s + r.v[k]) * s + r.x[k];
         vp[k] = (cv * s1 + r.a2[k]) * s2 + r.v[k];
     }
+}
+
+__global__ __launch_bounds__(256) void k_predict(
+    const JRaw *__restrict__ jraw, int nj, double ti, int order,
+    JPred *__restrict__ jpred, double *__restrict__ xp64,
+    double *__restrict__ vp64, int cap)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    double xp[3], vp[3];
+    predict_one(jraw[j], ti, order, xp, vp);
+    const JRaw &r = jraw[j];
     JPred p;
     split_ds(xp[0], p.xh, p.xl);
     split_ds(xp[1], p.yh, p.yl);
@@ -184,8 +192,14 @@ __device__ __forceinline__ float to_vgpr(float s)
     return v;
 }
 
-template <bool MASK, bool ZSKIP>
-__device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float xih,
+// The ten fields of a predicted j-particle as the fused small-block kernel
+// stages them in LDS (the leading fields of JPred, without its padding).
+struct JRec {
+    float xh, yh, zh, xl, yl, zl, vx, vy, vz, m;
+};
+
+template <bool MASK, bool ZSKIP, class J_T>
+__device__ __forceinline__ void pair(const J_T &J, int jj, unsigned rel, float xih,
                                      float yih, float zih, float xil, float yil,
                                      float zil, float vxi, float vyi, float vzi,
                                      float eps2, Acc32 &a)
@@ -223,8 +237,8 @@ __device__ __forceinline__ void pair(const JPred &J, int jj, unsigned rel, float
     a.best = __builtin_fminf(a.best, rnn);
 }
 
-template <bool MASK, bool ZSKIP>
-__device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
+template <bool MASK, bool ZSKIP, class J_T>
+__device__ __forceinline__ void tile_pairs(const J_T *__restrict__ jp, int tn,
                                            unsigned rel, float xih, float yih,
                                            float zih, float xil, float yil,
                                            float zil, float vxi, float vyi,
@@ -232,7 +246,7 @@ __device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
 {
     int jj = 0;
     for (; jj + 4 <= tn; jj += 4) {
-        const JPred *g = jp + jj;
+        const J_T *g = jp + jj;
         pair<MASK, ZSKIP>(g[0], jj + 0, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
         pair<MASK, ZSKIP>(g[1], jj + 1, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
         pair<MASK, ZSKIP>(g[2], jj + 2, rel, xih, yih, zih, xil, yil, zil, vxi, vyi, vzi, eps2, a);
@@ -251,10 +265,33 @@ __device__ __forceinline__ void tile_pairs(const JPred *__restrict__ jp, int tn,
 constexpr int kMaxWpg = 16;
 size_t force_fold_lds(int wpg) { return wpg > 1 ? (size_t)wpg * 64 * (7 * 8 + 3 * 4) : 0; }
 
-template <bool ZSKIP>
+// Fused small-block mode (one i-group): each wave predicts its own j-split
+// from the raw j records, 64 j at a time (one per lane), writes the FP64
+// prediction (for the reduction's nearest-neighbour rescan, neighbour lists
+// and read-back) and stages the FP32 fields in a wave-private LDS area that
+// the pair loop reads by broadcast.  No JPred array is written or read, and
+// the predictor launch disappears.  The pairs are accumulated in the same
+// order as from JPred, so both modes give bitwise the same partial sums.
+size_t force_fused_lds(int wpg)
+{
+    const size_t stage = (size_t)wpg * 64 * sizeof(JRec);
+    return std::max(stage, force_fold_lds(wpg));   // the fold reuses the staging area
+}
+
+// LDS written by some lanes of a wave and read by others of the same wave:
+// DS instructions of one wave complete in order, so the fences only keep
+// the compiler from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool ZSKIP, bool FUSED>
 __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
     const JPred *__restrict__ jpred, int nj, IBlock ib, int ni, float eps2,
-    int jchunk, int tile, int n_igroups, Partials P, int remap, int wpg)
+    int jchunk, int tile, int n_igroups, Partials P, int remap, int wpg, FusedPredict F)
 {
     extern __shared__ double fold[];
     int b = blockIdx.x;
@@ -295,13 +332,49 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
         a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.kx = a.ky = a.kz = a.p = 0.f;
         a.best = __builtin_inff();
         const unsigned rel = (unsigned)(excl - t);
-        const bool mine = rel < (unsigned)tn;
-        if (__ballot(mine) == 0)
-            tile_pairs<false, ZSKIP>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
-                                     vxi, vyi, vzi, eps2, a);
-        else
-            tile_pairs<true, ZSKIP>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
-                                    vxi, vyi, vzi, eps2, a);
+        if (FUSED) {
+            JRec *stage = reinterpret_cast<JRec *>(fold) + wave * 64;
+            for (int s0 = 0; s0 < tn; s0 += 64) {
+                const int sn = min(64, tn - s0);
+                wave_lds_sync();   // the previous chunk has been read
+                if (lane < sn) {
+                    const int j = t + s0 + lane;
+                    const JRaw &r = F.jraw[j];
+                    double xp[3], vp[3];
+                    predict_one(r, F.ti, F.order, xp, vp);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        F.xp64[(size_t)k * F.cap + j] = xp[k];
+                        F.vp64[(size_t)k * F.cap + j] = vp[k];
+                    }
+                    JRec q;
+                    split_ds(xp[0], q.xh, q.xl);
+                    split_ds(xp[1], q.yh, q.yl);
+                    split_ds(xp[2], q.zh, q.zl);
+                    q.vx = (float)vp[0];
+                    q.vy = (float)vp[1];
+                    q.vz = (float)vp[2];
+                    q.m = (float)r.m;
+                    stage[lane] = q;
+                }
+                wave_lds_sync();
+                const unsigned rs = rel - (unsigned)s0;
+                if (__ballot(rs < (unsigned)sn) == 0)
+                    tile_pairs<false, ZSKIP>(stage, sn, rs, xih, yih, zih, xil, yil, zil, vxi,
+                                             vyi, vzi, eps2, a);
+                else
+                    tile_pairs<true, ZSKIP>(stage, sn, rs, xih, yih, zih, xil, yil, zil, vxi,
+                                            vyi, vzi, eps2, a);
+            }
+        } else {
+            const bool mine = rel < (unsigned)tn;
+            if (__ballot(mine) == 0)
+                tile_pairs<false, ZSKIP>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
+                                         vxi, vyi, vzi, eps2, a);
+            else
+                tile_pairs<true, ZSKIP>(jpred + t, tn, rel, xih, yih, zih, xil, yil, zil,
+                                        vxi, vyi, vzi, eps2, a);
+        }
         A0 += (double)a.ax;
         A1 += (double)a.ay;
         A2 += (double)a.az;
@@ -320,6 +393,9 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
     }
 
     if (wpg > 1) {
+        // in fused mode the fold area is the staging area: every wave must
+        // have left its j loop first
+        if (FUSED) __syncthreads();
         const int stride = wpg * 64;
         const int slot = wave * 64 + lane;
         float *fb = reinterpret_cast<float *>(fold + 7 * stride);
@@ -580,12 +656,15 @@ int force_partials(int variant, int nsplit, int wpg)
 hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         const IBlock &ib, int ni, float eps2, int nsplit,
                         int jchunk, int tile, const Partials &p, bool xcd_remap, int variant,
-                        bool zskip, int wpg)
+                        bool zskip, int wpg, const FusedPredict *fused)
 {
     if (ni <= 0) return hipSuccess;
     if (variant == 1) wpg = 1;
     if (wpg < 1 || wpg > kMaxWpg || (wpg & (wpg - 1))) return hipErrorInvalidValue;
     const int n_igroups = (ni + force_igroup(variant) - 1) / force_igroup(variant);
+    // fused prediction: the scalar-load kernel with one i-group, so every j
+    // is predicted (and its FP64 prediction written) by exactly one lane
+    if (fused && (variant != 0 || n_igroups != 1)) return hipErrorInvalidValue;
     const int nwg = n_igroups * force_partials(variant, nsplit, wpg);
     const int remap = (xcd_remap && nwg % 8 == 0) ? 1 : 0;
     // r^2 >= eps2 > 0 when eps2 is a normal float, so the zero-r^2 rule only
@@ -601,12 +680,23 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
         else
             hipLaunchKernelGGL((k_force_lds<false, false>), dim3(nwg), dim3(kLdsWG), 0, s, jpred,
                                nj, ib, ni, eps2, jchunk, n_igroups, p, remap);
+    } else if (fused) {
+        const FusedPredict F = *fused;
+        const size_t lds = force_fused_lds(wpg);
+        if (zskip)
+            hipLaunchKernelGGL((k_force<true, true>), dim3(nwg), dim3(64 * wpg), lds, s, jpred,
+                               nj, ib, ni, 0.0f, jchunk, tile, n_igroups, p, remap, wpg, F);
+        else
+            hipLaunchKernelGGL((k_force<false, true>), dim3(nwg), dim3(64 * wpg), lds, s, jpred,
+                               nj, ib, ni, eps2, jchunk, tile, n_igroups, p, remap, wpg, F);
     } else if (zskip) {
-        hipLaunchKernelGGL(k_force<true>, dim3(nwg), dim3(64 * wpg), force_fold_lds(wpg), s,
-                           jpred, nj, ib, ni, 0.0f, jchunk, tile, n_igroups, p, remap, wpg);
+        hipLaunchKernelGGL((k_force<true, false>), dim3(nwg), dim3(64 * wpg), force_fold_lds(wpg),
+                           s, jpred, nj, ib, ni, 0.0f, jchunk, tile, n_igroups, p, remap, wpg,
+                           FusedPredict{});
     } else {
-        hipLaunchKernelGGL(k_force<false>, dim3(nwg), dim3(64 * wpg), force_fold_lds(wpg), s,
-                           jpred, nj, ib, ni, eps2, jchunk, tile, n_igroups, p, remap, wpg);
+        hipLaunchKernelGGL((k_force<false, false>), dim3(nwg), dim3(64 * wpg),
+                           force_fold_lds(wpg), s, jpred, nj, ib, ni, eps2, jchunk, tile,
+                           n_igroups, p, remap, wpg, FusedPredict{});
     }
     return hipGetLastError();
 }
@@ -646,7 +736,7 @@ constexpr int kReduceWaves = 4;   // waves per reduction workgroup
 
 template <int LOG2L>
 __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
-    Partials P, int nsplit, int ni, const JPred *__restrict__ jpred, int nj, IBlock ib,
+    Partials P, int nsplit, int ni, int nj, IBlock ib,
     float eps2, int tile_len, const double *__restrict__ xp64, int cap,
     const int *__restrict__ jindex, Results R)
 {
@@ -731,10 +821,15 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
             const int j = j0 + l;
             bool hit = false;
             if (j < tend && j != excl) {
-                const JPred &J = jpred[j];
+                // the double-single words the force kernel used, split again
+                // from the FP64 prediction they were made from
+                float xjh, xjl, yjh, yjl, zjh, zjl;
+                split_ds(xp64[j], xjh, xjl);
+                split_ds(xp64[(size_t)cap + j], yjh, yjl);
+                split_ds(xp64[2 * (size_t)cap + j], zjh, zjl);
                 float dx, dy, dz;
-                const float r2 = ds_r2(J.xh, J.yh, J.zh, J.xl, J.yl, J.zl, xih, yih, zih, xil,
-                                       yil, zil, eps2, dx, dy, dz);
+                const float r2 = ds_r2(xjh, yjh, zjh, xjl, yjl, zjl, xih, yih, zih, xil, yil,
+                                       zil, eps2, dx, dy, dz);
                 hit = __float_as_uint(r2) == bb;
                 m2 = __builtin_amdgcn_fmed3f(m1, m2, r2);
                 m1 = __builtin_fminf(m1, r2);
@@ -791,7 +886,7 @@ int reduce_lanes(int nsplit)
 }
 
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
-                         const JPred *jpred, int nj, const IBlock &ib, float eps2,
+                         int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
                          const Results &r, int lanes)
 {
@@ -801,23 +896,23 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
     const dim3 grid((ni + per_wg - 1) / per_wg), block(64 * kReduceWaves);
     switch (lanes) {
     case 4:
-        hipLaunchKernelGGL(k_reduce<2>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+        hipLaunchKernelGGL(k_reduce<2>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
                            tile, xp64, cap, jindex, r);
         break;
     case 8:
-        hipLaunchKernelGGL(k_reduce<3>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+        hipLaunchKernelGGL(k_reduce<3>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
                            tile, xp64, cap, jindex, r);
         break;
     case 16:
-        hipLaunchKernelGGL(k_reduce<4>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+        hipLaunchKernelGGL(k_reduce<4>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
                            tile, xp64, cap, jindex, r);
         break;
     case 32:
-        hipLaunchKernelGGL(k_reduce<5>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+        hipLaunchKernelGGL(k_reduce<5>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
                            tile, xp64, cap, jindex, r);
         break;
     case 64:
-        hipLaunchKernelGGL(k_reduce<6>, grid, block, 0, s, p, nsplit, ni, jpred, nj, ib, eps2,
+        hipLaunchKernelGGL(k_reduce<6>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
                            tile, xp64, cap, jindex, r);
         break;
     default:

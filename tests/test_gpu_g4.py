@@ -285,3 +285,30 @@ def test_full_size_65536_sample(g):
                                               xp, vp, s["m"])
     _check(_errors(acc[2 * samp], jerk[2 * samp], pot[2 * samp], racc, rjerk, rpot,
                    xp[block[samp]], vp[block[samp]], xp, vp, s["m"], 1e-8 / 2), "N=65536 sample")
+
+
+def test_fused_small_block_equals_the_predictor_path(g):
+    """32 particles in 64 pipes (one softening, one i-group) go through the
+    fused small-block kernel with the zero-distance rule; forces and
+    neighbour lists are bitwise those of the predictor + JPred path."""
+    s = plummer_state(3000, seed=13)
+    _load(g, s)
+    xp, vp = _predict(s, 0.0)
+    g.setti(0.0)
+    pipes = np.repeat(np.arange(0, 3000, 91)[:32], 2)
+    outs = []
+    try:
+        for fused in (1, 0):
+            g.option("fused_small", fused)
+            g.calc_firsthalf(3000, xp[pipes], vp[pipes], np.zeros(64), np.full(64, 0.01))
+            assert g.wait() == 0
+            acc, jerk, pot = g.calc_lasthalf(64)
+            g.nbread()
+            lists = [np.sort(g.nblist(k, 3000)) for k in range(64)]
+            outs.append((acc, jerk, pot, lists))
+    finally:
+        g.option("fused_small", 1)
+    for u, w in zip(outs[0][:3], outs[1][:3]):
+        np.testing.assert_array_equal(u, w)
+    for u, w in zip(outs[0][3], outs[1][3]):
+        np.testing.assert_array_equal(u, w)

@@ -291,6 +291,48 @@ def test_full_size_65536_sample(g):
     ep, _ = oracle.energies(m, xp, vp, 1e-8)
     assert abs(0.5 * (m * pot).sum() - ep) < 1e-7 * abs(ep)
     assert np.all(pot < 0) and np.all((nn >= 0) & (nn < n))
+    _whole_sweep_properties(m, xp, acc, jerk, nn)
+
+
+def _whole_sweep_properties(m, xp, acc, jerk, nn):
+    """Size-independent checks of a full self-sweep: Newton's third law for
+    acc and jerk (sum m a = sum m j = 0 up to the FP32 pair rounding), and
+    every particle's nearest neighbour at the exact FP64 nearest distance
+    (a k-d tree over all positions; distances are compared, so an exact tie
+    may resolve to either index)."""
+    from scipy.spatial import cKDTree
+    mj = (m[:, None] * jerk).sum(0)
+    assert np.abs(mj).max() < 1e-6 * (m[:, None] * np.abs(jerk)).sum(0).max()
+    d, k = cKDTree(xp).query(xp, k=2)
+    d_nn = np.sqrt(((xp[nn] - xp) ** 2).sum(1))
+    assert np.all(nn != np.arange(len(m)))
+    bad = np.flatnonzero(np.abs(d_nn - d[:, 1]) > 1e-12 * d[:, 1])
+    assert len(bad) == 0, f"{len(bad)} nearest neighbours off, e.g. i={bad[:5]}"
+
+
+def test_full_size_262144_sweep_properties(g):
+    """Config 3's j-memory size: Plummer N = 262144 swept in full, checked on
+    a 128-i sample against the oracle and on every particle through the
+    size-independent properties."""
+    from starlab_amd.ics import makeplummer
+    n = 262144
+    m, x, v = makeplummer(n, 262144, potential_fn=lambda m, x: -0.25 * 2.0)
+    rng = np.random.default_rng(1)
+    s = dict(m=m, x=x, v=v, a2=rng.normal(size=(n, 3)) * 0.5, j6=rng.normal(size=(n, 3)),
+             k18=np.zeros((n, 3)), tj=-rng.uniform(0, 2.0 ** -10, n), dtj=np.ones(n),
+             idx=np.arange(n, dtype=np.int32))
+    load_j(g, s)
+    assert g.self_sweep(n, 0, n, 0.0, 0.0) == 0
+    acc, jerk, pot, nn = g.read_results(n)
+    xp, vp = _predict(s, 0.0)
+    sel = rng.choice(n, 128, replace=False)
+    ref = oracle.force(xp[sel], vp[sel], s["idx"][sel], xp, vp, m, s["idx"], 0.0)
+    errs = compare(acc[sel], jerk[sel], pot[sel], nn[sel], ref, xp[sel], vp[sel], m, xp, vp,
+                   0.0, s["idx"][sel])
+    _check(errs, "262144 sample")
+    ma = (m[:, None] * acc).sum(0)
+    assert np.abs(ma).max() < 1e-6 * (m[:, None] * np.abs(acc)).sum(0).max()
+    _whole_sweep_properties(m, xp, acc, jerk, nn)
 
 
 @pytest.mark.parametrize("nsplit", [0, 7, 300])
@@ -427,3 +469,43 @@ def test_autotune_table_and_parity(g, tmp_path):
         assert g.tuning() == tab
     finally:
         g.clear_tuning()
+
+
+@pytest.mark.parametrize("nj", [1000, 65536])
+@pytest.mark.parametrize("eps2", [1e-8, 0.0])
+def test_fused_small_blocks_equal_the_predictor_path(g, nj, eps2):
+    """ABI blocks of one i-group predict j inside the force kernel.  Their
+    acc, jerk, pot, nearest neighbours, neighbour lists and the FP64
+    prediction read back are bitwise those of the predictor + JPred path
+    (same pair order, same partial sums), and they pass the oracle checks."""
+    s = plummer_state(nj, seed=71)
+    load_j(g, s)
+    rng = np.random.default_rng(nj)
+    ti = 0.0
+    xp, vp = _predict(s, ti)
+    try:
+        for ni in (1, 13, 64):
+            isel = np.sort(rng.choice(nj, ni, replace=False))
+            h2 = np.full(ni, 0.05 ** 2)
+            out = {}
+            for fused in (1, 0):
+                g.option("fused_small", fused)
+                g.set_ti(ti)
+                g.calc_firsthalf(nj, s["idx"][isel], xp[isel], vp[isel], eps2, h2)
+                rc, acc, jerk, pot, nn = g.calc_lasthalf2(nj, ni)
+                assert rc == 0, g.last_error()
+                assert g.read_neighbour_list() == 0
+                lists = [g.get_neighbour_list(k, 8192)[1] for k in range(ni)]
+                xr, vr = g.read_predicted(nj)
+                out[fused] = (acc, jerk, pot, nn, xr, vr, lists)
+            for u, w, name in zip(out[1][:6], out[0][:6], ("acc", "jerk", "pot", "nn", "xp", "vp")):
+                np.testing.assert_array_equal(u, w, err_msg=f"ni={ni} {name}")
+            for k, (u, w) in enumerate(zip(out[1][6], out[0][6])):
+                np.testing.assert_array_equal(u, w, err_msg=f"ni={ni} list {k}")
+            np.testing.assert_array_equal(out[1][4], xp)
+            ref = oracle.force(xp[isel], vp[isel], s["idx"][isel], xp, vp, s["m"], s["idx"], eps2)
+            acc, jerk, pot, nn = out[1][:4]
+            _check(compare(acc, jerk, pot, nn, ref, xp[isel], vp[isel], s["m"], xp, vp, eps2,
+                           isel), f"fused nj={nj} ni={ni}")
+    finally:
+        g.option("fused_small", 1)
