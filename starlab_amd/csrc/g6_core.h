@@ -114,6 +114,7 @@ struct Engine {
     // ---- i-block ----------------------------------------------------------
     int icap = 0;               // slots (multiple of 64)
     char *h_ibuf = nullptr;     // pinned, layout of ibuf_bytes()
+    const char *h_ibuf_dev = nullptr;   // its device address (read by k_scatter)
     char *d_ibuf = nullptr;
     int *h_slots = nullptr;     // pinned: i-slots with neighbour lists
 
@@ -128,13 +129,15 @@ struct Engine {
     char *h_res_dev = nullptr;  // its device address
 
     // Pinned buffers used by queued device work: h_stage (read by k_scatter),
-    // h_ibuf (read by the i upload), h_res (written by the reduction).  Each
+    // h_ibuf (read by k_scatter too, or by a copy), h_res (written by the
+    // reduction).  Each
     // such enqueue takes the next op_seq; the buffer remembers it.  synced_seq
     // is the last op_seq known complete (lasthalf's wait on ev_done, which
     // follows done_seq).  Host code that rewrites or frees a buffer waits only
     // if that buffer's op is newer than synced_seq.
     unsigned long long op_seq = 0, synced_seq = 0, done_seq = 0;
     unsigned long long stage_seq = 0, ibuf_seq = 0, res_seq = 0;
+    int deferred_scatter = 0;   // records packed in h_stage, not yet launched
 
     // ---- neighbour lists --------------------------------------------------
     size_t nbcap_entries = 0;
@@ -191,7 +194,10 @@ int set_j(int address, int index, double tj, double dtj, double mass,
           const double *k18, const double *j6, const double *a2,
           const double *v, const double *x);
 int excl_address(int index);
-int flush_j(int nj);
+// defer: leave a packed scatter in h_stage for calc_begin to launch together
+// with the i-block upload (deferred_scatter records); any other flush_j
+// launches a deferred scatter first.
+int flush_j(int nj, bool defer = false);
 int predict(int nj, bool force);
 
 // i-block and results (host or device copies share one layout)

@@ -178,9 +178,11 @@ int ensure_device_j(int need)
     HIPCHK(hipMemsetAsync(E.d_xp64, 0, sizeof(double) * 3 * (size_t)cap, E.stream));
     HIPCHK(hipMemsetAsync(E.d_vp64, 0, sizeof(double) * 3 * (size_t)cap, E.stream));
     E.dcap = cap;
-    // everything set so far must reach the new arrays
+    // everything set so far must reach the new arrays (a deferred scatter's
+    // records are in the mirror too)
     for (int a : E.dirty) E.dirty_flag[a] = 0;
     E.dirty.clear();
+    E.deferred_scatter = 0;
     const int n = std::min(E.n_addr, E.hcap);
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(E.d_jraw, E.h_jraw, sizeof(JRaw) * (size_t)n,
@@ -223,8 +225,24 @@ int ensure_stage(int n)
 
 // Push dirty j records to HBM: the whole used range when many changed,
 // otherwise a packed list scattered by a kernel.
-int flush_j(int nj)
+// Launch a scatter that calc_begin packed but did not get to launch.
+int launch_deferred_scatter()
 {
+    const int nd = E.deferred_scatter;
+    if (nd <= 0 || E.dcap == 0) return 0;
+    E.deferred_scatter = 0;
+    KTimer t(K_SCATTER);
+    HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
+                          reinterpret_cast<const JRaw *>(E.h_stage_dev), nd));
+    t.stop();
+    E.stage_seq = mark_queued();
+    E.pred_valid = false;
+    return 0;
+}
+
+int flush_j(int nj, bool defer)
+{
+    if (launch_deferred_scatter()) return -1;
     if (nj > E.n_addr) {
         // addresses never set are zero-mass particles at the origin
         if (grow_host_mirror(nj)) return -1;
@@ -251,11 +269,15 @@ int flush_j(int nj)
             addr[k] = a;
             addr[nd + k] = E.h_jindex[a];
         }
-        KTimer t(K_SCATTER);
-        HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
-                              reinterpret_cast<const JRaw *>(E.h_stage_dev), nd));
-        t.stop();
-        E.stage_seq = mark_queued();
+        if (defer) {
+            E.deferred_scatter = nd;
+        } else {
+            KTimer t(K_SCATTER);
+            HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
+                                  reinterpret_cast<const JRaw *>(E.h_stage_dev), nd));
+            t.stop();
+            E.stage_seq = mark_queued();
+        }
     }
     for (int a : E.dirty) E.dirty_flag[a] = 0;
     E.dirty.clear();
@@ -297,8 +319,14 @@ int ensure_i(int ni)
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots);
     hfree(E.h_res); dfree(E.d_res);
     E.h_res_dev = nullptr;
+    E.h_ibuf_dev = nullptr;
     E.icap = 0;
     HIPCHK(hipHostMalloc((void **)&E.h_ibuf, ibuf_bytes(cap), hipHostMallocDefault));
+    {
+        void *dev = nullptr;
+        HIPCHK(hipHostGetDevicePointer(&dev, E.h_ibuf, 0));
+        E.h_ibuf_dev = static_cast<const char *>(dev);
+    }
     HIPCHK(hipMalloc((void **)&E.d_ibuf, ibuf_bytes(cap)));
     HIPCHK(hipHostMalloc((void **)&E.h_slots, sizeof(int) * (size_t)cap, hipHostMallocDefault));
     HIPCHK(hipHostGetDevicePointer((void **)&E.h_slots_dev, E.h_slots, 0));
@@ -556,6 +584,8 @@ int do_close()
     hfree(E.h_ibuf); dfree(E.d_ibuf); hfree(E.h_slots);
     hfree(E.h_res); dfree(E.d_res);
     E.h_res_dev = nullptr;
+    E.h_ibuf_dev = nullptr;
+    E.deferred_scatter = 0;   // the next open uploads the whole mirror
     E.synced_seq = E.op_seq;
     E.icap = 0;
     dfree(E.d_part); E.pbytes = 0;
@@ -669,13 +699,13 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         covered += groups[g].n;
     }
     if (covered != ni) return fail(-1, "%s: i-groups cover %d of %d slots", who, covered, ni);
-    if (flush_j(nj)) return -1;
+    // the j scatter is launched below, together with the i-block upload
+    if (flush_j(nj, true)) return -1;
     // A block of one i-group with one softening predicts j inside the force
     // kernel (the predictor launch and the JPred round trip dominate such
     // blocks); any other block runs the predictor first.
     const bool fused = E.opt.fused_small && ngroups == 1 && ni <= 64 &&
                        choose_geometry(ni, nj).variant == 0;
-    if (!fused && predict(nj, false)) return -1;
     // the pinned i buffer may still be read by the previous upload
     if (wait_queued(E.ibuf_seq) || ensure_i(ni)) return -1;
     const int ld = (ni + 63) & ~63;
@@ -701,8 +731,19 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         hh[k] = (k < ni && h2) ? h2[k] : 0.0;
         ee[k] = (k < ni && e2) ? e2[k] : 0.0;
     }
-    HIPCHK(hipMemcpyAsync(E.d_ibuf, E.h_ibuf, ibuf_bytes(ld), hipMemcpyHostToDevice, E.stream));
-    E.ibuf_seq = mark_queued();
+    {
+        // one launch: the deferred j scatter and the i-block upload
+        const int nd = E.deferred_scatter;
+        E.deferred_scatter = 0;
+        KTimer t(K_SCATTER);
+        HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
+                              reinterpret_cast<const JRaw *>(E.h_stage_dev), nd, E.h_ibuf_dev,
+                              E.d_ibuf, ibuf_bytes(ld)));
+        t.stop();
+        E.ibuf_seq = mark_queued();
+        if (nd > 0) E.stage_seq = E.ibuf_seq;
+    }
+    if (!fused && predict(nj, false)) return -1;
     const IBlock ib = iblock_at(E.d_ibuf, ld);
     // the reduction writes the results straight into pinned host memory
     const Results r = results_at(E.h_res_dev, ni);

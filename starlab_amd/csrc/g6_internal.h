@@ -72,7 +72,10 @@ constexpr float kNNBand = 1.9073486328125e-06f;   // 2^-19
 
 // Kernel launchers (g6_kernels.hip).  All enqueue on `s` and return the
 // launch status.
-hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n);
+// Scatter n staged j records; with ibytes > 0 (a multiple of 16) the same
+// launch copies ibytes from isrc (pinned host memory) to idst (HBM).
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n,
+                          const void *isrc = nullptr, void *idst = nullptr, size_t ibytes = 0);
 hipError_t launch_predict(hipStream_t s, const JRaw *jraw, int nj, double ti,
                           int order, JPred *jpred, double *xp64, double *vp64,
                           int cap);

@@ -23,10 +23,14 @@ namespace g6 {
 // j-memory update: copy staged records to their addresses.
 // ---------------------------------------------------------------------------
 // The staged buffer holds n records, then n addresses, then n index fields.
+// The same launch can copy an i-block of n16 16-byte words from pinned host
+// memory to HBM (one launch per block instead of a kernel and a copy).
 __global__ void k_scatter(JRaw *__restrict__ jraw, int *__restrict__ jindex,
-                          const JRaw *__restrict__ staged, int n)
+                          const JRaw *__restrict__ staged, int n,
+                          const uint4 *__restrict__ isrc, uint4 *__restrict__ idst, int n16)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n16) idst[k] = isrc[k];
     if (k >= n) return;
     const int *addr = reinterpret_cast<const int *>(staged + n);
     const int a = addr[k];
@@ -34,11 +38,16 @@ __global__ void k_scatter(JRaw *__restrict__ jraw, int *__restrict__ jindex,
     jindex[a] = addr[n + k];
 }
 
-hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n)
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n,
+                          const void *isrc, void *idst, size_t ibytes)
 {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s,
-                       jraw, jindex, staged, n);
+    if (ibytes % 16 || ibytes / 16 > (size_t)INT_MAX) return hipErrorInvalidValue;
+    const int n16 = (int)(ibytes / 16);
+    const int total = n > n16 ? n : n16;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3((total + 255) / 256), dim3(256), 0, s, jraw, jindex,
+                       staged, n, static_cast<const uint4 *>(isrc), static_cast<uint4 *>(idst),
+                       n16);
     return hipGetLastError();
 }
 
