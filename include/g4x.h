@@ -22,7 +22,7 @@ const char *g4x_last_error(void);
 /* Options: "npipes", "jpmax", "nb_cap" (list entries stored per pipe before
  * the store grows), "exit_on_error", "dedupe" (compute identical adjacent
  * pipes once), "timing", "fused_small" (1, the default: blocks of at most
- * 64 distinct i with one softening predict j inside the force kernel, with
+ * 256 distinct i with one softening predict j inside the force kernel, with
  * the same results); read-only "last_slots" (distinct i-particles of
  * the last block) and "last_groups" (softening groups).  set returns 0 or
  * -1 for an unknown key; get returns -1 for an unknown key. */

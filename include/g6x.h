@@ -84,7 +84,7 @@ int g6x_sync(int cluster_id);
  *   "wpg"            waves per force workgroup, each on its own j-split, folded
  *                    in LDS before the reduction: 0 auto, 1, 2, 4, 8, 16  G6_WPG
  *   "last_wpg", "last_lanes"  (read only) of the last force launch
- *   "fused_small"    1 (default) = ABI blocks of at most 64 i with one
+ *   "fused_small"    1 (default) = ABI blocks of at most 256 i with one
  *                    softening predict j inside the force kernel (no
  *                    predictor launch, no JPred round trip; bitwise the
  *                    same results); 0 = predictor first   G6_FUSED_SMALL

@@ -701,10 +701,10 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     if (covered != ni) return fail(-1, "%s: i-groups cover %d of %d slots", who, covered, ni);
     // the j scatter is launched below, together with the i-block upload
     if (flush_j(nj, true)) return -1;
-    // A block of one i-group with one softening predicts j inside the force
-    // kernel (the predictor launch and the JPred round trip dominate such
-    // blocks); any other block runs the predictor first.
-    const bool fused = E.opt.fused_small && ngroups == 1 && ni <= 64 &&
+    // A block of at most kFusedMaxGroups i-groups with one softening predicts
+    // j inside the force kernel (the predictor launch and the JPred round
+    // trip dominate such blocks); any other block runs the predictor first.
+    const bool fused = E.opt.fused_small && ngroups == 1 && ni <= 64 * kFusedMaxGroups &&
                        choose_geometry(ni, nj).variant == 0;
     // the pinned i buffer may still be read by the previous upload
     if (wait_queued(E.ibuf_seq) || ensure_i(ni)) return -1;

@@ -90,10 +90,12 @@ hipError_t launch_iprep_self(hipStream_t s, const JPred *jpred,
 // each on its own j-split of the same 64 i; they fold their sums in LDS, so
 // force_partials() = ceil(nsplit / wpg) partial splits reach the reduction
 // (the LDS kernel always writes nsplit).
-// fused: predict j inside the force kernel (variant 0, one i-group only):
+// fused: predict j inside the force kernel (variant 0, at most
+// kFusedMaxGroups i-groups of 64):
 // the raw j records and the prediction time replace the JPred array, and
 // the FP64 prediction is written to xp64 / vp64 ([3][cap]) as the predictor
 // kernel writes it.
+constexpr int kFusedMaxGroups = 4;
 struct FusedPredict {
     const JRaw *jraw;
     double ti;

@@ -274,13 +274,15 @@ __device__ __forceinline__ void tile_pairs(const J_T *__restrict__ jp, int tn,
 constexpr int kMaxWpg = 16;
 size_t force_fold_lds(int wpg) { return wpg > 1 ? (size_t)wpg * 64 * (7 * 8 + 3 * 4) : 0; }
 
-// Fused small-block mode (one i-group): each wave predicts its own j-split
-// from the raw j records, 64 j at a time (one per lane), writes the FP64
+// Fused small-block mode (at most kFusedMaxGroups i-groups): each wave
+// predicts its own j-split from the raw j records, 64 j at a time (one per
+// lane), and stages the FP32 fields in a wave-private LDS area that the pair
+// loop reads by broadcast.  The waves of i-group 0 also write the FP64
 // prediction (for the reduction's nearest-neighbour rescan, neighbour lists
-// and read-back) and stages the FP32 fields in a wave-private LDS area that
-// the pair loop reads by broadcast.  No JPred array is written or read, and
-// the predictor launch disappears.  The pairs are accumulated in the same
-// order as from JPred, so both modes give bitwise the same partial sums.
+// and read-back); the other i-groups predict the same j again, from cache.
+// No JPred array is written or read, and the predictor launch disappears.
+// The pairs are accumulated in the same order as from JPred, so both modes
+// give bitwise the same partial sums.
 size_t force_fused_lds(int wpg)
 {
     const size_t stage = (size_t)wpg * 64 * sizeof(JRec);
@@ -351,10 +353,12 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
                     const JRaw &r = F.jraw[j];
                     double xp[3], vp[3];
                     predict_one(r, F.ti, F.order, xp, vp);
+                    if (ig == 0) {
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        F.xp64[(size_t)k * F.cap + j] = xp[k];
-                        F.vp64[(size_t)k * F.cap + j] = vp[k];
+                        for (int k = 0; k < 3; ++k) {
+                            F.xp64[(size_t)k * F.cap + j] = xp[k];
+                            F.vp64[(size_t)k * F.cap + j] = vp[k];
+                        }
                     }
                     JRec q;
                     split_ds(xp[0], q.xh, q.xl);
@@ -671,9 +675,9 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
     if (variant == 1) wpg = 1;
     if (wpg < 1 || wpg > kMaxWpg || (wpg & (wpg - 1))) return hipErrorInvalidValue;
     const int n_igroups = (ni + force_igroup(variant) - 1) / force_igroup(variant);
-    // fused prediction: the scalar-load kernel with one i-group, so every j
-    // is predicted (and its FP64 prediction written) by exactly one lane
-    if (fused && (variant != 0 || n_igroups != 1)) return hipErrorInvalidValue;
+    // fused prediction: the scalar-load kernel with few i-groups; the FP64
+    // prediction of each j is written by exactly one lane (i-group 0)
+    if (fused && (variant != 0 || n_igroups > kFusedMaxGroups)) return hipErrorInvalidValue;
     const int nwg = n_igroups * force_partials(variant, nsplit, wpg);
     const int remap = (xcd_remap && nwg % 8 == 0) ? 1 : 0;
     // r^2 >= eps2 > 0 when eps2 is a normal float, so the zero-r^2 rule only

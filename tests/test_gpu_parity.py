@@ -474,7 +474,8 @@ def test_autotune_table_and_parity(g, tmp_path):
 @pytest.mark.parametrize("nj", [1000, 65536])
 @pytest.mark.parametrize("eps2", [1e-8, 0.0])
 def test_fused_small_blocks_equal_the_predictor_path(g, nj, eps2):
-    """ABI blocks of one i-group predict j inside the force kernel.  Their
+    """ABI blocks of up to four i-groups predict j inside the force kernel
+    (257 i is the first size that does not).  Their
     acc, jerk, pot, nearest neighbours, neighbour lists and the FP64
     prediction read back are bitwise those of the predictor + JPred path
     (same pair order, same partial sums), and they pass the oracle checks."""
@@ -484,7 +485,7 @@ def test_fused_small_blocks_equal_the_predictor_path(g, nj, eps2):
     ti = 0.0
     xp, vp = _predict(s, ti)
     try:
-        for ni in (1, 13, 64):
+        for ni in (1, 13, 64, 65, 200, 256, 257):
             isel = np.sort(rng.choice(nj, ni, replace=False))
             h2 = np.full(ni, 0.05 ** 2)
             out = {}
