@@ -193,16 +193,26 @@ int firsthalf(int nj, int ni, const double (*xi)[3], const double (*vi)[3],
     std::vector<double> values;
     std::vector<int> group_of(nu);
     std::unordered_map<uint64_t, int> group_of_bits;
+    // kira passes one softening for every pipe: a run of equal values needs
+    // no hash lookup (same groups, same order)
+    uint64_t last_bits = 0;
+    int last_group = -1;
     for (int u = 0; u < nu; ++u) {
         const double e = eps2[first_pipe[u]];
         uint64_t bits;
         memcpy(&bits, &e, sizeof bits);
+        if (last_group >= 0 && bits == last_bits) {
+            group_of[u] = last_group;
+            continue;
+        }
         auto it = group_of_bits.find(bits);
         if (it == group_of_bits.end()) {
             it = group_of_bits.emplace(bits, (int)values.size()).first;
             values.push_back(e);
         }
         group_of[u] = it->second;
+        last_bits = bits;
+        last_group = it->second;
     }
     std::vector<int> count(values.size(), 0), start(values.size(), 0);
     for (int u = 0; u < nu; ++u) ++count[group_of[u]];
