@@ -510,3 +510,30 @@ def test_fused_small_blocks_equal_the_predictor_path(g, nj, eps2):
                            isel), f"fused nj={nj} ni={ni}")
     finally:
         g.option("fused_small", 1)
+
+
+def test_large_abi_block_equals_self_sweep(g):
+    """One ABI block of 20000 i (npipes raised; the scatter launch reads the
+    1.6 MB i-block from pinned memory) gives bitwise the results of the
+    device-resident sweep over the same particles: the host and device
+    predictions are bitwise equal, so both build the same double-single
+    i-block and run the same force geometry."""
+    n = 20000
+    s = plummer_state(n, seed=73)
+    load_j(g, s)
+    xp, vp = _predict(s, 0.0)
+    old = g.npipes()
+    try:
+        g.option("npipes", n)
+        g.set_ti(0.0)
+        acc, jerk, pot, nn = g.force(n, s["idx"], xp, vp, 1e-8)
+        assert g.self_sweep(n, 0, n, 0.0, 1e-8) == 0
+        a2, j2, p2, n2 = g.read_results(n)
+    finally:
+        g.option("npipes", old)
+    for u, w, name in ((acc, a2, "acc"), (jerk, j2, "jerk"), (pot, p2, "pot"), (nn, n2, "nn")):
+        np.testing.assert_array_equal(u, w, err_msg=name)
+    sel = np.random.default_rng(5).choice(n, 128, replace=False)
+    ref = oracle.force(xp[sel], vp[sel], s["idx"][sel], xp, vp, s["m"], s["idx"], 1e-8)
+    _check(compare(acc[sel], jerk[sel], pot[sel], nn[sel], ref, xp[sel], vp[sel], s["m"], xp, vp,
+                   1e-8, sel), "ABI block of 20000")
