@@ -168,10 +168,11 @@ __device__ __forceinline__ void ens_pair(const JPred &J, int jj, unsigned rel, c
     const float dx = (J.xh - I.xh) + (J.xl - I.xl);
     const float dy = (J.yh - I.yh) + (J.yl - I.yl);
     const float dz = (J.zh - I.zh) + (J.zl - I.zl);
-    float r2 = dz * dz;
+    // ((eps2 + dz*dz) + dy*dy) + dx*dx in three fused steps, as libg6's
+    // force kernel evaluates it
+    float r2 = __builtin_fmaf(dz, dz, eps2);
     r2 = __builtin_fmaf(dy, dy, r2);
     r2 = __builtin_fmaf(dx, dx, r2);
-    r2 = r2 + eps2;
     const float dvx = J.vx - I.vx, dvy = J.vy - I.vy, dvz = J.vz - I.vz;
     float rv = dz * dvz;
     rv = __builtin_fmaf(dy, dvy, rv);
