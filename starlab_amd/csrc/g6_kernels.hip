@@ -1,6 +1,8 @@
-// g6_kernels.hip -- gfx950 kernels of libg6: j scatter, j predictor, direct-sum
-// force (acc, jerk, pot, nearest neighbour), split reduction, FP64
-// nearest-neighbour refinement and FP64 neighbour lists.
+// g6_kernels.hip -- gfx950 kernels of libg6: j scatter (with the i-block
+// upload), j predictor, direct-sum force (acc, jerk, pot, nearest neighbour;
+// small blocks predict j inside it), split reduction with the FP64
+// nearest-neighbour refinement, potential-only and k-nearest-neighbour
+// sweeps, and FP64 neighbour lists.
 //
 // Force pair law: accumulate_acc_and_jerk, src/node/dyn/hdyn/evolve/hdyn_ev.C:2019-2036
 //   d = x_j - x_i, w = v_j - v_i, r2 = d.d + eps2
@@ -10,9 +12,9 @@
 // equals index[i] (hdyn_grape6.C:3462-3468).
 //
 // Precision: positions double-single (hi + lo FP32), pair arithmetic and
-// rsqrt FP32, 256-pair tile partial sums in FP32 folded into FP64
-// accumulators, FP64 across j-splits.  FP64 is also used by the predictor
-// and by the exact nearest-neighbour / neighbour-list checks.
+// rsqrt FP32, partial sums over j-tiles of 16 to 256 pairs in FP32 folded
+// into FP64 accumulators, FP64 across j-splits.  FP64 is also used by the
+// predictor and by the exact nearest-neighbour / neighbour-list checks.
 #include "g6_internal.h"
 
 #include <climits>
