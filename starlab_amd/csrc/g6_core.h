@@ -83,6 +83,7 @@ struct Engine {
     // ---- j-memory ---------------------------------------------------------
     int hcap = 0;               // host mirror capacity (addresses)
     JRaw *h_jraw = nullptr;     // pinned
+    const JRaw *h_jraw_dev = nullptr;   // its device address (read by k_scatter)
     int *h_jindex = nullptr;    // pinned
     int n_addr = 0;             // 1 + highest address ever set
     std::vector<int> dirty;
@@ -128,16 +129,17 @@ struct Engine {
     char *h_res = nullptr;      // pinned; the ABI path's reduction writes it directly
     char *h_res_dev = nullptr;  // its device address
 
-    // Pinned buffers used by queued device work: h_stage (read by k_scatter),
-    // h_ibuf (read by k_scatter too, or by a copy), h_res (written by the
-    // reduction).  Each
-    // such enqueue takes the next op_seq; the buffer remembers it.  synced_seq
-    // is the last op_seq known complete (lasthalf's wait on ev_done, which
-    // follows done_seq).  Host code that rewrites or frees a buffer waits only
-    // if that buffer's op is newer than synced_seq.
+    // Pinned buffers used by queued device work: h_stage (the scatter list),
+    // the mirror h_jraw / h_jindex (records read by k_scatter or a full
+    // upload), h_ibuf (read by k_scatter), h_res (written by the reduction).
+    // Each such enqueue takes the next op_seq; the buffer remembers it.
+    // synced_seq is the last op_seq known complete (lasthalf's wait on
+    // ev_done, which follows done_seq).  Host code that rewrites or frees a
+    // buffer waits only if that buffer's op is newer than synced_seq.
     unsigned long long op_seq = 0, synced_seq = 0, done_seq = 0;
     unsigned long long stage_seq = 0, ibuf_seq = 0, res_seq = 0;
-    int deferred_scatter = 0;   // records packed in h_stage, not yet launched
+    int deferred_scatter = 0;   // records listed in h_stage, not yet launched
+    unsigned long long mirror_seq = 0;   // last queued op that reads h_jraw / h_jindex
 
     // ---- neighbour lists --------------------------------------------------
     size_t nbcap_entries = 0;

@@ -130,9 +130,14 @@ void harvest_timing()
 template <class T> void dfree(T *&p) { if (p) { (void)hipFree(p); p = nullptr; } }
 template <class T> void hfree(T *&p) { if (p) { (void)hipHostFree(p); p = nullptr; } }
 
+int wait_queued(unsigned long long seq);
+unsigned long long mark_queued();
+
 int grow_host_mirror(int need)
 {
     if (need <= E.hcap) return 0;
+    // a queued scatter or upload may still read the old mirror
+    if (wait_queued(E.mirror_seq)) return -1;
     int cap = std::max(need, std::max(1024, E.hcap * 2));
     JRaw *nr = nullptr;
     int *ni = nullptr;
@@ -148,8 +153,12 @@ int grow_host_mirror(int need)
     hfree(E.h_jindex);
     E.h_jraw = nr;
     E.h_jindex = ni;
+    E.h_jraw_dev = nullptr;
     E.hcap = cap;
     E.dirty_flag.resize(cap, 0);
+    void *dev = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&dev, nr, 0));
+    E.h_jraw_dev = static_cast<const JRaw *>(dev);
     return 0;
 }
 
@@ -189,6 +198,7 @@ int ensure_device_j(int need)
                               hipMemcpyHostToDevice, E.stream));
         HIPCHK(hipMemcpyAsync(E.d_jindex, E.h_jindex, sizeof(int) * (size_t)n,
                               hipMemcpyHostToDevice, E.stream));
+        E.mirror_seq = mark_queued();
     }
     return 0;
 }
@@ -214,7 +224,7 @@ int ensure_stage(int n)
     hfree(E.h_stage);
     E.h_stage_dev = nullptr;
     E.scap = 0;
-    const size_t bytes = (sizeof(JRaw) + 2 * sizeof(int)) * (size_t)cap;
+    const size_t bytes = 2 * sizeof(int) * (size_t)cap;   // addresses, then index fields
     HIPCHK(hipHostMalloc((void **)&E.h_stage, bytes, hipHostMallocDefault));
     void *dev = nullptr;
     HIPCHK(hipHostGetDevicePointer(&dev, E.h_stage, 0));
@@ -232,10 +242,10 @@ int launch_deferred_scatter()
     if (nd <= 0 || E.dcap == 0) return 0;
     E.deferred_scatter = 0;
     KTimer t(K_SCATTER);
-    HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
-                          reinterpret_cast<const JRaw *>(E.h_stage_dev), nd));
+    HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex, E.h_jraw_dev,
+                          reinterpret_cast<const int *>(E.h_stage_dev), nd));
     t.stop();
-    E.stage_seq = mark_queued();
+    E.stage_seq = E.mirror_seq = mark_queued();
     E.pred_valid = false;
     return 0;
 }
@@ -256,27 +266,26 @@ int flush_j(int nj, bool defer)
                               hipMemcpyHostToDevice, E.stream));
         HIPCHK(hipMemcpyAsync(E.d_jindex, E.h_jindex, sizeof(int) * (size_t)E.n_addr,
                               hipMemcpyHostToDevice, E.stream));
+        E.mirror_seq = mark_queued();
     } else {
         // the staging buffer may still be read by an earlier scatter
         if (wait_queued(E.stage_seq) || ensure_stage(nd)) return -1;
-        // one packed list, read by k_scatter straight from pinned memory:
-        // records, then addresses, then index fields
-        JRaw *rec = reinterpret_cast<JRaw *>(E.h_stage);
-        int *addr = reinterpret_cast<int *>(rec + nd);
+        // the list k_scatter reads from pinned memory: addresses, then their
+        // index fields; the records themselves it reads from the mirror
+        int *list = reinterpret_cast<int *>(E.h_stage);
         for (int k = 0; k < nd; ++k) {
             const int a = E.dirty[k];
-            rec[k] = E.h_jraw[a];
-            addr[k] = a;
-            addr[nd + k] = E.h_jindex[a];
+            list[k] = a;
+            list[nd + k] = E.h_jindex[a];
         }
         if (defer) {
             E.deferred_scatter = nd;
         } else {
             KTimer t(K_SCATTER);
-            HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
-                                  reinterpret_cast<const JRaw *>(E.h_stage_dev), nd));
+            HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex, E.h_jraw_dev,
+                                  reinterpret_cast<const int *>(E.h_stage_dev), nd));
             t.stop();
-            E.stage_seq = mark_queued();
+            E.stage_seq = E.mirror_seq = mark_queued();
         }
     }
     for (int a : E.dirty) E.dirty_flag[a] = 0;
@@ -617,6 +626,10 @@ int set_j(int address, int index, double tj, double dtj, double mass,
           const double *v, const double *x)
 {
     if (address < 0) return fail(-1, "g6_set_j_particle: negative address %d", address);
+    // a queued scatter reads records from the mirror: a block in flight keeps
+    // the j state it was started with (no wait in kira's call order, where
+    // lasthalf has already synchronised)
+    if (wait_queued(E.mirror_seq)) return -1;
     if (grow_host_mirror(address + 1)) return -1;
     JRaw &r = E.h_jraw[address];
     for (int k = 0; k < 3; ++k) {
@@ -736,12 +749,12 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         const int nd = E.deferred_scatter;
         E.deferred_scatter = 0;
         KTimer t(K_SCATTER);
-        HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex,
-                              reinterpret_cast<const JRaw *>(E.h_stage_dev), nd, E.h_ibuf_dev,
+        HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex, E.h_jraw_dev,
+                              reinterpret_cast<const int *>(E.h_stage_dev), nd, E.h_ibuf_dev,
                               E.d_ibuf, ibuf_bytes(ld)));
         t.stop();
         E.ibuf_seq = mark_queued();
-        if (nd > 0) E.stage_seq = E.ibuf_seq;
+        if (nd > 0) E.stage_seq = E.mirror_seq = E.ibuf_seq;
     }
     if (!fused && predict(nj, false)) return -1;
     const IBlock ib = iblock_at(E.d_ibuf, ld);

@@ -72,10 +72,13 @@ constexpr float kNNBand = 1.9073486328125e-06f;   // 2^-19
 
 // Kernel launchers (g6_kernels.hip).  All enqueue on `s` and return the
 // launch status.
-// Scatter n staged j records; with ibytes > 0 (a multiple of 16) the same
-// launch copies ibytes from isrc (pinned host memory) to idst (HBM).
-hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n,
-                          const void *isrc = nullptr, void *idst = nullptr, size_t ibytes = 0);
+// Scatter n j records from the pinned host mirror (device address of
+// mirror) to HBM: list = n addresses, then n index fields (pinned too).
+// With ibytes > 0 (a multiple of 16) the same launch copies ibytes from isrc
+// (pinned host memory) to idst (HBM).
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *mirror,
+                          const int *list, int n, const void *isrc = nullptr,
+                          void *idst = nullptr, size_t ibytes = 0);
 hipError_t launch_predict(hipStream_t s, const JRaw *jraw, int nj, double ti,
                           int order, JPred *jpred, double *xp64, double *vp64,
                           int cap);

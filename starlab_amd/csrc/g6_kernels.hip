@@ -24,32 +24,32 @@ namespace g6 {
 // ---------------------------------------------------------------------------
 // j-memory update: copy staged records to their addresses.
 // ---------------------------------------------------------------------------
-// The staged buffer holds n records, then n addresses, then n index fields.
+// The list holds n addresses, then their n index fields; each record is
+// read from the pinned host mirror (mirror[address]) straight into HBM.
 // The same launch can copy an i-block of n16 16-byte words from pinned host
 // memory to HBM (one launch per block instead of a kernel and a copy).
 __global__ void k_scatter(JRaw *__restrict__ jraw, int *__restrict__ jindex,
-                          const JRaw *__restrict__ staged, int n,
+                          const JRaw *__restrict__ mirror, const int *__restrict__ list, int n,
                           const uint4 *__restrict__ isrc, uint4 *__restrict__ idst, int n16)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n16) idst[k] = isrc[k];
     if (k >= n) return;
-    const int *addr = reinterpret_cast<const int *>(staged + n);
-    const int a = addr[k];
-    jraw[a] = staged[k];
-    jindex[a] = addr[n + k];
+    const int a = list[k];
+    jraw[a] = mirror[a];
+    jindex[a] = list[n + k];
 }
 
-hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *staged, int n,
-                          const void *isrc, void *idst, size_t ibytes)
+hipError_t launch_scatter(hipStream_t s, JRaw *jraw, int *jindex, const JRaw *mirror,
+                          const int *list, int n, const void *isrc, void *idst, size_t ibytes)
 {
     if (ibytes % 16 || ibytes / 16 > (size_t)INT_MAX) return hipErrorInvalidValue;
     const int n16 = (int)(ibytes / 16);
     const int total = n > n16 ? n : n16;
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter, dim3((total + 255) / 256), dim3(256), 0, s, jraw, jindex,
-                       staged, n, static_cast<const uint4 *>(isrc), static_cast<uint4 *>(idst),
-                       n16);
+                       mirror, list, n, static_cast<const uint4 *>(isrc),
+                       static_cast<uint4 *>(idst), n16);
     return hipGetLastError();
 }
 

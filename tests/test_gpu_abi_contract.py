@@ -218,3 +218,37 @@ def test_extension_argument_errors(g):
     assert g.self_sweep(600, 0, 601, 0.0, 1e-8) != 0      # range outside j-memory
     with pytest.raises(RuntimeError, match="self_sweep"):
         g.read_results(600)
+
+
+@pytest.mark.parametrize("pre_dirty", [100, 1500], ids=["scatter", "full_upload"])
+def test_set_j_between_halves_keeps_the_block_in_flight(g, pre_dirty):
+    """j records sent between g6calc_firsthalf_ and g6calc_lasthalf2_ do not
+    reach the block already started (its flush reads the records
+    asynchronously, by a scatter or by a full upload); the next block sees
+    them."""
+    n, ni = 3000, 32
+    s = plummer_state(n, seed=77)
+    load_j(g, s)
+
+    def send(a, x):
+        assert g.set_j_particle(a, int(s["idx"][a]), s["tj"][a], s["dtj"][a], s["m"][a],
+                                s["k18"][a], s["j6"][a], s["a2"][a], s["v"][a], x) == 0
+
+    for a in range(pre_dirty):            # dirty records for this block's flush
+        send(a, s["x"][a])
+    g.set_ti(0.0)
+    xp, vp = oracle.predict(s["x"], s["v"], s["a2"], s["j6"], s["k18"], s["tj"], 0.0, 4)
+    isel = np.arange(ni)
+    g.calc_firsthalf(n, s["idx"][isel], xp[isel], vp[isel], 1e-8, np.zeros(ni))
+    moved = np.arange(50, 250)
+    for a in moved:
+        send(a, s["x"][a] + 5.0)
+    rc, acc, jerk, pot, nn = g.calc_lasthalf2(n, ni)
+    assert rc == 0, g.last_error()
+    ref = oracle.force(xp[isel], vp[isel], s["idx"][isel], xp, vp, s["m"], s["idx"], 1e-8)
+    errs = compare(acc, jerk, pot, nn, ref, xp[isel], vp[isel], s["m"], xp, vp, 1e-8, isel)
+    assert errs["nn_mismatch"] == 0 and errs["acc_norm"] < 1e-6, errs
+    s2 = dict(s, x=s["x"].copy())
+    s2["x"][moved] += 5.0
+    errs = _block(g, s2, isel)
+    assert errs["nn_mismatch"] == 0 and errs["acc_norm"] < 1e-6, errs
