@@ -17,6 +17,8 @@
  * bitwise identical whatever else is resident (its arithmetic depends only on
  * its own particles and N).
  *
+ * One engine per process, not re-entrant: call it from one thread at a time.
+ *
  * Continuous batching: g6e_submit queues jobs on the host; g6e_run admits
  * queued jobs into free slots while the particle pool (sized from an HBM
  * budget at g6e_open) has room, runs engine steps as HIP-graph replays, and

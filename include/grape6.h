@@ -22,6 +22,8 @@
  *     and retries).  read_neighbour_list > 0 = list storage overflow,
  *     < 0 = error.  get_neighbour_list != 0 = list longer than max_length.
  *   - cluster_id is accepted and ignored (one device per process).
+ *   - threads: the library holds one engine per process and is not
+ *     re-entrant; call it from one thread at a time, as kira does.
  */
 #ifndef LIBG6_MI355X_GRAPE6_H
 #define LIBG6_MI355X_GRAPE6_H
