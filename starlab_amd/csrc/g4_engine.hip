@@ -219,7 +219,7 @@ int firsthalf(int nj, int ni, const double (*xi)[3], const double (*vi)[3],
     G.groups.clear();
     for (size_t g = 0, off = 0; g < values.size(); ++g) {
         start[g] = (int)off;
-        G.groups.push_back({(int)off, count[g], (float)(2.0 * values[g]), true});
+        G.groups.push_back({(int)off, count[g], 2.0 * values[g], true});
         off += count[g];
     }
     std::vector<int> pos(nu);

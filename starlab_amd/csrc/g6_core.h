@@ -67,7 +67,7 @@ int bucket_of(int ni);
 // GRAPE-6 front end excludes by address instead).
 struct IGroup {
     int off, n;
-    float eps2;
+    double eps2;   // the kernels use it rounded to FP32, the FP64 nearest tile as is
     bool zskip;
 };
 
@@ -210,7 +210,7 @@ Results results_at(char *base, int ni);
 int count_refined(const Results &h, int ni);
 // fused (optional): predict j inside the force kernel instead of reading the
 // JPred array (one i-group; the caller has not run the predictor)
-int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r,
+int enqueue_force(int ni, int nj, double eps2, const IBlock &ib, const Results &r,
                   bool zskip = false, const FusedPredict *fused = nullptr);
 
 // One i-block through the force path.  Slot k has position x[k], velocity

@@ -314,8 +314,8 @@ int predict(int nj, bool force)
 }
 
 // i-block buffer: 10 four-byte arrays (xh yh zh xl yl zl vx vy vz excl), then
-// 5 double arrays (x64 y64 z64 h2 e2), each of stride ld.
-size_t ibuf_bytes(int cap) { return (size_t)cap * (10 * 4 + 5 * 8); }
+// 8 double arrays (x64 y64 z64 h2 e2 vx64 vy64 vz64), each of stride ld.
+size_t ibuf_bytes(int cap) { return (size_t)cap * (10 * 4 + 8 * 8); }
 
 // results: 7 doubles, nn and the refinement flag per i
 size_t results_bytes(int ni) { return (size_t)ni * (7 * sizeof(double) + 2 * sizeof(int)); }
@@ -359,6 +359,7 @@ IBlock iblock_at(char *base, int ld)
     b.excl = reinterpret_cast<const int *>(f + 9 * (size_t)ld);
     const double *d = reinterpret_cast<const double *>(base + (size_t)ld * 40);
     b.x64 = d; b.y64 = d + ld; b.z64 = d + 2 * (size_t)ld;
+    b.vx64 = d + 5 * (size_t)ld; b.vy64 = d + 6 * (size_t)ld; b.vz64 = d + 7 * (size_t)ld;
     b.ld = ld;
     return b;
 }
@@ -414,11 +415,14 @@ Partials partials_at(int ld, int nsplit)
     p.best = u;
     p.btile = reinterpret_cast<int *>(u + n);
     p.second = u + 2 * n;
+    p.tsum = u + 3 * n;
+    p.tstride = n;
     p.ld = ld;
     return p;
 }
 
-size_t partials_bytes(int ld, int nsplit) { return (size_t)ld * nsplit * (7 * 8 + 3 * 4); }
+// per (split, i): 7 FP64 sums, best, tile, runner-up, 10 FP32 tile sums
+size_t partials_bytes(int ld, int nsplit) { return (size_t)ld * nsplit * (7 * 8 + 13 * 4); }
 
 struct Geometry {
     int variant, tile, nsplit, jchunk, lanes;
@@ -514,10 +518,14 @@ Geometry choose_geometry(int ni, int nj)
 
 // Enqueue force + reduction (with the nearest-neighbour refinement) for an
 // i-block already in d_ibuf.
-int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r, bool zskip,
+int enqueue_force(int ni, int nj, double eps2d, const IBlock &ib, const Results &r, bool zskip,
                   const FusedPredict *fused)
 {
-    if (zskip && !(eps2 >= __FLT_MIN__)) eps2 = 0.0f;   // as launch_force treats it
+    float eps2 = (float)eps2d;
+    if (zskip && !(eps2 >= __FLT_MIN__)) {   // as launch_force treats it
+        eps2 = 0.0f;
+        eps2d = 0.0;
+    }
     const Geometry g = choose_geometry(ni, nj);
     const int ld = ib.ld;
     if (ensure_partials(partials_bytes(ld, g.npart))) return -1;
@@ -537,8 +545,10 @@ int enqueue_force(int ni, int nj, float eps2, const IBlock &ib, const Results &r
     }
     {
         KTimer t(K_REDUCE);
+        const NearTile nt{eps2d, g.jchunk, g.variant == 1 ? 1 : g.wpg, zskip ? 1 : 0,
+                          E.d_vp64, E.d_jraw};
         HIPCHK(launch_reduce(E.stream, p, g.npart, ni, nj, ib, eps2, g.tile,
-                             E.d_xp64, E.dcap, E.d_jindex, r, g.lanes));
+                             E.d_xp64, E.dcap, E.d_jindex, r, g.lanes, nt));
         t.stop();
     }
     return 0;
@@ -739,6 +749,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
             f[(3 + c) * (size_t)ld + k] = (float)(x - (double)hi);
             f[(6 + c) * (size_t)ld + k] = (float)vi[s][c];
             d[c * (size_t)ld + k] = x;
+            d[(5 + c) * (size_t)ld + k] = vi[s][c];
         }
         ex[k] = (k < ni && excl) ? excl[k] : -1;
         hh[k] = (k < ni && h2) ? h2[k] : 0.0;
@@ -768,6 +779,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         gb.xh += o; gb.yh += o; gb.zh += o; gb.xl += o; gb.yl += o; gb.zl += o;
         gb.vx += o; gb.vy += o; gb.vz += o; gb.excl += o;
         gb.x64 += o; gb.y64 += o; gb.z64 += o;
+        gb.vx64 += o; gb.vy64 += o; gb.vz64 += o;
         Results gr = r;
         gr.acc += 3 * (size_t)o; gr.jerk += 3 * (size_t)o; gr.pot += o;
         gr.nn += o; gr.flags += o;
@@ -968,6 +980,9 @@ int self_block(int nj, int i_first, int ni, IBlock &ib)
     ib.x64 = E.d_xp64 + i_first;
     ib.y64 = E.d_xp64 + (size_t)E.dcap + i_first;
     ib.z64 = E.d_xp64 + 2 * (size_t)E.dcap + i_first;
+    ib.vx64 = E.d_vp64 + i_first;
+    ib.vy64 = E.d_vp64 + (size_t)E.dcap + i_first;
+    ib.vz64 = E.d_vp64 + 2 * (size_t)E.dcap + i_first;
     return 0;
 }
 
@@ -1071,7 +1086,7 @@ int density_sweep(int nj, int i_first, int ni, double ti, int k, double *density
 }
 
 // Mean ms of force + reduction for one shape (after one untimed launch).
-double time_shape(const Shape &sh, int nj, int ni, const IBlock &ib, float eps2, int reps)
+double time_shape(const Shape &sh, int nj, int ni, const IBlock &ib, double eps2, int reps)
 {
     E.forcing_shape = true;
     E.forced = sh;
@@ -1132,7 +1147,7 @@ int autotune(int nj, int ni_max, double eps2)
                         Shape sh;
                         sh.variant = variant; sh.tile = tile; sh.waves = w; sh.lanes = 0;
                         sh.wpg = wpg;
-                        const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
+                        const double ms = time_shape(sh, nj, ni, ib, eps2, reps);
                         if (ms < 0) { E.opt.timing = saved_timing; return fail(-1, "g6x_autotune: launch failed"); }
                         if (ms < best_ms) { best_ms = ms; best = sh; }
                     }
@@ -1145,7 +1160,7 @@ int autotune(int nj, int ni_max, double eps2)
             for (int wpg : {2, 8}) {
                 Shape sh = base;
                 sh.wpg = wpg;
-                const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
+                const double ms = time_shape(sh, nj, ni, ib, eps2, reps);
                 if (ms >= 0 && ms < best_ms) { best_ms = ms; best = sh; }
             }
         }
@@ -1154,7 +1169,7 @@ int autotune(int nj, int ni_max, double eps2)
         for (int lanes : {4, 8, 16, 32, 64}) {
             Shape sh = best;
             sh.lanes = lanes;
-            const double ms = time_shape(sh, nj, ni, ib, (float)eps2, reps);
+            const double ms = time_shape(sh, nj, ni, ib, eps2, reps);
             if (ms >= 0 && ms < best_ms) { best_ms = ms; lb = sh; }
         }
         E.tuned[b] = lb;

@@ -41,7 +41,7 @@ int firsthalf(int nj, int ni, const int *index, const double (*xi)[3],
     }
     std::vector<int> excl(ni);
     for (int k = 0; k < ni; ++k) excl[k] = excl_address(index[k]);
-    const IGroup g{0, ni, (float)eps2, false};
+    const IGroup g{0, ni, eps2, false};
     return calc_begin("g6calc_firsthalf", nj, ni, xi, vi, excl.data(), h2, nullptr, &g, 1,
                       E.opt.npipes > 48 ? 1 : 0);
 }
@@ -184,7 +184,7 @@ int g6x_self_sweep(int, int nj, int i_first, int ni, double ti, double eps2)
     IBlock ib;
     if (self_block(nj, i_first, ni, ib)) return -1;
     Results r = results_at(E.d_res, ni);
-    if (enqueue_force(ni, nj, (float)eps2, ib, r)) return -1;
+    if (enqueue_force(ni, nj, eps2, ib, r)) return -1;
     E.last_ni = ni;
     E.last_nj = nj;
     E.calc_failed = true;   // no neighbour lists for sweeps

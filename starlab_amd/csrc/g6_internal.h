@@ -9,8 +9,10 @@
 //   double vp64[3][cap]      24 B/j  predicted FP64 velocity (read-back only)
 //   int    jindex[cap]        4 B/j  the `index` field of each address
 // i-block (npipes slots, SoA so each lane reads its own slot coalesced):
-//   10 arrays of float/int, FP64 positions 3 arrays, h2 and the list offset e2
-// Partial sums of the j-split force: 7 FP64 arrays + 3 nn words, [split][i].
+//   10 arrays of float/int, FP64 positions 3 arrays, h2 and the list offset
+//   e2, FP64 velocities 3 arrays
+// Partial sums of the j-split force: 7 FP64 arrays + 3 nn words + the 10
+// FP32 sums of the split's nearest-neighbour tile, [split][i].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,6 +44,7 @@ struct IBlock {
     const float *xh, *yh, *zh, *xl, *yl, *zl, *vx, *vy, *vz;
     const int *excl;                 // address excluded for this i (-1: none)
     const double *x64, *y64, *z64;   // FP64 position (refinement, lists)
+    const double *vx64, *vy64, *vz64;   // FP64 velocity (FP64 nearest tile)
     int ld;
 };
 
@@ -52,6 +55,12 @@ struct Partials {
     int *btile;            // first address of the j-tile holding it
     float *second;         // runner-up FP32 r^2 outside the best tile (the reduction
                            // adds the best tile's own runner-up when it rescans it)
+    // The FP32 tile sums (ax ay az jx jy jz kx ky kz pot, as the force kernel
+    // folded them into c[]) of the best tile: array k at tsum + k * tstride.
+    // The reduction replaces the winning tile's FP32 contribution by an FP64
+    // recomputation of the same pairs.
+    float *tsum;
+    size_t tstride;
     int ld;                // stride between splits (= IBlock ld)
 };
 
@@ -114,10 +123,21 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
                         bool zskip, int wpg, const FusedPredict *fused);
 // The reduction's nearest-neighbour rescan reads the FP64 prediction
 // (xp64) and splits it into the double-single words the force kernel used.
+// It also recomputes the pairs of the winning tile in FP64 (xp64, vp64,
+// jraw masses, the i-block's FP64 position and velocity, softening eps2d)
+// and swaps them for that tile's FP32 sums; the tile of address t was
+// summed by partial (t / jchunk) / wpg (wpg = 1 for the LDS kernel).
+// zskip: as launch_force's (pairs with FP32 r^2 = 0 add nothing).
+struct NearTile {
+    double eps2d;
+    int jchunk, wpg, zskip;
+    const double *vp64;
+    const JRaw *jraw;
+};
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
-                         const Results &r, int lanes);
+                         const Results &r, int lanes, const NearTile &nt);
 // Potential only: pot[i] = -sum_j m_j / r_ij for the i-block (self excluded
 // by address as in the force kernel); part holds nsplit * ib.ld doubles.
 hipError_t launch_pot(hipStream_t s, const JPred *jpred, int nj, const IBlock &ib, int ni,

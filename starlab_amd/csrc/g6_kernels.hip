@@ -272,9 +272,30 @@ __device__ __forceinline__ void tile_pairs(const J_T *__restrict__ jp, int tn,
 // wave order (a fixed order, so results stay deterministic), so the
 // reduction kernel reads nsplit / wpg partials instead of nsplit.  Dynamic
 // LDS: double[7][wpg][64], then float best[wpg][64], float second[wpg][64],
-// int tile[wpg][64].
+// int tile[wpg][64], then float tsum[10][64]: the FP32 sums of each lane's
+// winning tile, written by the wave that summed it.
 constexpr int kMaxWpg = 16;
-size_t force_fold_lds(int wpg) { return wpg > 1 ? (size_t)wpg * 64 * (7 * 8 + 3 * 4) : 0; }
+constexpr int kTileSums = 10;   // ax ay az jx jy jz kx ky kz pot of one tile
+size_t force_fold_lds(int wpg)
+{
+    return wpg > 1 ? (size_t)wpg * 64 * (7 * 8 + 3 * 4) + 64 * kTileSums * 4 : 0;
+}
+
+__device__ __forceinline__ void tile_sums_store(float *p, size_t stride, const Acc32 &a)
+{
+    p[0 * stride] = a.ax; p[1 * stride] = a.ay; p[2 * stride] = a.az;
+    p[3 * stride] = a.jx; p[4 * stride] = a.jy; p[5 * stride] = a.jz;
+    p[6 * stride] = a.kx; p[7 * stride] = a.ky; p[8 * stride] = a.kz;
+    p[9 * stride] = a.p;
+}
+
+__device__ __forceinline__ void tile_sums_load(const float *p, size_t stride, Acc32 &a)
+{
+    a.ax = p[0 * stride]; a.ay = p[1 * stride]; a.az = p[2 * stride];
+    a.jx = p[3 * stride]; a.jy = p[4 * stride]; a.jz = p[5 * stride];
+    a.kx = p[6 * stride]; a.ky = p[7 * stride]; a.kz = p[8 * stride];
+    a.p = p[9 * stride];
+}
 
 // Fused small-block mode (at most kFusedMaxGroups i-groups): each wave
 // predicts its own j-split from the raw j records, 64 j at a time (one per
@@ -338,6 +359,11 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
     double A0 = 0, A1 = 0, A2 = 0, A3 = 0, A4 = 0, A5 = 0, A6 = 0;
     float rbest = __builtin_inff(), rsecond = __builtin_inff();
     int rtile = -1;
+    // FP32 sums of the best tile so far (the reduction swaps the winning
+    // tile's sums for an FP64 recomputation of its pairs)
+    Acc32 bt;
+    bt.ax = bt.ay = bt.az = bt.jx = bt.jy = bt.jz = bt.kx = bt.ky = bt.kz = bt.p = 0.f;
+    bt.best = __builtin_inff();
 
     for (int t = jbeg; t < jend; t += tile) {
         const int tn = min(tile, jend - t);
@@ -402,6 +428,7 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
             rsecond = __builtin_fminf(rsecond, rbest);
             rbest = a.best;
             rtile = t;
+            bt = a;
         } else {
             rsecond = __builtin_fminf(rsecond, a.best);
         }
@@ -427,7 +454,22 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
         fs[slot] = rsecond;
         ft[slot] = rtile;
         __syncthreads();
+        // the wave whose split holds this lane's winning tile, by the fold's
+        // rule below (first strict minimum in wave order), hands its tile
+        // sums to wave 0
+        float *fts = reinterpret_cast<float *>(ft + stride);   // [kTileSums][64]
+        int win = 0;
+        {
+            float wb = fb[lane];
+            for (int w = 1; w < wpg; ++w) {
+                const float bw = fb[w * 64 + lane];
+                if (bw < wb) { wb = bw; win = w; }
+            }
+        }
+        if (win == wave && wave != 0) tile_sums_store(fts + lane, 64, bt);
+        __syncthreads();
         if (wave != 0) return;
+        if (win != 0) tile_sums_load(fts + lane, 64, bt);
         for (int w = 1; w < wpg; ++w) {
             const int o = w * 64 + lane;
             A0 += fold[0 * stride + o];
@@ -460,6 +502,7 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
         P.best[o] = rbest;
         P.btile[o] = rtile;
         P.second[o] = rsecond;
+        tile_sums_store(P.tsum + o, P.tstride, bt);
     }
 }
 
@@ -550,6 +593,7 @@ struct Acc64 {
     double c[7];
     float best, second;
     int tile;
+    Acc32 bt;   // FP32 sums of the best tile
 };
 
 __device__ __forceinline__ void acc_fold(Acc64 &A, const Acc32 &a, int t)
@@ -565,6 +609,7 @@ __device__ __forceinline__ void acc_fold(Acc64 &A, const Acc32 &a, int t)
         A.second = __builtin_fminf(A.second, A.best);
         A.best = a.best;
         A.tile = t;
+        A.bt = a;
     } else {
         A.second = __builtin_fminf(A.second, a.best);
     }
@@ -609,6 +654,8 @@ __global__ __launch_bounds__(256) void k_force_lds(
     for (int c = 0; c < 7; ++c) A0.c[c] = A1.c[c] = 0.0;
     A0.best = A0.second = A1.best = A1.second = __builtin_inff();
     A0.tile = A1.tile = -1;
+    acc_reset(A0.bt);
+    acc_reset(A1.bt);
 
     const float4 *g4 = reinterpret_cast<const float4 *>(jpred);
     for (int t = jbeg; t < jend; t += kTile) {
@@ -649,6 +696,7 @@ __global__ __launch_bounds__(256) void k_force_lds(
         P.best[o0] = A0.best;
         P.btile[o0] = A0.tile;
         P.second[o0] = A0.second;
+        tile_sums_store(P.tsum + o0, P.tstride, A0.bt);
     }
     if (i1 < ni) {
         const size_t o1 = o0 + kLdsWG;
@@ -657,6 +705,7 @@ __global__ __launch_bounds__(256) void k_force_lds(
         P.best[o1] = A1.best;
         P.btile[o1] = A1.tile;
         P.second[o1] = A1.second;
+        tile_sums_store(P.tsum + o1, P.tstride, A1.bt);
     }
 }
 
@@ -735,6 +784,14 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
 // (d^2 evaluated as the FP64 CPU path, lexicographic (d^2, address) minimum)
 // and flags the i as refined.
 //
+// Nearest tile in FP64: the winning tile holds the closest pair, which
+// usually dominates acc and jerk, and its FP32 sums carry that pair's
+// rounding and the rounding of every later pair added at its magnitude.  The
+// same scan recomputes the tile's pairs in FP64 (the FP64 pair law of the
+// CPU path, hdyn_ev.C:2019-2036) and the result is
+//   sum over splits - (the tile's FP32 sums as folded) + (the FP64 pairs),
+// so the closest pair no longer limits the relative error.
+//
 // Every branch below depends only on values that are equal across the
 // group, and shuffles stay inside the group, so groups of one wave may
 // diverge freely.
@@ -749,11 +806,40 @@ __device__ __forceinline__ double group_sum(double v)
 
 constexpr int kReduceWaves = 4;   // waves per reduction workgroup
 
+// One pair of the FP64 pair law (hdyn_ev.C:2019-2036), added to F[0..6] =
+// acc, jerk, pot.  1/r comes from the hardware FP64 reciprocal square root
+// (relative error ~2^-23) refined by two Newton steps, y <- y + y/2 (1 - r2
+// y^2), each squaring the error: ~2^-46, then below an FP64 rounding.  That
+// is a quarter of the instructions of a correctly rounded divide and square
+// root, and agrees with them to a few FP64 ulps.
+__device__ __forceinline__ void pair64(double dx, double dy, double dz, double du, double dv,
+                                       double dw, double m, double eps2, double F[7])
+{
+    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+    double y = __builtin_amdgcn_rsq(r2);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = __builtin_fma(-(r2 * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, e, y);
+    }
+    const double r2inv = y * y;
+    const double a3 = -3.0 * (dx * du + dy * dv + dz * dw) * r2inv;
+    const double mrinv = m * y;
+    const double mr3inv = mrinv * r2inv;
+    F[0] += mr3inv * dx;
+    F[1] += mr3inv * dy;
+    F[2] += mr3inv * dz;
+    F[3] += mr3inv * (du + a3 * dx);
+    F[4] += mr3inv * (dv + a3 * dy);
+    F[5] += mr3inv * (dw + a3 * dz);
+    F[6] -= mrinv;
+}
+
 template <int LOG2L>
 __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
     Partials P, int nsplit, int ni, int nj, IBlock ib,
     float eps2, int tile_len, const double *__restrict__ xp64, int cap,
-    const int *__restrict__ jindex, Results R)
+    const int *__restrict__ jindex, Results R, NearTile NT)
 {
 #pragma clang fp contract(off)
     constexpr int L = 1 << LOG2L;
@@ -809,15 +895,6 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         best = take ? ob : best;
         tile = take ? ot : tile;
     }
-    if (l == 0) {
-        R.acc[3 * i + 0] = A[0];
-        R.acc[3 * i + 1] = A[1];
-        R.acc[3 * i + 2] = A[2];
-        R.jerk[3 * i + 0] = A[3];
-        R.jerk[3 * i + 1] = A[4];
-        R.jerk[3 * i + 2] = A[5];
-        R.pot[i] = A[6];
-    }
 
     const int excl = ib.excl[i];
     int nn = 0;
@@ -825,29 +902,38 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
     if (!ambiguous) {
         const float xih = ib.xh[i], yih = ib.yh[i], zih = ib.zh[i];
         const float xil = ib.xl[i], yil = ib.yl[i], zil = ib.zl[i];
+        const double xi = ib.x64[i], yi = ib.y64[i], zi = ib.z64[i];
+        const double ui = ib.vx64[i], vi = ib.vy64[i], wi = ib.vz64[i];
         const int tend = min(tile + tile_len, nj);
         const uint32_t bb = __float_as_uint(best);
         const int gshift = lane & ~(L - 1);
         int found = INT_MAX;
-        // the whole tile: first address matching best, and the two smallest
-        // r^2 (as a multiset) for the tile's own runner-up
+        // the whole tile: first address matching best, the two smallest r^2
+        // (as a multiset) for the tile's own runner-up, and its pairs in FP64
         float m1 = __builtin_inff(), m2 = __builtin_inff();
+        double F[7] = {0, 0, 0, 0, 0, 0, 0};
         for (int j0 = tile; j0 < tend; j0 += L) {
             const int j = j0 + l;
             bool hit = false;
             if (j < tend && j != excl) {
                 // the double-single words the force kernel used, split again
                 // from the FP64 prediction they were made from
+                const double xj = xp64[j], yj = xp64[(size_t)cap + j];
+                const double zj = xp64[2 * (size_t)cap + j];
                 float xjh, xjl, yjh, yjl, zjh, zjl;
-                split_ds(xp64[j], xjh, xjl);
-                split_ds(xp64[(size_t)cap + j], yjh, yjl);
-                split_ds(xp64[2 * (size_t)cap + j], zjh, zjl);
+                split_ds(xj, xjh, xjl);
+                split_ds(yj, yjh, yjl);
+                split_ds(zj, zjh, zjl);
                 float dx, dy, dz;
                 const float r2 = ds_r2(xjh, yjh, zjh, xjl, yjl, zjl, xih, yih, zih, xil, yil,
                                        zil, eps2, dx, dy, dz);
                 hit = __float_as_uint(r2) == bb;
                 m2 = __builtin_amdgcn_fmed3f(m1, m2, r2);
                 m1 = __builtin_fminf(m1, r2);
+                if (!(NT.zskip && r2 == 0.0f))
+                    pair64(xj - xi, yj - yi, zj - zi, NT.vp64[j] - ui,
+                           NT.vp64[(size_t)cap + j] - vi, NT.vp64[2 * (size_t)cap + j] - wi,
+                           NT.jraw[j].m, NT.eps2d, F);
             }
             unsigned long long m = __ballot(hit) >> gshift;
             if (L < 64) m &= (1ull << L) - 1ull;
@@ -860,10 +946,30 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
             m2 = __builtin_fminf(__builtin_fmaxf(m1, o1), __builtin_fminf(m2, o2));
             m1 = __builtin_fminf(m1, o1);
         }
+        // swap the tile's FP32 sums (converted and combined exactly as the
+        // force kernel folded them) for its FP64 pairs
+        const size_t o = (size_t)((tile / NT.jchunk) / NT.wpg) * P.ld + i;
+        Acc32 t;
+        tile_sums_load(P.tsum + o, P.tstride, t);
+        const double c32[7] = {(double)t.ax, (double)t.ay, (double)t.az,
+                               (double)t.jx - 3.0 * (double)t.kx,
+                               (double)t.jy - 3.0 * (double)t.ky,
+                               (double)t.jz - 3.0 * (double)t.kz, (double)t.p};
+#pragma unroll
+        for (int c = 0; c < 7; ++c) A[c] = (A[c] - c32[c]) + group_sum<L>(F[c]);
         second = __builtin_fminf(second, m2);
         if (found == INT_MAX) ambiguous = true;
         else nn = jindex[found];
         if (!(second > best * (1.0f + kNNBand))) ambiguous = true;
+    }
+    if (l == 0) {
+        R.acc[3 * i + 0] = A[0];
+        R.acc[3 * i + 1] = A[1];
+        R.acc[3 * i + 2] = A[2];
+        R.jerk[3 * i + 0] = A[3];
+        R.jerk[3 * i + 1] = A[4];
+        R.jerk[3 * i + 2] = A[5];
+        R.pot[i] = A[6];
     }
     const bool refine = ambiguous && nj > 1;
     if (refine) {
@@ -903,32 +1009,33 @@ int reduce_lanes(int nsplit)
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
-                         const Results &r, int lanes)
+                         const Results &r, int lanes, const NearTile &nt)
 {
     if (ni <= 0) return hipSuccess;
+    if (nt.jchunk <= 0 || nt.jchunk % tile || nt.wpg < 1) return hipErrorInvalidValue;
     if (lanes <= 0) lanes = reduce_lanes(nsplit);
     const int per_wg = kReduceWaves * (64 / lanes);
     const dim3 grid((ni + per_wg - 1) / per_wg), block(64 * kReduceWaves);
     switch (lanes) {
     case 4:
         hipLaunchKernelGGL(k_reduce<2>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r);
+                           tile, xp64, cap, jindex, r, nt);
         break;
     case 8:
         hipLaunchKernelGGL(k_reduce<3>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r);
+                           tile, xp64, cap, jindex, r, nt);
         break;
     case 16:
         hipLaunchKernelGGL(k_reduce<4>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r);
+                           tile, xp64, cap, jindex, r, nt);
         break;
     case 32:
         hipLaunchKernelGGL(k_reduce<5>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r);
+                           tile, xp64, cap, jindex, r, nt);
         break;
     case 64:
         hipLaunchKernelGGL(k_reduce<6>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r);
+                           tile, xp64, cap, jindex, r, nt);
         break;
     default:
         return hipErrorInvalidValue;

@@ -5,14 +5,38 @@ import numpy as np
 
 import oracle
 
-# Tolerances (FP32 pair arithmetic on double-single positions, FP64 sums):
+# Tolerances (FP32 pair arithmetic on double-single positions, FP64 sums,
+# the nearest-neighbour tile recomputed in FP64):
 #   pot:   max |dpot/pot|                      <= POT_TOL
 #   acc:   max |da| / sum_j |a_ij|             <= ACC_TOL   (normwise)
 #   jerk:  max |dj| / sum_j |j_ij|             <= JERK_TOL  (normwise)
+#   acc:   max |da| / |a|                      <= ACC_REL_TOL (per particle;
+#          asserted where no particle's acceleration cancels to ~0)
 #   nn:    exact index agreement
+# Per-particle jerk error |dj| / |j| is reported, not bounded: where the
+# pair terms cancel (|j| << sum_j |j_ij|) FP32 pair arithmetic cannot hold it
+# at 1e-6.
 POT_TOL = 1e-6
 ACC_TOL = 1e-6
 JERK_TOL = 1e-6
+ACC_REL_TOL = 1e-6
+
+
+def percentiles(a):
+    """median, p99, max of an error array (a dict of floats)."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.size == 0:
+        return dict(median=0.0, p99=0.0, max=0.0)
+    return dict(median=float(np.median(a)), p99=float(np.percentile(a, 99)),
+                max=float(a.max()))
+
+
+def per_particle(acc, jerk, pot, racc, rjerk, rpot):
+    """Per-particle relative errors |da|/|a|, |dj|/|j|, |dpot/pot|."""
+    ea = np.linalg.norm(acc - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    ej = np.linalg.norm(jerk - rjerk, axis=1) / np.linalg.norm(rjerk, axis=1)
+    ep = np.abs(pot - rpot) / np.abs(rpot)
+    return ea, ej, ep
 
 
 def plummer_state(n, seed=42, tspread=2.0 ** -10, k18=False):
@@ -69,13 +93,15 @@ def compare(acc, jerk, pot, nn, ref, xi, vi, mj, xj, vj, eps2, excl):
         sj[k] = (mj * (np.sqrt((w * w).sum(1)) * 4.0) / (r2 * r)).sum()
     ea = np.linalg.norm(acc - racc, axis=1)
     ej = np.linalg.norm(jerk - rjerk, axis=1)
+    ra, rj, rp = per_particle(acc, jerk, pot, racc, rjerk, rpot)
+    pa, pj, pp = percentiles(ra), percentiles(rj), percentiles(rp)
     out = dict(
         acc_norm=float((ea / sa).max()),
         jerk_norm=float((ej / sj).max()),
-        pot_rel=float((np.abs(pot - rpot) / np.abs(rpot)).max()),
-        acc_rel_median=float(np.median(ea / np.linalg.norm(racc, axis=1))),
-        acc_rel_max=float((ea / np.linalg.norm(racc, axis=1)).max()),
-        jerk_rel_median=float(np.median(ej / np.linalg.norm(rjerk, axis=1))),
+        pot_rel=pp["max"],
+        acc_rel_median=pa["median"], acc_rel_p99=pa["p99"], acc_rel_max=pa["max"],
+        jerk_rel_median=pj["median"], jerk_rel_p99=pj["p99"], jerk_rel_max=pj["max"],
+        pot_rel_median=pp["median"], pot_rel_p99=pp["p99"],
         nn_mismatch=int((nn != rnn).sum()),
     )
     return out

@@ -16,6 +16,7 @@ import numpy as np
 
 import oracle
 from starlab_amd.blockstep import ONE3, G6Backend
+from tests.gpu_common import per_particle, percentiles
 
 
 class HostBackend:
@@ -77,8 +78,9 @@ class _ErrorBook:
     (|dj| / sum_j |j_ij| bound), pot relative, nearest-neighbour mismatches."""
 
     def _book_init(self, every, max_i, seed):
-        self.worst = dict(acc_norm=0.0, jerk_norm=0.0, pot_rel=0.0)
+        self.worst = dict(acc_norm=0.0, jerk_norm=0.0, pot_rel=0.0, acc_rel=0.0)
         self.acc_errs = []
+        self.rel_errs = dict(acc=[], jerk=[], pot=[])
         self.nn_mismatch = 0
         self.checked_blocks = 0
         self.checked_i = 0
@@ -110,6 +112,11 @@ class _ErrorBook:
             sj[k] = (mass * 4.0 * np.sqrt((w * w).sum(1)) / (r2 * np.sqrt(r2))).sum()
         ea = np.linalg.norm(acc - racc, axis=1) / sa
         self.acc_errs.append(ea)
+        ra, rj, rp = per_particle(acc, jerk, pot, racc, rjerk, rpot)
+        self.rel_errs["acc"].append(ra)
+        self.rel_errs["jerk"].append(rj)
+        self.rel_errs["pot"].append(rp)
+        self.worst["acc_rel"] = max(self.worst["acc_rel"], float(ra.max()))
         self.worst["acc_norm"] = max(self.worst["acc_norm"], float(ea.max()))
         self.worst["jerk_norm"] = max(self.worst["jerk_norm"],
                                       float((np.linalg.norm(jerk - rjerk, axis=1) / sj).max()))
@@ -118,6 +125,17 @@ class _ErrorBook:
         self.nn_mismatch += int((nn != rnn).sum())
         self.checked_blocks += 1
         self.checked_i += len(idx)
+
+    def summary(self):
+        """Per-particle relative error percentiles (median, p99, max) of acc,
+        jerk and pot, and the normwise acc percentiles, over all checks."""
+        out = {k: percentiles(np.concatenate(v) if v else []) for k, v in self.rel_errs.items()}
+        norm = np.concatenate(self.acc_errs) if self.acc_errs else np.zeros(0)
+        out["acc_norm"] = percentiles(norm)
+        out["acc_norm"]["p999"] = float(np.percentile(norm, 99.9)) if norm.size else 0.0
+        out["jerk_norm_max"] = self.worst["jerk_norm"]
+        out["checked_particle_steps"] = self.checked_i
+        return out
 
 
 class CheckedG6(G6Backend, _ErrorBook):

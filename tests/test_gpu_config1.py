@@ -4,10 +4,13 @@ eps = 0, eta = 0.1) from t = 0 to t = 1 with individual power-of-two block
 steps, forces from libg6 through the GRAPE-6 ABI.
 
 * every block step: libg6's acc/jerk/pot agree with the FP64 oracle on the
-  same inputs and the nearest-neighbour index is exact.  Bars: normwise acc
-  error (|da| / sum_j |a_ij|) <= 1e-6 for 99.9 % of particle steps and
-  <= 2e-6 for every one (a single dominant pair term carries up to ~1.1e-6
-  from FP32 rounding of r^2, rsqrt and r^-3); jerk <= 2e-6; pot <= 1e-6;
+  same inputs and the nearest-neighbour index is exact.  Bars, for every
+  particle step: normwise acc and jerk error (|da| / sum_j |a_ij|) <= 1e-6
+  and pot <= 1e-6; per-particle acc error |da| / |a| <= 1e-6 for 99.9 % of
+  particle steps (the rest are steps whose pair terms cancel to |a| well
+  below sum_j |a_ij|, where FP32 pair rounding is amplified; measured max
+  2.4e-6 in one of 257725 steps, normwise 1.5e-7).  Per-particle
+  percentiles (median, p99, max) of acc, jerk and pot are printed;
 * the energy error |dE/E| at t = 1 stays within a factor 10 (+1e-8) of the
   all-CPU FP64 run (kira's host force path) from the same initial state.
 """
@@ -51,12 +54,10 @@ def test_config1_plummer_1024_to_t1():
     assert np.all(gpu.t == 1.0)
     assert be.checked_blocks == gpu.blocks + 1
     assert be.nn_mismatch == 0
-    errs = np.concatenate(be.acc_errs)
-    print(f"acc normwise error: median {np.median(errs):.2e} p99.9 "
-          f"{np.percentile(errs, 99.9):.2e} max {errs.max():.2e}", flush=True)
-    assert np.percentile(errs, 99.9) <= 1e-6
-    assert be.worst["acc_norm"] <= 2e-6
-    assert be.worst["jerk_norm"] <= 2e-6
+    print(f"errors vs FP64: {be.summary()}", flush=True)
+    assert be.worst["acc_norm"] <= 1e-6
+    assert be.worst["jerk_norm"] <= 1e-6
+    assert np.percentile(np.concatenate(be.rel_errs["acc"]), 99.9) <= 1e-6
     assert be.worst["pot_rel"] <= 1e-6
     assert de_gpu <= 10 * de_cpu + 1e-8
 
@@ -82,6 +83,6 @@ def test_config1_native_driver_equals_python_driver():
         for name in ("x", "v", "a", "j", "pot", "dt"):
             np.testing.assert_array_equal(getattr(other, name), getattr(py, name), err_msg=name)
     assert chk.checked_blocks == nat.blocks and chk.nn_mismatch == 0
-    assert np.percentile(np.concatenate(chk.acc_errs), 99.9) <= 1e-6
-    assert chk.worst["acc_norm"] <= 2e-6 and chk.worst["jerk_norm"] <= 2e-6
+    assert chk.worst["acc_norm"] <= 1e-6 and chk.worst["jerk_norm"] <= 1e-6
+    assert np.percentile(np.concatenate(chk.rel_errs["acc"]), 99.9) <= 1e-6
     assert chk.worst["pot_rel"] <= 1e-6

@@ -6,9 +6,12 @@ t = 0.25 by the native block-step driver (libkira_blockstep.so), forces from
 libg6 through the GRAPE-6 ABI (eps = 0, eta = 0.1).
 
 * a seeded sample of block steps (every 97th force call, at most 16 i each)
-  is recomputed by the FP64 oracle on the same inputs: acc normwise error
-  <= 1e-6 for 99.9 % of sampled particle steps and <= 2e-6 for all, jerk
-  <= 2e-6, pot <= 1e-6, nearest-neighbour index exact;
+  is recomputed by the FP64 oracle on the same inputs: acc and jerk
+  normwise error <= 1e-6 and pot <= 1e-6 for every sampled particle step,
+  per-particle acc error |da|/|a| <= 1e-6 for 99.9 % of them (the 100:1
+  masses let a heavy j that is not the nearest dominate, and sums that
+  cancel 5-20 fold amplify the FP32 pair rounding), nearest-neighbour index
+  exact; per-particle percentiles of acc, jerk and pot are printed;
 * every particle lands on t = 0.25 and block times are exact binary
   fractions (the power-of-two schedule);
 * the total energy (FP64 oracle pair sum) changes by |dE/E| <= 1e-4.
@@ -87,8 +90,10 @@ def test_config3_king_salpeter_binaries_262144_to_quarter():
     assert all(abs(t * 2.0 ** 40 - round(t * 2.0 ** 40)) == 0 for t in times)
     assert np.all(np.log2(b.dt) == np.round(np.log2(b.dt)))
     assert be.checked_blocks >= 400 and be.nn_mismatch == 0
-    assert np.percentile(errs, 99.9) <= 1e-6
-    assert be.worst["acc_norm"] <= 2e-6
-    assert be.worst["jerk_norm"] <= 2e-6
+    summ = be.summary()
+    print(f"errors vs FP64: {summ}", flush=True)
+    assert be.worst["acc_norm"] <= 1e-6
+    assert be.worst["jerk_norm"] <= 1e-6
     assert be.worst["pot_rel"] <= 1e-6
+    assert np.percentile(np.concatenate(be.rel_errs["acc"]), 99.9) <= 1e-6
     assert de <= 1e-4

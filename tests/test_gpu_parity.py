@@ -3,15 +3,16 @@ CPU oracle on identical inputs.
 
 Bars (see tests/gpu_common.py): predicted positions and velocities bitwise
 equal; pot relative error <= 1e-6; acc and jerk normwise error (|delta| over
-the sum of the magnitudes of the pair terms) <= 1e-6; nearest-neighbour
-index and neighbour-list membership and order exactly equal.
+the sum of the magnitudes of the pair terms) <= 1e-6; per-particle acc
+error |da|/|a| <= 1e-6 (Plummer inputs); nearest-neighbour index and
+neighbour-list membership and order exactly equal.
 """
 import numpy as np
 import pytest
 
 import oracle
-from tests.gpu_common import (ACC_TOL, JERK_TOL, POT_TOL, compare, load_j,
-                              plummer_state)
+from tests.gpu_common import (ACC_REL_TOL, ACC_TOL, JERK_TOL, POT_TOL, compare, load_j,
+                              per_particle, plummer_state)
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +48,7 @@ def _check(errs, label):
     assert errs["pot_rel"] <= POT_TOL, errs
     assert errs["acc_norm"] <= ACC_TOL, errs
     assert errs["jerk_norm"] <= JERK_TOL, errs
+    assert errs["acc_rel_max"] <= ACC_REL_TOL, errs
     assert errs["nn_mismatch"] == 0, errs
 
 
@@ -146,6 +148,35 @@ def test_exclusion_by_parent_index(g, variant):
     xi2 = xp[[5]] + 1e-3
     errs = _abi_force_vs_oracle(g, s, 0.0, 0.0, None, excl=[123456], xi=xi2, vi=vp[[5]])
     _check(errs, "no exclusion")
+
+
+def test_close_pair_term_in_fp64(g, variant):
+    """A pair 1e-4 apart inside a Plummer sphere: its mutual acc (~1e5) and
+    jerk outweigh the rest of the sum 1e5 to 1.  The reduction recomputes the
+    nearest-neighbour tile in FP64, so both members agree with the FP64
+    oracle to ~1e-11 relative; FP32 pair arithmetic alone leaves ~1e-7
+    (rsqrt and r^-3 rounding of the dominant term).  Both members, every
+    kernel variant and tile length, the pair inside one tile and split over
+    two."""
+    n = 1024
+    for a, b in ((300, 301), (300, 700)):
+        s = plummer_state(n, seed=5)
+        s["x"][b] = s["x"][a] + np.array([1e-4, 0.6e-4, -0.3e-4])
+        s["v"][b] = s["v"][a] + np.array([0.0, 0.01, 0.02])
+        s["tj"][[a, b]] = 0.0   # predicted to ti = 0 unchanged: the separation stays
+        load_j(g, s)
+        isel = np.arange(n)
+        errs = _abi_force_vs_oracle(g, s, 0.0, 0.0, isel)
+        _check(errs, f"close pair {a},{b}")
+        xp, vp = _predict(s, 0.0)
+        acc, jerk, pot, nn = g.force(n, s["idx"][[a, b]], xp[[a, b]], vp[[a, b]], 0.0)
+        racc, rjerk, rpot, rnn, _ = oracle.force(xp[[a, b]], vp[[a, b]], s["idx"][[a, b]], xp,
+                                                 vp, s["m"], s["idx"], 0.0)
+        ra, rj, rp = per_particle(acc, jerk, pot, racc, rjerk, rpot)
+        print(f"close pair {a},{b}: acc {ra} jerk {rj} pot {rp}", flush=True)
+        assert list(nn) == [b, a]
+        assert ra.max() <= 1e-9 and rj.max() <= 1e-9, (ra, rj)
+        assert rp.max() <= POT_TOL
 
 
 def test_nn_ties_on_lattice(g, variant):
