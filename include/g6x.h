@@ -53,6 +53,31 @@ int g6x_potential(int cluster_id, int nj, int i_first, int ni, double ti, double
 int g6x_densities(int cluster_id, int nj, int i_first, int ni, double ti, int k,
                   double *density, double *dk2);
 
+/* g6x_self_sweep for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  The same computation, geometry and
+ * results as g6x_self_sweep, but its scratch -- the i-block, the j-split
+ * partial sums and the refinement flags -- lives in a fixed 48 MiB device
+ * array per GPU (held until the process ends) instead of library
+ * allocations, and the results go straight to the caller's DEVICE arrays
+ * acc[ni][3], jerk[ni][3], pot[ni], nnbindex[ni].  It calls no hipMalloc,
+ * hipFree or other allocating routine, and event timing is off for the call.
+ * Preconditions, each refused with an error (g6x_last_error) instead of
+ * growing a buffer: j-memory holds at least nj addresses and is resident on
+ * the device (any force call, g6x_self_sweep or g6x_autotune since the last
+ * new address places it), and the geometry for (ni, nj) fits the array (the
+ * default geometry, waves_target 16384, fits every ni up to 65536 at any
+ * nj: at most 36.5 MiB).  j updates made
+ * since then go up by copy or by the existing staging buffer.
+ * Asynchronous on the library stream: g6x_sync() waits.  Every call on a GPU
+ * shares that GPU's array and keeps the previous call's contents (nothing in
+ * it needs clearing); calls are ordered on the library stream, so only
+ * callers that enter the library from several threads at once race.
+ * g6x_read_results does not return these results. */
+int g6x_self_sweep_noalloc(int cluster_id, int nj, int i_first, int ni, double ti,
+                           double eps2, double *acc, double *jerk, double *pot,
+                           int *nnbindex);
+
 /* Copy the results of the last g6x_self_sweep (ni entries) to the host.
  * Returns -1 if the last calculation was anything else (an ABI block,
  * g6x_potential, g6x_densities, g6x_autotune), if that sweep failed, or if a

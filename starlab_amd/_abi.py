@@ -41,6 +41,8 @@ GRAPE6 = {
 G6X = {
     "g6x_set_j_particles": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "g6x_self_sweep": (_i, [_i, _i, _i, _i, C.c_double, C.c_double]),
+    "g6x_self_sweep_noalloc": (_i, [_i, _i, _i, _i, C.c_double, C.c_double, _vp, _vp, _vp,
+                                    _vp]),
     "g6x_potential": (_i, [_i, _i, _i, _i, C.c_double, C.c_double, _vp]),
     "g6x_densities": (_i, [_i, _i, _i, _i, C.c_double, _i, _vp, _vp]),
     "g6x_read_results": (_i, [_i, _i, _vp, _vp, _vp, _vp]),

@@ -199,7 +199,9 @@ int excl_address(int index);
 // defer: leave a packed scatter in h_stage for calc_begin to launch together
 // with the i-block upload (deferred_scatter records); any other flush_j
 // launches a deferred scatter first.
-int flush_j(int nj, bool defer = false);
+// noalloc: fail instead of growing any buffer (a dirty list longer than the
+// staging buffer then goes up as one copy of the used range)
+int flush_j(int nj, bool defer = false, bool noalloc = false);
 int predict(int nj, bool force);
 
 // i-block and results (host or device copies share one layout)
@@ -210,8 +212,11 @@ Results results_at(char *base, int ni);
 int count_refined(const Results &h, int ni);
 // fused (optional): predict j inside the force kernel instead of reading the
 // JPred array (one i-group; the caller has not run the predictor)
+// part (optional): device memory for the j-split partial sums, at least
+// partials_bytes(ib.ld, partial splits of the geometry); nullptr = the
+// library's own (grown) buffer.
 int enqueue_force(int ni, int nj, double eps2, const IBlock &ib, const Results &r,
-                  bool zskip = false, const FusedPredict *fused = nullptr);
+                  bool zskip = false, const FusedPredict *fused = nullptr, char *part = nullptr);
 
 // One i-block through the force path.  Slot k has position x[k], velocity
 // v[k], excluded j address excl[k] (-1: none; nullptr: none for all), and
@@ -243,6 +248,15 @@ void reset_state();
 
 // device-resident sweeps and the autotuner
 int self_block(int nj, int i_first, int ni, IBlock &ib);
+// g6x_self_sweep without allocation: scratch (i-block, flags, partials) in
+// the caller's workspace ws (ws_bytes, device memory), results to the device
+// arrays of out (acc, jerk, pot, nn; its flags are ignored).  Fails, with a
+// message, when ws_bytes < sweep_workspace_bytes(nj, ni), when j-memory does
+// not cover nj addresses or is not resident on the device.  Asynchronous on
+// the library stream.
+size_t sweep_workspace_bytes(int nj, int ni);
+int self_sweep_into(const char *who, int nj, int i_first, int ni, double ti, double eps2,
+                    const Results &out, char *ws, size_t ws_bytes);
 // pot of the j-particles [i_first, i_first + ni) against [0, nj) at time ti
 // (self excluded), to the host array pot[ni]
 int potential_sweep(int nj, int i_first, int ni, double ti, double eps2, double *pot);

@@ -250,9 +250,17 @@ int launch_deferred_scatter()
     return 0;
 }
 
-int flush_j(int nj, bool defer)
+int flush_j(int nj, bool defer, bool noalloc)
 {
     if (launch_deferred_scatter()) return -1;
+    if (noalloc) {
+        // only copies and kernels: no buffer may grow
+        if (nj > E.n_addr)
+            return fail(-1, "j-memory holds %d addresses, not %d (set them first)", E.n_addr, nj);
+        if (E.n_addr > E.dcap)
+            return fail(-1, "j-memory is not resident on the device yet (any force call, "
+                            "sweep or g6x_autotune after the last new address places it)");
+    }
     if (nj > E.n_addr) {
         // addresses never set are zero-mass particles at the origin
         if (grow_host_mirror(nj)) return -1;
@@ -261,7 +269,9 @@ int flush_j(int nj, bool defer)
     if (ensure_device_j(E.n_addr)) return -1;
     const int nd = (int)E.dirty.size();
     if (nd == 0) return 0;
-    if ((size_t)nd * 4 >= (size_t)E.n_addr) {
+    // without allocation a dirty list longer than the staging buffer goes
+    // up as one copy of the whole used range
+    if ((size_t)nd * 4 >= (size_t)E.n_addr || (noalloc && nd > E.scap)) {
         HIPCHK(hipMemcpyAsync(E.d_jraw, E.h_jraw, sizeof(JRaw) * (size_t)E.n_addr,
                               hipMemcpyHostToDevice, E.stream));
         HIPCHK(hipMemcpyAsync(E.d_jindex, E.h_jindex, sizeof(int) * (size_t)E.n_addr,
@@ -405,11 +415,11 @@ int ensure_partials(size_t bytes)
     return 0;
 }
 
-Partials partials_at(int ld, int nsplit)
+Partials partials_at(int ld, int nsplit, char *base)
 {
     Partials p;
     const size_t n = (size_t)ld * nsplit;
-    double *d = reinterpret_cast<double *>(E.d_part);
+    double *d = reinterpret_cast<double *>(base ? base : E.d_part);
     for (int c = 0; c < 7; ++c) p.c[c] = d + c * n;
     float *u = reinterpret_cast<float *>(d + 7 * n);
     p.best = u;
@@ -519,7 +529,7 @@ Geometry choose_geometry(int ni, int nj)
 // Enqueue force + reduction (with the nearest-neighbour refinement) for an
 // i-block already in d_ibuf.
 int enqueue_force(int ni, int nj, double eps2d, const IBlock &ib, const Results &r, bool zskip,
-                  const FusedPredict *fused)
+                  const FusedPredict *fused, char *part)
 {
     float eps2 = (float)eps2d;
     if (zskip && !(eps2 >= __FLT_MIN__)) {   // as launch_force treats it
@@ -528,8 +538,8 @@ int enqueue_force(int ni, int nj, double eps2d, const IBlock &ib, const Results 
     }
     const Geometry g = choose_geometry(ni, nj);
     const int ld = ib.ld;
-    if (ensure_partials(partials_bytes(ld, g.npart))) return -1;
-    Partials p = partials_at(ld, g.npart);
+    if (!part && ensure_partials(partials_bytes(ld, g.npart))) return -1;
+    Partials p = partials_at(ld, g.npart, part);
     E.geom_nsplit = g.nsplit;
     E.geom_jchunk = g.jchunk;
     E.geom_tile = g.tile;
@@ -983,6 +993,69 @@ int self_block(int nj, int i_first, int ni, IBlock &ib)
     ib.vx64 = E.d_vp64 + i_first;
     ib.vy64 = E.d_vp64 + (size_t)E.dcap + i_first;
     ib.vz64 = E.d_vp64 + 2 * (size_t)E.dcap + i_first;
+    return 0;
+}
+
+// Workspace of self_sweep_into: the i-block's ten FP32/index arrays
+// (ld * 40 B), the refinement flags (ld ints) and the j-split partial sums of
+// the force geometry for (ni, nj), each start aligned to 256 bytes.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t sweep_workspace_bytes(int nj, int ni)
+{
+    const int ld = (ni + 63) & ~63;
+    const Geometry g = choose_geometry(ni, nj);
+    return align256((size_t)ld * 40) + align256((size_t)ld * sizeof(int)) +
+           partials_bytes(ld, g.npart);
+}
+
+int self_sweep_into(const char *who, int nj, int i_first, int ni, double ti, double eps2,
+                    const Results &out, char *ws, size_t ws_bytes)
+{
+    if (ensure_open(who)) return -1;
+    if (ni <= 0 || i_first < 0 || i_first + ni > nj || nj <= 0)
+        return fail(-1, "%s: bad range i_first=%d ni=%d nj=%d", who, i_first, ni, nj);
+    const size_t need = sweep_workspace_bytes(nj, ni);
+    if (need > ws_bytes)
+        return fail(-1, "%s: ni = %d, nj = %d needs %zu bytes of scratch, more than the %zu "
+                        "it has", who, ni, nj, need, ws_bytes);
+    if (E.calc_pending) HIPCHK(hipStreamSynchronize(E.stream));
+    E.calc_pending = false;
+    E.nb_valid = false;
+    E.calc_failed = true;      // no neighbour lists for sweeps
+    E.sweep_results = false;   // the results went to the caller's arrays
+    // event timers would create events: off for this call
+    const int timing = E.opt.timing;
+    E.opt.timing = 0;
+    E.ti = ti;
+    int rc = flush_j(nj, false, true);
+    if (rc == 0) rc = predict(nj, true);
+    if (rc == 0) {
+        const int ld = (ni + 63) & ~63;
+        float *ibuf = reinterpret_cast<float *>(ws);
+        hipError_t e = launch_iprep_self(E.stream, E.d_jpred, E.d_xp64, E.dcap, i_first, ni,
+                                         ibuf, ld);
+        if (e != hipSuccess) {
+            rc = fail(-1, "%s: %s", who, hipGetErrorString(e));
+        } else {
+            IBlock ib = iblock_at(ws, ld);
+            ib.x64 = E.d_xp64 + i_first;
+            ib.y64 = E.d_xp64 + (size_t)E.dcap + i_first;
+            ib.z64 = E.d_xp64 + 2 * (size_t)E.dcap + i_first;
+            ib.vx64 = E.d_vp64 + i_first;
+            ib.vy64 = E.d_vp64 + (size_t)E.dcap + i_first;
+            ib.vz64 = E.d_vp64 + 2 * (size_t)E.dcap + i_first;
+            char *flags = ws + align256((size_t)ld * 40);
+            char *part = flags + align256((size_t)ld * sizeof(int));
+            Results r = out;
+            r.flags = reinterpret_cast<int *>(flags);
+            rc = enqueue_force(ni, nj, eps2, ib, r, false, nullptr, part);
+        }
+    }
+    E.opt.timing = timing;
+    if (rc) return rc;
+    E.last_ni = ni;
+    E.last_nj = nj;
     return 0;
 }
 

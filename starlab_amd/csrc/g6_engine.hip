@@ -26,6 +26,37 @@ const char *env_prefix() { return "G6_"; }
 
 namespace {
 
+// Scratch of g6x_self_sweep_noalloc: a fixed device array, so that call
+// allocates nothing.  Each GPU the library runs on holds its own copy until
+// the process ends.  Host code must not pass `scratch` itself to a kernel
+// (on the host it only names the symbol): scratch_dev[device] is the
+// device address, looked up once per device when the library opens it.
+constexpr size_t kScratchBytes = (size_t)48 << 20;   // the default geometry needs <= 36.5 MiB for ni <= 65536
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+char *scratch_dev[kMaxDevices] = {};
+
+int setup_scratch()
+{
+    const int dev = E.device;
+    if (dev < 0 || dev >= kMaxDevices) return 0;   // the entry point refuses on such a device
+    if (scratch_dev[dev]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(-1, "open: no device address for the static scratch on device %d (%s)", dev,
+                    hipGetErrorString(e));
+    scratch_dev[dev] = static_cast<char *>(p);
+    return 0;
+}
+
+int open_device(int dev)
+{
+    const int rc = do_open(dev);
+    if (rc) return rc;
+    return setup_scratch();
+}
+
 int firsthalf(int nj, int ni, const int *index, const double (*xi)[3],
               const double (*vi)[3], double eps2, const double *h2)
 {
@@ -76,7 +107,7 @@ void g6_set_xunit_(int *) {}
 int g6_open_(int *)
 {
     read_env();
-    return do_open(E.opt.device);
+    return open_device(E.opt.device);
 }
 
 int g6_close_(int *) { return do_close(); }
@@ -84,7 +115,7 @@ int g6_close_(int *) { return do_close(); }
 int g6_open_special(int ngpu, int *list)
 {
     read_env();
-    return do_open(ngpu > 0 && list ? list[0] : E.opt.device);
+    return open_device(ngpu > 0 && list ? list[0] : E.opt.device);
 }
 
 int g6_open_special2(int ngpu, int *list, int)
@@ -243,6 +274,23 @@ int g6x_read_predicted(int, int nj, double *xp, double *vp)
             for (int c = 0; c < 3; ++c) dst[3 * (size_t)j + c] = buf[(size_t)c * nj + j];
     }
     return 0;
+}
+
+int g6x_self_sweep_noalloc(int, int nj, int i_first, int ni, double ti, double eps2,
+                           double *acc, double *jerk, double *pot, int *nnbindex)
+{
+    const char *who = "g6x_self_sweep_noalloc";
+    if (ensure_open(who)) return -1;
+    if (!acc || !jerk || !pot || !nnbindex) return fail(-1, "%s: null output array", who);
+    if (E.device < 0 || E.device >= kMaxDevices || !scratch_dev[E.device])
+        return fail(-1, "%s: no static scratch on device %d", who, E.device);
+    Results out{};
+    out.acc = acc;
+    out.jerk = jerk;
+    out.pot = pot;
+    out.nn = nnbindex;
+    return self_sweep_into(who, nj, i_first, ni, ti, eps2, out, scratch_dev[E.device],
+                           kScratchBytes);
 }
 
 int g6x_sync(int)
