@@ -82,12 +82,15 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * per system (256 threads up to wg_small_max = 512, else 1024), up to this
  * many steps per launch, bitwise equal to the step graphs; default 0 = step
  * graphs, which measured faster on mixed cluster batches), "wg_small_max",
- * "wg_max_n" (<= 2048; fixed while open); read-only "medium_steps", and the
+ * "wg_max_n" (<= 2048); "light_max", "fused_steps", "wg_steps", "max_split",
+ * "wg_small_max" and "wg_max_n" are fixed while open (set them before
+ * g6e_open: they decide the lane each resident system steps in); read-only
+ * "medium_steps", and the
  * host time of g6e_run since g6e_open in microseconds by phase:
  * "host_us_retire", "host_us_admit", "host_us_rebuild", "host_us_graphs",
  * "host_us_lanes" (launching and waiting for the lanes), "host_us_phase".
  * Options are library-wide and keep their values across g6e_close.
- * Returns 0, or -1 for an unknown key. */
+ * Returns 0, or -1 for an unknown key or an option fixed while open. */
 int       g6e_set_option(const char *key, long long value);
 long long g6e_get_option(const char *key);
 

@@ -66,7 +66,10 @@ int  g6_npipes_(void);
 void g6_set_ti_(int *cluster_id, double *ti);
 
 /* Store j-particle `address`: tj, dtj (unused), mass, k18 = a''/18,
- * j6 = jerk/6, a2 = acc/2, v, x.  Reference: grape6.h:39-49. */
+ * j6 = jerk/6, a2 = acc/2, v, x.  Reference: grape6.h:39-49.
+ * A block in flight keeps the j-memory it was started with: a call made
+ * between g6calc_firsthalf_ and g6calc_lasthalf2_ (kira makes none,
+ * hdyn_grape6.C:675-689) waits until that block's device work is done. */
 int  g6_set_j_particle_(int *cluster_id, int *address, int *index,
                         double *tj, double *dtj, double *mass,
                         double k18[3], double j6[3], double a2[3],

@@ -300,7 +300,9 @@ int flush_j(int nj, bool defer, bool noalloc)
     }
     for (int a : E.dirty) E.dirty_flag[a] = 0;
     E.dirty.clear();
+    // the device j-memory changed: neither prediction describes it any more
     E.pred_valid = false;
+    E.xp_valid = false;
     return 0;
 }
 

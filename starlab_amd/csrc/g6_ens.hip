@@ -1644,8 +1644,10 @@ int g6e_run(int steps)
         ECHK(hipStreamSynchronize(X.stream));
         X.host_s[5] += since(t0);
         X.dev_seconds += ms_total * 1e-3;
-        const int round_steps = heavy ? X.steps_per_graph
-                                      : (med[2] || med[3]) ? X.wg_steps : X.steps_per_graph;
+        // every round counts at least one step, so g6e_run(steps) ends
+        const int round_steps = std::max(1, heavy ? X.steps_per_graph
+                                            : (med[2] || med[3]) ? X.wg_steps
+                                                                 : X.steps_per_graph);
         X.engine_steps += round_steps;
         done += round_steps;
     }
@@ -1701,9 +1703,15 @@ int g6e_set_option(const char *key, long long value)
     } else if (k == "lookahead") X.lookahead = (int)std::max(1LL, value);
     else if (k == "dt_max_bits") X.dt_max = std::ldexp(1.0, (int)value);
     else if (k == "max_steps") X.max_steps = std::max(1LL, value);
-    else if (k == "light_max") X.light_max = (int)std::max(0LL, value);
-    else if (k == "fused_steps") X.fused_steps = (int)std::max(0LL, value);
-    else if (k == "wg_steps") X.wg_steps = (int)std::max(0LL, value);
+    else if (k == "light_max" || k == "fused_steps" || k == "wg_steps") {
+        // they decide which lane and kernel each resident system steps in:
+        // changed under resident systems, a lane could stop advancing
+        if (X.open) return efail(-1, "%s is fixed while open", k.c_str());
+        const int v = (int)std::max(0LL, value);
+        if (k == "light_max") X.light_max = v;
+        else if (k == "fused_steps") X.fused_steps = v;
+        else X.wg_steps = v;
+    }
     else if (k == "wg_small_max" || k == "wg_max_n") {
         // records of 64 B per particle must fit the 160 KB of LDS
         if (X.open) return efail(-1, "%s is fixed while open", k.c_str());

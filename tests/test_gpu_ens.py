@@ -239,3 +239,28 @@ def test_failed_open_releases_and_bad_arguments_are_errors():
         assert res[jid]["t"] == job["t_end"] and not res[jid]["failed"]
     finally:
         eng.close()
+
+
+def test_lane_options_fixed_while_open():
+    """light_max, fused_steps and wg_steps decide which lane and kernel a
+    resident system steps in: refused while an engine is open (a change
+    under resident systems could leave a lane that never advances), taken
+    before g6e_open, and a run with them set up front finishes."""
+    from starlab_amd import load
+    L = load()
+    e = Ensemble(GB, 16, **DEFAULTS)
+    try:
+        for k in ("light_max", "fused_steps", "wg_steps"):
+            before = L.g6e_get_option(k.encode())
+            assert L.g6e_set_option(k.encode(), 0) == -1
+            assert b"fixed while open" in L.g6e_last_error()
+            assert L.g6e_get_option(k.encode()) == before
+        j = _jobs(5, [96], 1.0 / 64)[0]
+        jid = e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"])
+        res = e.run_until_idle()
+        assert jid in res and res[jid]["t"] == j["t_end"] and not res[jid]["failed"]
+    finally:
+        e.close()
+        for k, v in DEFAULTS.items():
+            e.option(k, v)
+    assert L.g6e_set_option(b"wg_steps", 0) == 0
