@@ -113,6 +113,15 @@ int g6x_sync(int cluster_id);
  *                    softening predict j inside the force kernel (no
  *                    predictor launch, no JPred round trip; bitwise the
  *                    same results); 0 = predictor first   G6_FUSED_SMALL
+ *   "fail_lasthalf", "fail_read_nl", "fail_nn"   host-side failure injection
+ *                    for a caller's recovery path (kira resets, reloads and
+ *                    retries, hdyn_grape6.C:405-438, 853-900, 2110-2166):
+ *                    the next k g6calc_lasthalf(2)_ calls return 1 (results
+ *                    withheld), the next k g6_read_neighbour_list_ calls
+ *                    return -1, the next k g6calc_lasthalf2_ calls return 0
+ *                    with nnbindex[0] = -1 (kira's error 42).  The device
+ *                    work runs normally; each counts down to 0.
+ *                    G6_FAIL_LASTHALF, G6_FAIL_READNL, G6_FAIL_NN
  * Returns 0, or -1 for an unknown key. */
 int       g6x_set_option(const char *key, long long value);
 long long g6x_get_option(const char *key);

@@ -241,6 +241,37 @@ class _JMem(_C.Structure):
 
 _OBSERVER = _C.CFUNCTYPE(_C.c_int, _C.c_void_p, _C.POINTER(_JMem), _C.c_double, _C.c_int,
                          _IP, _DP, _DP, _DP, _DP, _DP, _IP)
+
+
+class _JSent(_C.Structure):
+    _fields_ = [("n", _C.c_int), ("t", _DP), ("dt", _DP), ("m", _DP), ("x", _DP), ("v", _DP),
+                ("a2", _DP), ("j6", _DP), ("k18", _DP)]
+
+
+class _Call(_C.Structure):
+    _fields_ = [("kind", _C.c_int), ("ti", _C.c_double), ("nj", _C.c_int), ("ni", _C.c_int),
+                ("index", _IP), ("xi", _DP), ("vi", _DP), ("h2", _DP), ("rc", _C.c_int),
+                ("acc", _DP), ("jerk", _DP), ("pot", _DP), ("nn", _IP),
+                ("read_status", _C.c_int), ("pipe", _C.c_int), ("max_length", _C.c_int),
+                ("nblen", _C.c_int), ("nbl", _IP)]
+
+
+class _KiraOpts(_C.Structure):
+    _fields_ = [("star_radius", _C.c_double), ("coll_freq", _C.c_int), ("pert_freq", _C.c_int),
+                ("gamma", _C.c_double), ("nbin", _C.c_int), ("bin_node", _IP),
+                ("bin_m1", _DP), ("bin_dx", _DP), ("bin_dv", _DP), ("dt_log", _C.c_double),
+                ("densities", _C.c_int), ("release_every", _C.c_int), ("e_log", _DP),
+                ("e_log_cap", _C.c_int), ("density", _DP)]
+
+
+_CALL_FN = _C.CFUNCTYPE(_C.c_int, _C.c_void_p, _C.POINTER(_JSent), _C.POINTER(_Call))
+# kinds of bs_call (include/kira_blockstep.h)
+CALL_BLOCK, CALL_REDO, CALL_PERTURB, CALL_ENERGY, CALL_DENSITY, CALL_LIST = range(6)
+NOT_CALLED = -99
+KIRA_STATS = ("blocks", "particle_steps", "abi_seconds", "seconds", "force_calls", "list_reads",
+              "read_overflows", "list_gets", "get_overflows", "redo_calls", "perturb_calls",
+              "energy_sweeps", "density_rounds", "resets", "reopens", "full_reloads",
+              "nn_from_lists", "perturber_overflows", "error42", "energy_logs")
 LIBKB = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "libkira_blockstep.so")
 _kb = None
 
@@ -256,6 +287,10 @@ def _load_kb():
         L.bs_run.argtypes = [_C.c_char_p, _C.c_int, _DP, _DP, _DP, _C.c_double, _C.c_double,
                              _C.c_double, _C.c_double, _C.c_int, _DP, _DP, _DP, _DP, _DP,
                              _OBSERVER, _C.c_void_p]
+        L.bs_run_kira.restype = _C.c_int
+        L.bs_run_kira.argtypes = [_C.c_char_p, _C.c_int, _DP, _DP, _DP, _C.c_double, _C.c_double,
+                                  _C.c_double, _C.c_double, _C.POINTER(_KiraOpts), _DP, _DP,
+                                  _DP, _DP, _DP, _CALL_FN, _C.c_void_p]
         L.bs_last_error.restype = _C.c_char_p
         _kb = L
     return _kb
@@ -325,4 +360,104 @@ class NativeBlockHermite:
         self.t[:] = t_end
         self.blocks, self.particle_steps = int(stats[0]), int(stats[1])
         self.abi_seconds, self.seconds = float(stats[2]), float(stats[3])
+        return self
+
+
+class KiraCallMix:
+    """kira's whole GRAPE-6 call mix in native code (bs_run_kira,
+    include/kira_blockstep.h): the block-step integration of NativeBlockHermite
+    plus neighbour radii and list rounds, perturbation calls of binary centre-
+    of-mass nodes, energy sweeps and density rounds at log times, release and
+    re-attach, and reset-and-retry on library errors, each as kira's GRAPE-6
+    driver (hdyn_grape6.C) makes it.
+
+    binaries: None or dict(node=[k], m1=[...], dx=[[...]], dv=[[...]]) -- node
+    k is the centre of mass of a binary with component masses m1 and m[k] - m1
+    and relative position / velocity dx, dv (component 2 minus component 1).
+
+    observer(jmem, call) is called for every ABI call (numpy views valid only
+    during the call): jmem = dict(t, dt, m, x, v, a2, j6, k18) as sent, call =
+    dict(kind, ti, nj, index, xi, vi, h2, rc, acc, jerk, pot, nn,
+    read_status, pipe, max_length, nbl); a true return stops the run."""
+
+    def __init__(self, m, x, v, eta=0.1, eps2=0.0, dt_max=0.125):
+        self.n = len(m)
+        self.m = np.ascontiguousarray(m, dtype=np.float64).copy()
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3).copy()
+        self.v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3).copy()
+        self.a = np.zeros_like(self.x)
+        self.j = np.zeros_like(self.x)
+        self.pot = np.zeros(self.n)
+        self.dt = np.zeros(self.n)
+        self.t = np.zeros(self.n)
+        self.density = np.zeros(self.n)
+        self.eta, self.eps2, self.dt_max = eta, eps2, dt_max
+        self.stats = {}
+        self.energies = np.zeros(0)
+
+    def run(self, t_end, grape_lib=None, star_radius=0.0, binaries=None, dt_log=0.0,
+            densities=False, release_every=0, coll_freq=0, pert_freq=0, gamma=0.0,
+            observer=None):
+        L = _load_kb()
+        if grape_lib is None:
+            from . import LIBG6
+            grape_lib = LIBG6
+        keep = []
+        o = _KiraOpts()
+        o.star_radius, o.coll_freq, o.pert_freq, o.gamma = star_radius, coll_freq, pert_freq, gamma
+        if binaries is not None and len(binaries["node"]):
+            node = np.ascontiguousarray(binaries["node"], dtype=np.int32)
+            m1 = np.ascontiguousarray(binaries["m1"], dtype=np.float64)
+            dx = np.ascontiguousarray(binaries["dx"], dtype=np.float64).reshape(-1, 3)
+            dv = np.ascontiguousarray(binaries["dv"], dtype=np.float64).reshape(-1, 3)
+            keep += [node, m1, dx, dv]
+            o.nbin = len(node)
+            o.bin_node = node.ctypes.data_as(_IP)
+            o.bin_m1, o.bin_dx, o.bin_dv = _dp(m1), _dp(dx), _dp(dv)
+        o.dt_log, o.densities, o.release_every = dt_log, int(bool(densities)), release_every
+        cap = int(t_end / dt_log) + 2 if dt_log > 0 else 0
+        elog = np.zeros(max(cap, 1))
+        o.e_log, o.e_log_cap = _dp(elog), cap
+        o.density = _dp(self.density)
+        cb = _CALL_FN()
+        if observer is not None:
+            def _cb(_user, jm, call):
+                J, c = jm.contents, call.contents
+                n, ni = J.n, c.ni
+                view = lambda p, *shape: np.ctypeslib.as_array(p, shape) if shape[0] else None
+                jmem = dict(t=view(J.t, n), dt=view(J.dt, n), m=view(J.m, n), x=view(J.x, n, 3),
+                            v=view(J.v, n, 3), a2=view(J.a2, n, 3), j6=view(J.j6, n, 3),
+                            k18=view(J.k18, n, 3))
+                d = dict(kind=c.kind, ti=c.ti, nj=c.nj, rc=c.rc, read_status=c.read_status,
+                         index=view(c.index, ni), xi=view(c.xi, ni, 3), vi=view(c.vi, ni, 3),
+                         h2=view(c.h2, ni))
+                if c.kind == CALL_LIST:
+                    d.update(pipe=c.pipe, max_length=c.max_length,
+                             nbl=np.ctypeslib.as_array(c.nbl, (max(c.nblen, 1),))[:c.nblen])
+                elif ni and c.rc == 0:
+                    d.update(acc=view(c.acc, ni, 3), jerk=view(c.jerk, ni, 3),
+                             pot=view(c.pot, ni), nn=view(c.nn, ni))
+                try:
+                    stop = observer(jmem, d)
+                except Exception as e:          # surface it after the run
+                    self._observer_error = e
+                    return 1
+                return 1 if stop else 0
+            cb = _CALL_FN(_cb)
+        self._observer_error = None
+        stats = np.zeros(len(KIRA_STATS))
+        rc = L.bs_run_kira(grape_lib.encode(), self.n, _dp(self.m), _dp(self.x), _dp(self.v),
+                           float(t_end), float(self.eta), float(self.eps2), float(self.dt_max),
+                           _C.byref(o), _dp(self.a), _dp(self.j), _dp(self.pot), _dp(self.dt),
+                           _dp(stats), cb, None)
+        if self._observer_error is not None:
+            raise self._observer_error
+        if rc:
+            raise RuntimeError((L.bs_last_error() or b"").decode())
+        self.t[:] = t_end
+        self.stats = {k: (int(v) if "seconds" not in k else float(v))
+                      for k, v in zip(KIRA_STATS, stats)}
+        self.blocks, self.particle_steps = self.stats["blocks"], self.stats["particle_steps"]
+        self.abi_seconds, self.seconds = self.stats["abi_seconds"], self.stats["seconds"]
+        self.energies = elog[:min(cap, self.stats["energy_logs"])].copy()
         return self

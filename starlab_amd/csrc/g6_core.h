@@ -47,6 +47,12 @@ struct Options {
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every
                               // size so far, so off unless asked for)
+    // Host-side failure injection for callers' recovery paths (kira's reset
+    // and retry): the next k calls of the named kind report an error; the
+    // device work itself runs normally.
+    int fail_lasthalf = 0;    // lasthalf returns 1 (results withheld)
+    int fail_read_nl = 0;     // g6_read_neighbour_list_ returns -1
+    int fail_nn = 0;          // lasthalf2 returns 0 with nnbindex[0] = -1
 };
 
 struct TimedPair { int kernel; hipEvent_t a, b; };

@@ -73,6 +73,9 @@ void read_env()
     o.reduce_lanes = (int)env_ll("REDUCE_LANES", o.reduce_lanes);
     o.wpg = (int)env_ll("WPG", o.wpg);
     o.fused_small = (int)env_ll("FUSED_SMALL", o.fused_small);
+    o.fail_lasthalf = (int)env_ll("FAIL_LASTHALF", o.fail_lasthalf);
+    o.fail_read_nl = (int)env_ll("FAIL_READNL", o.fail_read_nl);
+    o.fail_nn = (int)env_ll("FAIL_NN", o.fail_nn);
     if (o.npipes < 1) o.npipes = 1;
     if (o.nb_cap < 1) o.nb_cap = 1;
 }

@@ -81,12 +81,29 @@ int lasthalf(int ni, double (*acc)[3], double (*jerk)[3], double *pot, int *nn)
 {
     const Results *h = calc_wait("g6calc_lasthalf");
     if (!h) return 1;
+    if (E.opt.fail_lasthalf > 0) {
+        --E.opt.fail_lasthalf;
+        return fail(1, "g6calc_lasthalf: injected failure (option fail_lasthalf)");
+    }
     const int n = std::min(ni, E.last_ni);
     memcpy(acc, h->acc, sizeof(double) * 3 * (size_t)n);
     memcpy(jerk, h->jerk, sizeof(double) * 3 * (size_t)n);
     memcpy(pot, h->pot, sizeof(double) * (size_t)n);
     if (nn) memcpy(nn, h->nn, sizeof(int) * (size_t)n);
+    if (nn && n > 0 && E.opt.fail_nn > 0) {
+        --E.opt.fail_nn;
+        nn[0] = -1;
+    }
     return 0;
+}
+
+int read_nl_checked()
+{
+    if (E.opt.fail_read_nl > 0) {
+        --E.opt.fail_read_nl;
+        return fail(-1, "g6_read_neighbour_list: injected failure (option fail_read_nl)");
+    }
+    return read_nl();
 }
 
 }  // namespace
@@ -170,8 +187,8 @@ int g6calc_lasthalf2_(int *, int *, int *ni, int[], double[][3], double[][3],
     return lasthalf(*ni, acc, jerk, pot, nnbindex);
 }
 
-int g6_read_neighbour_list_(int *) { return read_nl(); }
-int g6_read_neighbour_list_old_(int *) { return read_nl(); }
+int g6_read_neighbour_list_(int *) { return read_nl_checked(); }
+int g6_read_neighbour_list_old_(int *) { return read_nl_checked(); }
 
 int g6_get_neighbour_list_(int *, int *pipe, int *max_length, int *nblen, int nbl[])
 {
@@ -318,6 +335,9 @@ int g6x_set_option(const char *key, long long value)
     else if (k == "lds_min_ni") o.lds_min_ni = (int)value;
     else if (k == "tile") o.tile = (int)value;
     else if (k == "fused_small") o.fused_small = value != 0 ? 1 : 0;
+    else if (k == "fail_lasthalf") o.fail_lasthalf = (int)std::max(0LL, value);
+    else if (k == "fail_read_nl") o.fail_read_nl = (int)std::max(0LL, value);
+    else if (k == "fail_nn") o.fail_nn = (int)std::max(0LL, value);
     else if (k == "wpg") {
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
             return -1;
@@ -348,6 +368,9 @@ long long g6x_get_option(const char *key)
     if (k == "force_variant") return o.force_variant;
     if (k == "lds_min_ni") return o.lds_min_ni;
     if (k == "tile") return o.tile;
+    if (k == "fail_lasthalf") return o.fail_lasthalf;
+    if (k == "fail_read_nl") return o.fail_read_nl;
+    if (k == "fail_nn") return o.fail_nn;
     if (k == "fused_small") return o.fused_small;
     if (k == "last_tile") return E.geom_tile;
     if (k == "reduce_lanes") return o.reduce_lanes;

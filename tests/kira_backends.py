@@ -182,3 +182,62 @@ class NativeChecker(_ErrorBook):
         self._book(pred, None, jm["m"].copy(), jidx, t, idx.copy(), xi.copy(), vi.copy(),
                    acc.copy(), jerk.copy(), pot.copy(), nn.copy(), self.eps2)
         return False
+
+
+class CallMixChecker(_ErrorBook):
+    """Observer for KiraCallMix (bs_run_kira): every `every`-th force call of
+    any kind (blocks, single-node redos, perturbation pairs, energy sweeps,
+    density rounds) against the FP64 oracle on the j-memory exactly as sent;
+    every `list_every`-th neighbour list against the brute-force FP64 set with
+    the library's conventions (the i itself included when npipes > 48,
+    overflow iff the set has max_length or more entries, then the first
+    max_length addresses).  Counts calls by kind, list reads by status, and
+    calls whose return code or nn flagged an (injected) error."""
+
+    def __init__(self, eps2, npipes, every=1, max_i=None, seed=0, list_every=1):
+        from collections import Counter
+        self.eps2 = eps2
+        self.include_self = npipes > 48
+        self.kinds = Counter()
+        self.read_status = Counter()
+        self.errors = 0
+        self.list_checked = 0
+        self.list_mismatch = []
+        self.list_every = list_every
+        self.lists_seen = 0
+        self._book_init(every, max_i, seed)
+
+    def _pred(self, jm, ti):
+        return oracle.predict(jm["x"], jm["v"], jm["a2"], jm["j6"], jm["k18"], jm["t"], ti, 4)
+
+    def __call__(self, jm, c):
+        from starlab_amd.blockstep import CALL_LIST, NOT_CALLED
+        if c["read_status"] != NOT_CALLED:
+            self.read_status[c["read_status"]] += 1
+            return False
+        self.kinds[c["kind"]] += 1
+        n = len(jm["m"])
+        jidx = np.arange(n, dtype=np.int32)
+        if c["kind"] == CALL_LIST:
+            self.lists_seen += 1
+            if (self.lists_seen - 1) % self.list_every:
+                return False
+            xp, _ = self._pred(jm, c["ti"])
+            mx = c["max_length"]
+            counts, lists = oracle.neighbours(c["xi"].copy(), c["h2"].copy(), c["index"].copy(),
+                                              self.include_self, xp, jidx, mx)
+            cnt = int(counts[0])
+            want_rc = 1 if cnt >= mx else 0
+            want = lists[0][:min(cnt, mx)]
+            got = c["nbl"]
+            self.list_checked += 1
+            if c["rc"] != want_rc or len(got) != len(want) or not np.array_equal(got, want):
+                self.list_mismatch.append((c["ti"], int(c["index"][0]), cnt, c["rc"], len(got)))
+            return False
+        if c["rc"] != 0 or np.any(c["nn"] < 0):
+            self.errors += 1
+            return False
+        self._book(lambda: self._pred(jm, c["ti"]), None, jm["m"].copy(), jidx, c["ti"],
+                   c["index"].copy(), c["xi"].copy(), c["vi"].copy(), c["acc"].copy(),
+                   c["jerk"].copy(), c["pot"].copy(), c["nn"].copy(), self.eps2)
+        return False

@@ -6,8 +6,19 @@
  * machine without a GPU, against forces that the Python driver can reproduce
  * exactly: j-memory holds what g6_set_j_particle_ receives, firsthalf
  * predicts every j with or_predict (4th order, the GRAPE form) and sums with
- * or_force; lasthalf2 returns the stored results.  Only the entry points the
- * driver needs are provided.
+ * or_force; lasthalf2 returns the stored results.  Neighbour lists are the
+ * brute-force FP64 sets of or_neighbours with libg6's conventions (the i
+ * itself included when npipes > 48, at most NB_CAP entries stored per i
+ * (8192 as libg6's nb_cap, or ORACLE_G6ABI_NB_CAP);
+ * read returns 1 when a list exceeds that, get returns 1 with nblen =
+ * max_length when the list has max_length or more entries).  Resets and
+ * re-opens keep nothing on the "device" side that the driver relies on.
+ * Failure injection for the driver's recovery path, read once from the
+ * environment: ORACLE_G6ABI_FAIL_LASTHALF=k makes the next k lasthalf2 calls
+ * return 1, ORACLE_G6ABI_FAIL_READNL=k the next k read_neighbour_list calls
+ * return -1, ORACLE_G6ABI_FAIL_NN=k the next k lasthalf2 calls report
+ * nn = -1 for slot 0 (kira's error 42).  Only the entry points the driver
+ * needs are provided.
  */
 #include "grape6.h"
 #include "g6_oracle.h"
@@ -19,9 +30,26 @@ static int cap, npipes = 64;
 static double *X, *V, *A2, *J6, *K18, *T, *M;
 static int *IDX;
 static double ti_now;
-static int last_ni;
+static int last_ni, last_nj;
 static double *r_acc, *r_jerk, *r_pot, *xp, *vp;
 static int *r_nn;
+static double *xi_now, *h2_now;
+static int *excl_now;
+static int NB_CAP = 8192;   /* ORACLE_G6ABI_NB_CAP */
+static int *nb_counts, *nb_lists;
+static int nb_valid, fail_lasthalf, fail_readnl, fail_nn, env_read;
+
+static void read_env(void)
+{
+    if (env_read) return;
+    env_read = 1;
+    const char *e;
+    if ((e = getenv("ORACLE_G6ABI_NPIPES"))) npipes = atoi(e);
+    if ((e = getenv("ORACLE_G6ABI_FAIL_LASTHALF"))) fail_lasthalf = atoi(e);
+    if ((e = getenv("ORACLE_G6ABI_FAIL_READNL"))) fail_readnl = atoi(e);
+    if ((e = getenv("ORACLE_G6ABI_FAIL_NN"))) fail_nn = atoi(e);
+    if ((e = getenv("ORACLE_G6ABI_NB_CAP")) && atoi(e) > 0) NB_CAP = atoi(e);
+}
 
 static void grow(int need)
 {
@@ -45,18 +73,31 @@ static void grow(int need)
 int g6_open_(int *c)
 {
     (void)c;
-    const char *e = getenv("ORACLE_G6ABI_NPIPES");
-    if (e) npipes = atoi(e);
+    read_env();
     return 0;
 }
 
 int g6_close_(int *c)
 {
     (void)c;
+    nb_valid = 0;
     return 0;
 }
 
-int g6_npipes_(void) { return npipes; }
+int g6_reset_(int *c)
+{
+    (void)c;
+    nb_valid = 0;
+    return 0;
+}
+
+int g6_reset_fofpga_(int *c) { return g6_reset_(c); }
+
+int g6_npipes_(void)
+{
+    read_env();
+    return npipes;
+}
 
 void g6_set_ti_(int *c, double *ti)
 {
@@ -88,7 +129,7 @@ void g6calc_firsthalf_(int *c, int *nj, int *ni, int index[], double xi[][3],
                        double vi[][3], double aold[][3], double j6old[][3],
                        double phiold[], double *eps2, double h2[])
 {
-    (void)c; (void)aold; (void)j6old; (void)phiold; (void)h2;
+    (void)c; (void)aold; (void)j6old; (void)phiold;
     const int n = *nj, k = *ni;
     grow(n > k ? n : k);
     or_predict(n, X, V, A2, J6, K18, T, ti_now, 4, xp, vp);
@@ -101,6 +142,14 @@ void g6calc_firsthalf_(int *c, int *nj, int *ni, int index[], double xi[][3],
              r_nn, d2);
     free(d2);
     last_ni = k;
+    last_nj = n;
+    xi_now = realloc(xi_now, sizeof(double) * 3 * k);
+    h2_now = realloc(h2_now, sizeof(double) * k);
+    excl_now = realloc(excl_now, sizeof(int) * k);
+    memcpy(xi_now, &xi[0][0], sizeof(double) * 3 * k);
+    memcpy(excl_now, index, sizeof(int) * k);
+    for (int q = 0; q < k; ++q) h2_now[q] = h2 ? h2[q] : 0.0;
+    nb_valid = 0;
 }
 
 int g6calc_lasthalf2_(int *c, int *nj, int *ni, int index[], double xi[][3], double vi[][3],
@@ -109,9 +158,59 @@ int g6calc_lasthalf2_(int *c, int *nj, int *ni, int index[], double xi[][3], dou
 {
     (void)c; (void)nj; (void)index; (void)xi; (void)vi; (void)eps2; (void)h2;
     if (*ni != last_ni) return 1;
+    if (fail_lasthalf > 0) {
+        --fail_lasthalf;
+        return 1;
+    }
     memcpy(&acc[0][0], r_acc, sizeof(double) * 3 * last_ni);
     memcpy(&jerk[0][0], r_jerk, sizeof(double) * 3 * last_ni);
     memcpy(pot, r_pot, sizeof(double) * last_ni);
     memcpy(nnbindex, r_nn, sizeof(int) * last_ni);
+    if (fail_nn > 0 && last_ni > 0) {
+        --fail_nn;
+        nnbindex[0] = -1;
+    }
     return 0;
+}
+
+int g6_read_neighbour_list_(int *c)
+{
+    (void)c;
+    if (fail_readnl > 0) {
+        --fail_readnl;
+        return -1;
+    }
+    const int k = last_ni;
+    int *ex = malloc(sizeof(int) * (k > 0 ? k : 1));
+    for (int q = 0; q < k; ++q) {   /* the excluded address: the j whose index field is excl */
+        ex[q] = excl_now[q];
+    }
+    nb_counts = realloc(nb_counts, sizeof(int) * (k > 0 ? k : 1));
+    nb_lists = realloc(nb_lists, sizeof(int) * (size_t)NB_CAP * (k > 0 ? k : 1));
+    or_neighbours(k, xi_now, h2_now, ex, npipes > 48, last_nj, xp, IDX, NB_CAP, nb_counts,
+                  nb_lists);
+    free(ex);
+    nb_valid = 1;
+    for (int q = 0; q < k; ++q)
+        if (nb_counts[q] > NB_CAP) return 1;
+    return 0;
+}
+
+int g6_get_neighbour_list_(int *c, int *pipe, int *max_length, int *nblen, int nbl[])
+{
+    (void)c;
+    if (!nb_valid || *pipe < 0 || *pipe >= last_ni) {
+        *nblen = 0;
+        return -1;
+    }
+    const int cnt = nb_counts[*pipe];
+    const int stored = cnt < NB_CAP ? cnt : NB_CAP;
+    const int n = stored < *max_length ? stored : *max_length;
+    if (n > 0) memcpy(nbl, nb_lists + (size_t)NB_CAP * *pipe, sizeof(int) * n);
+    if (cnt >= *max_length) {
+        *nblen = *max_length;
+        return 1;
+    }
+    *nblen = cnt;
+    return cnt > NB_CAP ? 1 : 0;
 }
