@@ -10,8 +10,11 @@ reduction + FP64 nearest-neighbour refinement, inputs resident in HBM.
 Prints one JSON line: pair interactions per second (N*(N-1)*steps / time),
 the roofline of the force kernel against the FP32 vector peak (60 flop per
 pair interaction), the ensemble legs (batched clusters; a served Poisson
-stream with p50/p99 latency) and the CPU direct-sum baseline measured on
-this host.
+stream with p50/p99 latency), kira end to end (plain and with kira's whole
+GRAPE-6 call mix), the CPU direct-sum baselines measured on this host, and
+the error against FP64 for every configuration (error_vs_fp64).  The GPU
+legs keep samples of their inputs and results; the FP64 references are
+computed by the CPU leg, outside every timed region.
 """
 from __future__ import annotations
 
@@ -62,30 +65,137 @@ def make_input(g, n):
     return dict(m=m, x=x, v=v, a2=acc / 2.0, j6=jerk / 6.0, tj=tj, idx=idx)
 
 
-def cpu_baseline(inp, target_s=10.0):
-    """The oracle's FP64 direct sum (OpenMP over i, all host threads) on a
-    bounded sample of the same workload: i-blocks of the first particles
-    against all N j (predicted on the host), repeated until ~target_s."""
+def _pct(a):
+    a = np.asarray(a, dtype=np.float64)
+    if a.size == 0:
+        return None
+    return {"median": float(np.median(a)), "p99": float(np.percentile(a, 99)),
+            "max": float(a.max())}
+
+
+def fp64_errors(samples, eps2):
+    """Error of kept GPU results against the FP64 oracle on the same inputs:
+    per-particle |da|/|a|, |dj|/|j|, |dpot/pot| percentiles, the normwise
+    acc / jerk error (|da| over sum_j |a_ij|), nearest-neighbour mismatches.
+    Each sample: the j-memory as the library held it (t, m, x, v, acc, jerk,
+    k18; predicted here to t), the i inputs and the library's results."""
     import oracle
+    ra_, rj_, rp_, na_, nj_ = [], [], [], [], []
+    mism = 0
+    for s in samples:
+        jm = s["jm"]
+        n = len(jm["m"])
+        jidx = np.arange(n, dtype=np.int32)
+        a2 = jm["a2"] if "a2" in jm else jm["acc"] / 2.0     # as sent to the library
+        j6 = jm["j6"] if "j6" in jm else jm["jerk"] / 6.0
+        xp, vp = oracle.predict(jm["x"], jm["v"], a2, j6, jm["k18"], jm["t"], s["t"], 4)
+        racc, rjerk, rpot, rnn, _ = oracle.force(s["xi"], s["vi"], s["idx"], xp, vp, jm["m"],
+                                                 jidx, eps2)
+        for k, i in enumerate(s["idx"]):
+            d = xp - s["xi"][k]
+            w = vp - s["vi"][k]
+            r2 = (d * d).sum(1) + eps2
+            if 0 <= i < n:
+                r2[i] = np.inf
+            sa = (jm["m"] / r2).sum()
+            sj = (jm["m"] * 4.0 * np.sqrt((w * w).sum(1)) / (r2 * np.sqrt(r2))).sum()
+            da = np.linalg.norm(s["acc"][k] - racc[k])
+            dj = np.linalg.norm(s["jerk"][k] - rjerk[k])
+            ra_.append(da / np.linalg.norm(racc[k]))
+            rj_.append(dj / np.linalg.norm(rjerk[k]))
+            rp_.append(abs(s["pot"][k] - rpot[k]) / abs(rpot[k]))
+            na_.append(da / sa)
+            nj_.append(dj / sj)
+        mism += int((np.asarray(s["nn"]) != rnn).sum())
+    return {"particle_steps": len(ra_), "acc_rel": _pct(ra_), "jerk_rel": _pct(rj_),
+            "pot_rel": _pct(rp_), "acc_normwise": _pct(na_), "jerk_normwise": _pct(nj_),
+            "nn_mismatch": mism}
+
+
+def cpu_leg(inp, gpu, target_s=10.0, clusters=None, kira=None):
+    """CPU side, outside every timed GPU region.
+
+    Baseline (config 2's workload): the FP64 direct sum + nearest neighbour of
+    oracle/g6_oracle.c built for speed (-O3, FMA, OpenMP over i, x86-64-v4 on
+    AVX-512 hosts else v3) on all host threads, on a bounded sample of i
+    against all N j, ~target_s of work; the bit-exact oracle build (-O2, no
+    contraction) is timed beside it.  Config 3: the kira leg's pair
+    interactions at that rate (the CPU force path of kira, one N-body unit).
+    Config 4: a size-stratified sample of the ensemble's clusters integrated
+    with the same shared-step Hermite rules in FP64, one system per host
+    thread.  Errors against FP64 for configs 1-3 from the kept GPU samples."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
     n = len(inp["m"])
     xp, vp = oracle.predict(inp["x"], inp["v"], inp["a2"], inp["j6"], np.zeros((n, 3)),
                             inp["tj"], 0.0, 4)
     idx = inp["idx"]
-    ni = 256
-    t0 = time.perf_counter()
-    oracle.force(xp[:ni], vp[:ni], idx[:ni], xp, vp, inp["m"], idx, EPS2)
-    dt = time.perf_counter() - t0
-    ni = int(min(n, max(256, ni * target_s / max(dt, 1e-6))))
-    reps = 1 if ni < n else max(1, min(8, int(target_s / max(dt * n / 256, 1e-3))))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        oracle.force(xp[:ni], vp[:ni], idx[:ni], xp, vp, inp["m"], idx, EPS2)
-    dt = time.perf_counter() - t0
-    pairs = reps * ni * (n - 1)
-    return {"value": pairs / dt, "unit": "pair interactions/s", "cores": oracle.threads(),
-            "kind": "port", "seconds": round(dt, 3),
-            "sample": f"{reps} x {ni} i-particles x {n} j (FP64 direct sum + nearest "
-                      f"neighbour, oracle/g6_oracle.c, OpenMP over i)"}
+    L, flags = oracle.fast_lib()
+    model, _ = oracle.host_cpu()
+
+    def force_with(lib, ni):
+        acc = np.zeros((ni, 3)); jerk = np.zeros((ni, 3)); pot = np.zeros(ni)
+        nn = np.zeros(ni, np.int32); d2 = np.zeros(ni)
+        lib.or_force(ni, oracle._d(xp[:ni]), oracle._d(vp[:ni]), oracle._i(idx[:ni]), n,
+                     oracle._d(xp), oracle._d(vp), oracle._d(inp["m"]), oracle._i(idx), EPS2,
+                     acc, jerk, pot, nn, d2)
+
+    def rate(lib, budget):
+        t0 = time.perf_counter()
+        force_with(lib, 256)
+        dt = time.perf_counter() - t0
+        ni = int(min(n, max(256, 256 * budget / max(dt, 1e-6))))
+        reps = 1 if ni < n else max(1, min(8, int(budget / max(dt * n / 256, 1e-3))))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            force_with(lib, ni)
+        dt = time.perf_counter() - t0
+        return reps * ni * (n - 1) / dt, reps, ni, dt
+
+    value, reps, ni, dt = rate(L, target_s)
+    strict, _, _, _ = rate(oracle.lib(), target_s / 4)
+    out = {"value": value, "unit": "pair interactions/s", "cores": oracle.threads(),
+           "kind": "port", "seconds": round(dt, 3), "cpu_model": model, "build": flags,
+           "sample": f"{reps} x {ni} i-particles x {n} j of config 2 (FP64 direct sum + "
+                     f"nearest neighbour, oracle/g6_oracle.c, OpenMP over i)",
+           "oracle_build_value": strict}
+    if kira is not None:
+        pairs = kira["particle_steps"] * (kira["n"] - 1)
+        out["config3"] = {"pairs": pairs, "force_seconds_at_cpu_rate": pairs / value,
+                          "per_nbody_unit_s": pairs / value / kira["t_end"],
+                          "gpu_end_to_end_s": kira["seconds"],
+                          "note": "the kira leg's pair interactions at the CPU rate above "
+                                  "(kira's host force path for the same blocks)"}
+    if clusters:
+        threads = oracle.threads()
+        sample = clusters["sample"]
+
+        def one(j):
+            _, _, _, steps, _, _, _ = oracle.hermite_shared(j["m"], j["x"], j["v"], j["t_end"],
+                                                            j["eta"], j["eps2"], L=L)
+            return steps * len(j["m"]) * (len(j["m"]) - 1)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            pairs = sum(ex.map(one, sample))
+        dt4 = time.perf_counter() - t0
+        out["config4"] = {"pairs_per_s": pairs / dt4, "systems_per_s": len(sample) / dt4,
+                          "seconds": round(dt4, 3), "threads": threads,
+                          "sample": f"{len(sample)} of the 4096 clusters (every "
+                                    f"{clusters['stride']}th), FP64, to t = {sample[0]['t_end']}",
+                          "gpu_batch_pairs_per_s": clusters.get("gpu_pairs_per_s"),
+                          "gpu_batch_t1_s_at_cpu_rate":
+                              clusters["gpu_pairs_t1"] / (pairs / dt4)
+                              if clusters.get("gpu_pairs_t1") else None}
+    errs = {}
+    sel = np.arange(0, n, max(1, n // 512))[:512]
+    errs["config2"] = fp64_errors([{"t": 0.0, "jm": gpu["jm"], "idx": idx[sel], "xi": xp[sel],
+                                    "vi": vp[sel], "acc": gpu["acc"][sel],
+                                    "jerk": gpu["jerk"][sel], "pot": gpu["pot"][sel],
+                                    "nn": gpu["nn"][sel]}], EPS2)
+    for name, samples, eps2 in gpu.get("samples", ()):
+        errs[name] = fp64_errors(samples, eps2)
+    return out, errs
 
 
 def ensemble_legs(seconds=2.0):
@@ -109,6 +219,28 @@ def ensemble_legs(seconds=2.0):
     jobs = serve.serving_mix(int(2000 * seconds), 5)
     r = serve.run_poisson(jobs, 2000.0)
     out["serving"] = r
+    # error against FP64: the same 256 clusters (every 16th of the batch) in
+    # both precisions to t = 1/32; the FP64 engine is the oracle integrator
+    # bit for bit (tests/test_gpu_ens.py)
+    sample = serve.cluster_batch(4096, 32, 2048, 4, 1.0 / 32, prec=64)[::16]
+    r64 = serve.run_batch(sample, keep=True)["results"]
+    for j in sample:
+        j["prec"] = 32
+    r32 = serve.run_batch(sample, keep=True)["results"]
+    dx, de32, de64 = [], [], []
+    for a, b in zip(r32, r64):
+        scale = np.sqrt((b["x"] ** 2).sum(1).mean())
+        dx.append(float(np.abs(a["x"] - b["x"]).max() / scale))
+        de32.append(abs(a["e1"] / a["e0"] - 1))
+        de64.append(abs(b["e1"] / b["e0"] - 1))
+    out["error_vs_fp64"] = {
+        "systems": len(sample), "t_end": 1.0 / 32,
+        "position_rel_max_fp32_vs_fp64": {"median": float(np.median(dx)),
+                                          "p99": float(np.percentile(dx, 99)),
+                                          "max": float(np.max(dx))},
+        "de_fp32": {"median": float(np.median(de32)), "max": float(np.max(de32))},
+        "de_fp64": {"median": float(np.median(de64)), "max": float(np.max(de64))},
+        "force": "FP32 pairs on double-single positions: the pair law of config 2"}
     return out
 
 
@@ -181,7 +313,46 @@ def abi_leg(nj=N, blocks=200, sizes=(1, 16, 256, 2048), h2=0.0):
             "per_block": [{k: json.loads(line)[k] for k in keys} for line in r.stdout.splitlines()]}
 
 
-def kira_leg(n=262144, t_end=0.25):
+def _keeper(samples, every, max_i, seed):
+    """Observer for NativeBlockHermite keeping every `every`-th force call's
+    j-memory, i inputs and results (at most max_i i, seeded) for the error
+    against FP64; returns (observer, seconds spent in it)."""
+    rng = np.random.default_rng(seed)
+    calls = [0]
+    spent = [0.0]
+
+    def obs(jm, t, idx, xi, vi, acc, jerk, pot, nn):
+        calls[0] += 1
+        if (calls[0] - 1) % every:
+            return False
+        t0 = time.perf_counter()
+        sel = np.arange(len(idx))
+        if max_i and len(idx) > max_i:
+            sel = np.sort(rng.choice(len(idx), max_i, replace=False))
+        samples.append(dict(t=t, jm={k: np.array(jm[k]) for k in jm}, idx=np.array(idx[sel]),
+                            xi=np.array(xi[sel]), vi=np.array(vi[sel]), acc=np.array(acc[sel]),
+                            jerk=np.array(jerk[sel]), pot=np.array(pot[sel]),
+                            nn=np.array(nn[sel])))
+        spent[0] += time.perf_counter() - t0
+        return False
+
+    return obs, spent
+
+
+def config1_leg(samples, every=23):
+    """Config 1: Plummer N = 1024 (makeplummer -s 42), eps = 0, eta = 0.1,
+    t = 0 -> 1 through the native block-step driver on libg6; every
+    `every`-th block is kept for the error against FP64."""
+    from starlab_amd.blockstep import NativeBlockHermite
+    from starlab_amd.ics import makeplummer
+    m, x, v = makeplummer(1024, 42)
+    obs, spent = _keeper(samples, every, None, 1)
+    b = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(1.0, observer=obs)
+    return {"blocks": b.blocks, "particle_steps": b.particle_steps,
+            "seconds": round(b.seconds - spent[0], 3), "kept_blocks": len(samples)}
+
+
+def kira_leg(n=262144, t_end=0.25, samples=None):
     """kira end to end on libg6 (config 3's workload): King W0 = 6, N = 262144
     top-level nodes, Salpeter 1-100, 5 % binaries as centres of mass, eta =
     0.1, eps = 0, t = 0 -> 1/4, integrated by the native block-step driver
@@ -207,28 +378,53 @@ def kira_leg(n=262144, t_end=0.25):
     m, x, v = c["m"], c["x"], c["v"]
     e0 = energy(m, x, v)
     g.close()
-    b = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(t_end)
+    obs, spent = (None, [0.0]) if samples is None else _keeper(samples, 4603, 32, 3)
+    b = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(t_end, observer=obs)
     g = GRAPE6()
     if g.open() != 0:
         raise RuntimeError(g.last_error())
     e1 = energy(b.m, b.x, b.v)
     g.close()
+    wall = b.seconds - spent[0]
     return {"workload": f"King W0=6 N={n}, Salpeter 1-100, 5% binaries (centres of mass), "
                         f"eta=0.1, eps=0, t=0->{t_end}; native kira block-step driver, "
                         "per-particle g6_set_j_particle_",
-            "blocks": b.blocks, "particle_steps": b.particle_steps,
-            "seconds": round(b.seconds, 3), "seconds_in_abi": round(b.abi_seconds, 3),
-            "pairs_per_s": b.particle_steps * (n - 1) / b.seconds,
-            "de_over_e": abs((e1 - e0) / e0)}
+            "n": n, "t_end": t_end, "blocks": b.blocks, "particle_steps": b.particle_steps,
+            "seconds": round(wall, 3), "seconds_in_abi": round(b.abi_seconds, 3),
+            "pairs_per_s": b.particle_steps * (n - 1) / wall,
+            "de_over_e": abs((e1 - e0) / e0)}, c
+
+
+def kira_callmix_leg(c, t_end=2.0 ** -8):
+    """Config 3's cluster with kira's whole GRAPE-6 call mix (bs_run_kira):
+    perturber lists and a perturbation pair call for every binary centre of
+    mass at every step, list overflows and single-node redos as kira makes
+    them; timed end to end, no checks."""
+    from starlab_amd.blockstep import KiraCallMix
+    n = len(c["m"])
+    bins = dict(node=c["bin_index"], m1=c["m1"], dx=c["dx"], dv=c["dv"])
+    k = KiraCallMix(c["m"], c["x"], c["v"], eta=0.1, eps2=0.0).run(t_end, binaries=bins)
+    st = dict(k.stats)
+    return {"workload": f"King W0=6 N={n}, {len(bins['node'])} binaries with their orbits, "
+                        f"t=0->{t_end}; bs_run_kira (lists, perturbation pairs, redos)",
+            "seconds": round(st.pop("seconds"), 3),
+            "seconds_in_abi": round(st.pop("abi_seconds"), 3),
+            "pairs_per_s_blocks": k.particle_steps * (n - 1) / k.seconds, "calls": st}
+
+
+TRAFFIC_FILE = os.path.join("profiles", "force_n65536_traffic.json")
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "force_n65536_traffic.json")
+    """HBM bytes per k_force launch of this workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in TRAFFIC_FILE (counters cannot
+    run inside this timed process); None if absent."""
     try:
-        with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch"), d.get("measured")
     except (OSError, ValueError):
-        return None
+        return None, None
 
 
 def main():
@@ -322,6 +518,11 @@ def main():
     wg, nsplit, jchunk = g.geometry()
     acc, jerk, pot, nn = g.read_results(n)
     refinements = g.refinements()
+    traffic, traffic_when = load_traffic()
+    # config 2's results and j-memory, kept for the error against FP64
+    gpu = {"acc": acc, "jerk": jerk, "pot": pot, "nn": nn, "samples": [],
+           "jm": {"t": inp["tj"], "m": inp["m"], "x": inp["x"], "v": inp["v"],
+                  "a2": inp["a2"], "j6": inp["j6"], "k18": np.zeros((n, 3))}}
 
     out = {
         "metric": "pair interactions/s at N=64K vs FP32 roofline; ensemble p50/p99 latency",
@@ -342,7 +543,8 @@ def main():
                    "n": n, "eps2": EPS2, "pairs_per_step": pairs_per_step},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": load_traffic(),
+                     "traffic": traffic,
+                     "traffic_source": f"{TRAFFIC_FILE} ({traffic_when})",
                      "kernel": "k_force", "kernel_ms": force_ms,
                      "flop_per_pair": FLOP_PER_PAIR},
         "kernel_ms": breakdown,
@@ -365,19 +567,45 @@ def main():
         # about 30 neighbours per i in the uniform sphere of abi_bench
         out["aux"]["grape6_abi_blocks_with_lists"] = abi_leg(blocks=100, sizes=(1, 256, 2048),
                                                              h2=0.006)
+    kira = None
     if not a.no_kira:
+        log("bench: config 1 (native block-step driver, Plummer N=1024 to t=1)")
+        s1 = []
+        out["config1"] = config1_leg(s1)
+        gpu["samples"].append(("config1", s1, 0.0))
         log("bench: kira end to end (native block-step driver, N=262144 to t=1/4)")
-        out["kira_end_to_end"] = kira_leg()
+        s3 = []
+        kira, cluster = kira_leg(samples=s3)
+        out["kira_end_to_end"] = kira
+        gpu["samples"].append(("config3", s3, 0.0))
+        log("bench: kira's call mix on config 3 (bs_run_kira, to t=1/256)")
+        out["kira_callmix"] = kira_callmix_leg(cluster)
+        del cluster
+    clusters = None
     if not a.no_ens:
         log("bench: ensemble and serving legs")
         ens = ensemble_legs()
         out["ensemble"] = ens
         out["ensemble_p50_s"] = ens["serving"]["p50_s"]
         out["ensemble_p99_s"] = ens["serving"]["p99_s"]
+        from starlab_amd import serve
+        stride = 64
+        clusters = {"sample": serve.cluster_batch(4096, 32, 2048, 4, 1.0 / 32)[::stride],
+                    "stride": stride,
+                    "gpu_pairs_per_s": ens["batch_fp32_t1"]["pairs_per_s"],
+                    "gpu_pairs_t1": ens["batch_fp32_t1"]["pairs_per_s"]
+                    * ens["batch_fp32_t1"]["wall_s"]}
     if not a.no_cpu:
-        log("bench: CPU baseline")
-        cb = cpu_baseline(inp)
+        log("bench: CPU baselines and errors against FP64")
+        cb, errs = cpu_leg(inp, gpu, clusters=clusters, kira=kira)
         out["cpu_baseline"] = cb
+        if "ensemble" in out:
+            e = out["ensemble"]["error_vs_fp64"]
+            errs["config4"] = e
+            errs["config5"] = {"3body": "FP64 pairs: the oracle integrator bit for bit "
+                                        "(tests/test_gpu_ens.py)",
+                               "clusters": "FP32 pairs, as config4"}
+        out["error_vs_fp64"] = errs
     print(json.dumps(out), flush=True)
 
 

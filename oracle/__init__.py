@@ -32,39 +32,76 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def host_cpu():
+    """(model name, has AVX-512F) of this host, from /proc/cpuinfo."""
+    model, avx512 = "unknown", False
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags"):
+                    avx512 = " avx512f " in f" {line.split(':', 1)[1].strip()} "
+                    break
+    except OSError:
+        pass
+    return model, avx512
+
+
+_fast = None
+
+
+def fast_lib():
+    """The speed build of the same source for the benchmark's CPU baseline
+    (-O3, FMA contraction, OpenMP; x86-64-v4 on AVX-512 hosts, else
+    x86-64-v3): (CDLL, compiler flags).  Not a checker."""
+    global _fast
+    if _fast is None:
+        _, avx512 = host_cpu()
+        level = "v4" if avx512 else "v3"
+        path = os.path.join(HERE, f"liboracle_g6_{level}.so")
+        if not os.path.exists(path):
+            build()
+        _fast = (_prototypes(C.CDLL(path)), f"gcc -O3 -march=x86-64-{level} -fopenmp")
+    return _fast
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
-        L.or_ran2_seed.argtypes = [C.c_int]
-        L.or_ran2_seed.restype = C.c_int
-        L.or_randinter.argtypes = [C.c_double, C.c_double]
-        L.or_randinter.restype = C.c_double
-        L.or_n_rand.restype = C.c_int
-        L.or_threads.restype = C.c_int
-        L.or_makeplummer.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
-        L.or_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
-                                 C.c_double, C.c_int, _dp, _dp]
-        L.or_force.argtypes = [C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp,
-                               _ip, C.c_double, _dp, _dp, _dp, _ip, _dp]
-        L.or_force_ld.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
-                                  _ip, C.c_double, _dp, _dp, _dp]
-        L.or_neighbours.argtypes = [C.c_int, _dp, _dp, _ip, C.c_int, C.c_int,
-                                    _dp, _ip, C.c_int, _ip, _ip]
-        L.or_g4_force.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
-                                  _dp, _dp, _dp, C.c_int, _ip, _ip]
-        L.or_density.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
-        L.or_energies.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
-                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.or_hermite_shared.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
-                                        C.c_double, C.c_double, C.c_longlong,
-                                        C.POINTER(C.c_double), C.POINTER(C.c_longlong),
-                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.or_hermite_shared.restype = C.c_int
-        _lib = L
+        _lib = _prototypes(C.CDLL(_LIB))
     return _lib
+
+
+def _prototypes(L):
+    L.or_ran2_seed.argtypes = [C.c_int]
+    L.or_ran2_seed.restype = C.c_int
+    L.or_randinter.argtypes = [C.c_double, C.c_double]
+    L.or_randinter.restype = C.c_double
+    L.or_n_rand.restype = C.c_int
+    L.or_threads.restype = C.c_int
+    L.or_makeplummer.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.or_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                             C.c_double, C.c_int, _dp, _dp]
+    L.or_force.argtypes = [C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp,
+                           _ip, C.c_double, _dp, _dp, _dp, _ip, _dp]
+    L.or_force_ld.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
+                              _ip, C.c_double, _dp, _dp, _dp]
+    L.or_neighbours.argtypes = [C.c_int, _dp, _dp, _ip, C.c_int, C.c_int,
+                                _dp, _ip, C.c_int, _ip, _ip]
+    L.or_g4_force.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
+                              _dp, _dp, _dp, C.c_int, _ip, _ip]
+    L.or_density.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+    L.or_energies.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
+                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_hermite_shared.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_longlong,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_hermite_shared.restype = C.c_int
+    return L
 
 
 def _d(a, shape=None):
@@ -150,14 +187,16 @@ def density(m, x, k: int = 12, i_first: int = 0, ni: int | None = None):
     return rho, dk2
 
 
-def hermite_shared(m, x, v, t_end, eta, eps2, dt_max=0.125, max_steps=10**8):
+def hermite_shared(m, x, v, t_end, eta, eps2, dt_max=0.125, max_steps=10**8, L=None):
     """Integrate one system with the shared-step Hermite rules; returns
-    (x, v, t_final, steps, e0, e1, rc)."""
+    (x, v, t_final, steps, e0, e1, rc).  L: another build of the same source
+    (fast_lib()[0], for the CPU baseline); default the oracle."""
     x = np.array(x, dtype=np.float64, order="C")
     v = np.array(v, dtype=np.float64, order="C")
     t = C.c_double(); ns = C.c_longlong(); e0 = C.c_double(); e1 = C.c_double()
-    rc = lib().or_hermite_shared(len(m), _d(m), x, v, t_end, eta, eps2, dt_max, max_steps,
-                                 C.byref(t), C.byref(ns), C.byref(e0), C.byref(e1))
+    rc = (L or lib()).or_hermite_shared(len(m), _d(m), x, v, t_end, eta, eps2, dt_max,
+                                        max_steps, C.byref(t), C.byref(ns), C.byref(e0),
+                                        C.byref(e1))
     return x, v, t.value, ns.value, e0.value, e1.value, rc
 
 

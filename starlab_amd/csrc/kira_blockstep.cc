@@ -66,7 +66,9 @@ struct Abi {
 
     int load(const char *path)
     {
-        h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+        // never unloaded: a GRAPE library may own threads (a GPU runtime, an
+        // OpenMP pool) that would outlive its code; kira links it for good
+        h = dlopen(path, RTLD_NOW | RTLD_LOCAL | RTLD_NODELETE);
         if (!h) return bs_fail(-1, "bs_run: dlopen failed: ", dlerror());
         const char *missing = nullptr;
         auto sym = [&](auto &f, const char *name) {

@@ -58,10 +58,11 @@ def _pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
 
-def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, **opt):
+def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep=False, **opt):
     """Submit every job at t = 0 and run to completion.  With tune=True the
     engine's shape options come from ens.autotune on a sample of the jobs
-    (its time is reported, not counted in wall_s)."""
+    (its time is reported, not counted in wall_s); keep=True adds each job's
+    result (x, v, t, steps, e0, e1, failed) in job order."""
     tuned = None
     if tune:
         best, timings, secs = autotune(jobs, max_systems=max_systems)
@@ -88,12 +89,15 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, **op
         eng.close()
     lat = [done_at[i] for i in ids]
     de = [abs((results[i]["e1"] - results[i]["e0"]) / results[i]["e0"]) for i in ids]
-    return dict(jobs=len(jobs), wall_s=wall, systems_per_s=len(jobs) / wall,
-                pairs=st["pairs"], pairs_per_s=st["pairs"] / wall,
-                device_s=st["device_seconds"], engine_steps=st["engine_steps"],
-                p50_s=_pct(lat, 50), p99_s=_pct(lat, 99),
-                failed=sum(results[i]["failed"] for i in ids),
-                de_median=_pct(de, 50), de_max=float(np.max(de)), tuned=tuned)
+    out = dict(jobs=len(jobs), wall_s=wall, systems_per_s=len(jobs) / wall,
+               pairs=st["pairs"], pairs_per_s=st["pairs"] / wall,
+               device_s=st["device_seconds"], engine_steps=st["engine_steps"],
+               p50_s=_pct(lat, 50), p99_s=_pct(lat, 99),
+               failed=sum(results[i]["failed"] for i in ids),
+               de_median=_pct(de, 50), de_max=float(np.max(de)), tuned=tuned)
+    if keep:
+        out["results"] = [results[i] for i in ids]
+    return out
 
 
 def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, **opt):

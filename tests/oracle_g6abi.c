@@ -13,8 +13,8 @@
  * read returns 1 when a list exceeds that, get returns 1 with nblen =
  * max_length when the list has max_length or more entries).  Resets and
  * re-opens keep nothing on the "device" side that the driver relies on.
- * Failure injection for the driver's recovery path, read once from the
- * environment: ORACLE_G6ABI_FAIL_LASTHALF=k makes the next k lasthalf2 calls
+ * Failure injection for the driver's recovery path, read from the
+ * environment whenever a variable's text has changed since the last open: ORACLE_G6ABI_FAIL_LASTHALF=k makes the next k lasthalf2 calls
  * return 1, ORACLE_G6ABI_FAIL_READNL=k the next k read_neighbour_list calls
  * return -1, ORACLE_G6ABI_FAIL_NN=k the next k lasthalf2 calls report
  * nn = -1 for slot 0 (kira's error 42).  Only the entry points the driver
@@ -39,16 +39,33 @@ static int NB_CAP = 8192;   /* ORACLE_G6ABI_NB_CAP */
 static int *nb_counts, *nb_lists;
 static int nb_valid, fail_lasthalf, fail_readnl, fail_nn, env_read;
 
+/* A setting is taken when its variable's text differs from the text seen
+ * last (the library stays loaded across driver runs, and the re-opens of a
+ * reset must not re-arm a failure count). */
+static int env_changed(const char *name, char *last, int *value, int dflt)
+{
+    const char *e = getenv(name);
+    const char *txt = e ? e : "";
+    if (strncmp(txt, last, 63) == 0) return 0;
+    strncpy(last, txt, 63);
+    last[63] = 0;
+    *value = e ? atoi(e) : dflt;
+    return 1;
+}
+
 static void read_env(void)
 {
-    if (env_read) return;
-    env_read = 1;
-    const char *e;
-    if ((e = getenv("ORACLE_G6ABI_NPIPES"))) npipes = atoi(e);
-    if ((e = getenv("ORACLE_G6ABI_FAIL_LASTHALF"))) fail_lasthalf = atoi(e);
-    if ((e = getenv("ORACLE_G6ABI_FAIL_READNL"))) fail_readnl = atoi(e);
-    if ((e = getenv("ORACLE_G6ABI_FAIL_NN"))) fail_nn = atoi(e);
-    if ((e = getenv("ORACLE_G6ABI_NB_CAP")) && atoi(e) > 0) NB_CAP = atoi(e);
+    static char s_np[64], s_fl[64], s_fr[64], s_fn[64], s_nb[64];
+    if (!env_read) {   /* first call: "" differs from an unset-variable marker */
+        strcpy(s_np, "\x01"); strcpy(s_fl, "\x01"); strcpy(s_fr, "\x01");
+        strcpy(s_fn, "\x01"); strcpy(s_nb, "\x01");
+        env_read = 1;
+    }
+    env_changed("ORACLE_G6ABI_NPIPES", s_np, &npipes, 64);
+    env_changed("ORACLE_G6ABI_FAIL_LASTHALF", s_fl, &fail_lasthalf, 0);
+    env_changed("ORACLE_G6ABI_FAIL_READNL", s_fr, &fail_readnl, 0);
+    env_changed("ORACLE_G6ABI_FAIL_NN", s_fn, &fail_nn, 0);
+    if (env_changed("ORACLE_G6ABI_NB_CAP", s_nb, &NB_CAP, 8192) && NB_CAP < 1) NB_CAP = 8192;
 }
 
 static void grow(int need)

@@ -86,3 +86,46 @@ def test_config1_native_driver_equals_python_driver():
     assert chk.worst["acc_norm"] <= 1e-6 and chk.worst["jerk_norm"] <= 1e-6
     assert np.percentile(np.concatenate(chk.rel_errs["acc"]), 99.9) <= 1e-6
     assert chk.worst["pot_rel"] <= 1e-6
+
+
+def test_config1_snapshot_restart():
+    """kira's restart (-R, kira.C:46-113; the j-memory is reloaded after
+    g6_open_, hdyn_grape6.C:3004-3054): config 1's initial conditions go
+    through a Starlab snapshot (bit-exact round trip), the run stops at
+    t = 1/8, writes a snapshot, closes libg6, and a new run restarts from the
+    snapshot read back to t = 1/4.  Every restarted block agrees with the
+    FP64 oracle, the restart state is the stopped state bit for bit, and the
+    energy change over the two legs stays within config 1's factor of the
+    all-CPU run."""
+    import io
+
+    from starlab_amd import snapshot
+    from starlab_amd.blockstep import NativeBlockHermite
+    from tests.kira_backends import NativeChecker
+    m, x, v = oracle.makeplummer(1024, 42)
+    root = snapshot.read_snapshot(io.StringIO(snapshot.write_snapshot(None,
+                                                                      snapshot.from_arrays(m, x, v))))
+    m0, x0, v0 = snapshot.top_level(root)
+    for a, b in ((m0, m), (x0, x), (v0, v)):
+        np.testing.assert_array_equal(a, b)
+    e0 = sum(oracle.energies(m, x, v, 0.0))
+    first = NativeBlockHermite(m0, x0, v0, eta=0.1, eps2=0.0).run(0.125)
+    text = snapshot.write_snapshot(None, snapshot.from_arrays(first.m, first.x, first.v,
+                                                              system_time=0.125))
+    back = snapshot.read_snapshot(io.StringIO(text))
+    assert back.system_time == 0.125
+    m1, x1, v1 = snapshot.top_level(back)
+    np.testing.assert_array_equal(x1, first.x)
+    np.testing.assert_array_equal(v1, first.v)
+    chk = NativeChecker(0.0)
+    second = NativeBlockHermite(m1, x1, v1, eta=0.1, eps2=0.0).run(0.125, observer=chk)
+    cpu = BlockHermite(m, x, v, eta=0.1, eps2=0.0)
+    cpu.run(HostBackend(), 0.25)
+    de_cpu = abs((_energy(cpu) - e0) / e0)
+    de = abs(sum(oracle.energies(second.m, second.x, second.v, 0.0)) / e0 - 1)
+    print(f"restart: {first.blocks} + {second.blocks} blocks, dE/E {de:.3e} (all-CPU, "
+          f"uninterrupted: {de_cpu:.3e}); {chk.summary()}", flush=True)
+    assert chk.checked_blocks == second.blocks and chk.nn_mismatch == 0
+    assert chk.worst["acc_norm"] <= 1e-6 and chk.worst["jerk_norm"] <= 1e-6
+    assert chk.worst["pot_rel"] <= 1e-6
+    assert de <= 10 * de_cpu + 1e-8
