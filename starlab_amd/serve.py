@@ -100,8 +100,10 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep
     return out
 
 
-def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, **opt):
-    """Serve `jobs` arriving as a Poisson stream of `rate` per second."""
+def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, keep=False,
+                **opt):
+    """Serve `jobs` arriving as a Poisson stream of `rate` per second;
+    keep=True adds each job's result in job order."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
     eng = Ensemble(budget, max_systems, steps_per_graph=chunk, **opt)
@@ -112,11 +114,13 @@ def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, *
         done_at = {}
         results = {}
         kind = {}
+        order = []
         while len(done_at) < len(jobs):
             now = time.perf_counter() - t0
             while nxt < len(jobs) and arrivals[nxt] <= now:
                 j = jobs[nxt]
                 jid = _submit(eng, j)
+                order.append(jid)
                 arrived[jid] = arrivals[nxt]
                 kind[jid] = j["kind"]
                 nxt += 1
@@ -139,13 +143,16 @@ def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, *
         d = t - arrived[jid]
         lat[kind[jid]].append(d)
         allat.append(d)
-    return dict(jobs=len(jobs), rate_req_s=rate, offered_s=float(arrivals[-1]), wall_s=wall,
-                throughput_req_s=len(jobs) / wall, pairs_per_s=st["pairs"] / wall,
-                device_busy=st["device_seconds"] / wall,
-                p50_s=_pct(allat, 50), p99_s=_pct(allat, 99),
-                p50_3body_s=_pct(lat["3body"], 50), p99_3body_s=_pct(lat["3body"], 99),
-                p50_cluster_s=_pct(lat["cluster"], 50), p99_cluster_s=_pct(lat["cluster"], 99),
-                failed=sum(r["failed"] for r in results.values()))
+    out = dict(jobs=len(jobs), rate_req_s=rate, offered_s=float(arrivals[-1]), wall_s=wall,
+               throughput_req_s=len(jobs) / wall, pairs_per_s=st["pairs"] / wall,
+               device_busy=st["device_seconds"] / wall,
+               p50_s=_pct(allat, 50), p99_s=_pct(allat, 99),
+               p50_3body_s=_pct(lat["3body"], 50), p99_3body_s=_pct(lat["3body"], 99),
+               p50_cluster_s=_pct(lat["cluster"], 50), p99_cluster_s=_pct(lat["cluster"], 99),
+               failed=sum(r["failed"] for r in results.values()))
+    if keep:
+        out["results"] = [results[j] for j in order]
+    return out
 
 
 def main(argv=None):

@@ -264,3 +264,87 @@ def test_lane_options_fixed_while_open():
         for k, v in DEFAULTS.items():
             e.option(k, v)
     assert L.g6e_set_option(b"wg_steps", 0) == 0
+
+
+def test_config4_sample_fp32_to_t1_against_fp64():
+    """Config 4 at its real shape: a size-stratified sample of 256 of the 4096
+    Plummer clusters (N ~ U[32, 2048], seed 4) integrated to t = 1 in FP32
+    pairs, and the same systems in FP64 pairs (bitwise the oracle integrator,
+    test_fp64_bitwise_equals_oracle).  Every system finishes.  The FP32 step
+    rule subtracts a force-noise floor from kira's Aarseth criterion; against
+    FP64 it takes within 6 % of the steps of every system (measured 0.984 to
+    1.037, median 0.992), and its energy error stays below 2e-7 (measured
+    max 8.6e-8, median 7.4e-9; FP64 max 4.3e-9, bound 1e-8).  Four of the systems are also
+    run by the oracle itself on the CPU (systems of at most 512 bodies, one
+    j-split, so FP64 is bitwise the oracle): FP64 bitwise, FP32 within the
+    same energy bound."""
+    from starlab_amd import serve
+    batch = serve.cluster_batch(4096, 32, 2048, 4, 1.0, prec=32)
+    order = np.argsort([j["n"] for j in batch], kind="stable")
+    sample = [batch[k] for k in order[::16]]
+    r32 = serve.run_batch(sample, keep=True)["results"]
+    for j in sample:
+        j["prec"] = 64
+    r64 = serve.run_batch(sample, keep=True)["results"]
+    de32 = np.array([abs(r["e1"] / r["e0"] - 1) for r in r32])
+    de64 = np.array([abs(r["e1"] / r["e0"] - 1) for r in r64])
+    ratio = np.array([a["steps"] / b["steps"] for a, b in zip(r32, r64)])
+    print(f"FP32 |dE/E| median {np.median(de32):.2e} max {de32.max():.2e}; FP64 median "
+          f"{np.median(de64):.2e} max {de64.max():.2e}; steps FP32/FP64 median "
+          f"{np.median(ratio):.3f} min {ratio.min():.3f} max {ratio.max():.3f}", flush=True)
+    assert not any(r["failed"] for r in r32 + r64)
+    assert all(r["t"] == 1.0 for r in r32 + r64)
+    assert de32.max() <= 2e-7 and de64.max() <= 1e-8
+    assert 0.94 <= ratio.min() and ratio.max() <= 1.06
+    small = [k for k, j in enumerate(sample) if j["n"] <= 512]
+    for k in small[::max(1, len(small) // 4)][:4]:
+        j = sample[k]
+        x, v, t, steps, e0, e1, rc = oracle.hermite_shared(j["m"], j["x"], j["v"], 1.0, j["eta"],
+                                                           j["eps2"])
+        assert rc == 0 and t == 1.0
+        np.testing.assert_array_equal(r64[k]["x"], x)
+        assert r64[k]["steps"] == steps
+        assert abs(r32[k]["e1"] / r32[k]["e0"] - 1) <= 2e-7
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_alone_equals_mixed_at_the_largest_serving_cluster(prec):
+    """The top of config 5's cluster range (N = 4096) gives the same result
+    alone and admitted into a running mix of scatterings and clusters."""
+    target = _jobs(13, [4096], t_end=1.0 / 64)[0]
+    alone, _ = _run([target], precision=prec)
+    mix = _jobs(14, [3, 3, 300, 3, 2048, 3, 3, 1000, 3], t_end=1.0 / 64)
+    e = Ensemble(2 * GB, 64, precision=prec)
+    try:
+        first = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in mix[:5]]
+        e.run(8)
+        a = e.submit(target["m"], target["x"], target["v"], target["t_end"], target["eta"],
+                     target["eps2"])
+        rest = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in mix[5:]]
+        res = e.run_until_idle()
+    finally:
+        e.close()
+        for k, v in DEFAULTS.items():
+            e.option(k, v)
+    np.testing.assert_array_equal(res[a]["x"], alone[0]["x"])
+    np.testing.assert_array_equal(res[a]["v"], alone[0]["v"])
+    assert res[a]["steps"] == alone[0]["steps"] and len(res) == len(mix) + 1
+
+
+def test_poisson_serving_every_job_finishes_and_equals_alone():
+    """A short served Poisson stream (2000 req/s, 400 jobs: 70 % three-body
+    scatterings in FP64, 30 % clusters N ~ U[256, 4096] in FP32, admitted
+    while others run): every job finishes, and a sample of them equals its
+    run alone bit for bit."""
+    from starlab_amd import serve
+    jobs = serve.serving_mix(400, 21)
+    r = serve.run_poisson(jobs, 2000.0, keep=True)
+    print({k: v for k, v in r.items() if k != "results"}, flush=True)
+    assert r["failed"] == 0 and len(r["results"]) == len(jobs)
+    assert all(res["t"] == j["t_end"] for res, j in zip(r["results"], jobs))
+    picks = [k for k, j in enumerate(jobs) if j["kind"] == "3body"][:4] + \
+            [k for k, j in enumerate(jobs) if j["kind"] == "cluster"][:3]
+    for k in picks:
+        alone = serve.run_batch([jobs[k]], keep=True)["results"][0]
+        np.testing.assert_array_equal(r["results"][k]["x"], alone["x"])
+        assert r["results"][k]["steps"] == alone["steps"]
