@@ -103,7 +103,7 @@ struct Engine {
     JRaw *d_jraw = nullptr;
     JPred *d_jpred = nullptr;
     double *d_xp64 = nullptr;   // [3][dcap]
-    double *d_vp64 = nullptr;   // [3][dcap]
+    double *d_vp64 = nullptr;   // [3][dcap] velocities, then the FP64 masses m64[dcap]
     int *d_jindex = nullptr;
 
     int scap = 0;               // scatter staging capacity (records)

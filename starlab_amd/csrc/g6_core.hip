@@ -183,12 +183,13 @@ int ensure_device_j(int need)
     HIPCHK(hipMalloc((void **)&E.d_jraw, sizeof(JRaw) * (size_t)cap));
     HIPCHK(hipMalloc((void **)&E.d_jpred, sizeof(JPred) * (size_t)cap));
     HIPCHK(hipMalloc((void **)&E.d_xp64, sizeof(double) * 3 * (size_t)cap));
-    HIPCHK(hipMalloc((void **)&E.d_vp64, sizeof(double) * 3 * (size_t)cap));
+    // vp64 [3][cap] followed by m64 [cap]
+    HIPCHK(hipMalloc((void **)&E.d_vp64, sizeof(double) * 4 * (size_t)cap));
     HIPCHK(hipMalloc((void **)&E.d_jindex, sizeof(int) * (size_t)cap));
     HIPCHK(hipMemsetAsync(E.d_jraw, 0, sizeof(JRaw) * (size_t)cap, E.stream));
     HIPCHK(hipMemsetAsync(E.d_jpred, 0, sizeof(JPred) * (size_t)cap, E.stream));
     HIPCHK(hipMemsetAsync(E.d_xp64, 0, sizeof(double) * 3 * (size_t)cap, E.stream));
-    HIPCHK(hipMemsetAsync(E.d_vp64, 0, sizeof(double) * 3 * (size_t)cap, E.stream));
+    HIPCHK(hipMemsetAsync(E.d_vp64, 0, sizeof(double) * 4 * (size_t)cap, E.stream));
     E.dcap = cap;
     // everything set so far must reach the new arrays (a deferred scatter's
     // records are in the mirror too)
@@ -561,7 +562,7 @@ int enqueue_force(int ni, int nj, double eps2d, const IBlock &ib, const Results 
     {
         KTimer t(K_REDUCE);
         const NearTile nt{eps2d, g.jchunk, g.variant == 1 ? 1 : g.wpg, zskip ? 1 : 0,
-                          E.d_vp64, E.d_jraw};
+                          E.d_vp64, E.d_vp64 + 3 * (size_t)E.dcap};
         HIPCHK(launch_reduce(E.stream, p, g.npart, ni, nj, ib, eps2, g.tile,
                              E.d_xp64, E.dcap, E.d_jindex, r, g.lanes, nt));
         t.stop();

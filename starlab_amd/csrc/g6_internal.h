@@ -6,7 +6,9 @@
 //                                    FP32 velocity and mass (force kernel)
 //   double xp64[3][cap]      24 B/j  predicted FP64 position (nn refinement,
 //                                    neighbour lists; bit-exact distances)
-//   double vp64[3][cap]      24 B/j  predicted FP64 velocity (read-back only)
+//   double vp64[3][cap]      24 B/j  predicted FP64 velocity (read-back, FP64 nearest tile)
+//   double m64[cap]           8 B/j  FP64 mass, right after vp64 (FP64 nearest tile);
+//                                    written with every prediction
 //   int    jindex[cap]        4 B/j  the `index` field of each address
 // i-block (npipes slots, SoA so each lane reads its own slot coalesced):
 //   10 arrays of float/int, FP64 positions 3 arrays, h2 and the list offset
@@ -124,7 +126,7 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
 // The reduction's nearest-neighbour rescan reads the FP64 prediction
 // (xp64) and splits it into the double-single words the force kernel used.
 // It also recomputes the pairs of the winning tile in FP64 (xp64, vp64,
-// jraw masses, the i-block's FP64 position and velocity, softening eps2d)
+// m64, the i-block's FP64 position and velocity, softening eps2d)
 // and swaps them for that tile's FP32 sums; the tile of address t was
 // summed by partial (t / jchunk) / wpg (wpg = 1 for the LDS kernel).
 // zskip: as launch_force's (pairs with FP32 r^2 = 0 add nothing).
@@ -132,7 +134,7 @@ struct NearTile {
     double eps2d;
     int jchunk, wpg, zskip;
     const double *vp64;
-    const JRaw *jraw;
+    const double *m64;     // FP64 masses (vp64 + 3 * cap)
 };
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          int nj, const IBlock &ib, float eps2,

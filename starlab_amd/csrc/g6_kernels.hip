@@ -91,6 +91,8 @@ __global__ __launch_bounds__(256) void k_predict(
     JPred *__restrict__ jpred, double *__restrict__ xp64,
     double *__restrict__ vp64, int cap)
 {
+    // m64 = the FP64 masses, right after vp64 ([3][cap] each)
+    double *__restrict__ m64 = vp64 + 3 * (size_t)cap;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nj) return;
     double xp[3], vp[3];
@@ -112,6 +114,7 @@ __global__ __launch_bounds__(256) void k_predict(
         xp64[(size_t)k * cap + j] = xp[k];
         vp64[(size_t)k * cap + j] = vp[k];
     }
+    m64[j] = r.m;
 }
 
 hipError_t launch_predict(hipStream_t s, const JRaw *jraw, int nj, double ti,
@@ -387,6 +390,7 @@ __global__ __launch_bounds__(64 * kMaxWpg) void k_force(
                             F.xp64[(size_t)k * F.cap + j] = xp[k];
                             F.vp64[(size_t)k * F.cap + j] = vp[k];
                         }
+                        F.vp64[3 * (size_t)F.cap + j] = r.m;   // m64
                     }
                     JRec q;
                     split_ds(xp[0], q.xh, q.xl);
@@ -907,6 +911,11 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         const int tend = min(tile + tile_len, nj);
         const uint32_t bb = __float_as_uint(best);
         const int gshift = lane & ~(L - 1);
+        // the tile's FP32 sums as the force kernel kept them, loaded before
+        // the scan so their latency overlaps it
+        const size_t o = (size_t)((tile / NT.jchunk) / NT.wpg) * P.ld + i;
+        Acc32 t;
+        tile_sums_load(P.tsum + o, P.tstride, t);
         int found = INT_MAX;
         // the whole tile: first address matching best, the two smallest r^2
         // (as a multiset) for the tile's own runner-up, and its pairs in FP64
@@ -933,7 +942,7 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
                 if (!(NT.zskip && r2 == 0.0f))
                     pair64(xj - xi, yj - yi, zj - zi, NT.vp64[j] - ui,
                            NT.vp64[(size_t)cap + j] - vi, NT.vp64[2 * (size_t)cap + j] - wi,
-                           NT.jraw[j].m, NT.eps2d, F);
+                           NT.m64[j], NT.eps2d, F);
             }
             unsigned long long m = __ballot(hit) >> gshift;
             if (L < 64) m &= (1ull << L) - 1ull;
@@ -948,9 +957,6 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         }
         // swap the tile's FP32 sums (converted and combined exactly as the
         // force kernel folded them) for its FP64 pairs
-        const size_t o = (size_t)((tile / NT.jchunk) / NT.wpg) * P.ld + i;
-        Acc32 t;
-        tile_sums_load(P.tsum + o, P.tstride, t);
         const double c32[7] = {(double)t.ax, (double)t.ay, (double)t.az,
                                (double)t.jx - 3.0 * (double)t.kx,
                                (double)t.jy - 3.0 * (double)t.ky,
