@@ -21,6 +21,14 @@
  *
  * Prints one JSON line per NI: median and mean microseconds per block and per
  * phase, and pair interactions per second.  Exit status 0 on success.
+ *
+ * With ABI_BENCH_DUMP naming a file, the last block of each NI is appended
+ * to it (native byte order) so that a checker can recompute it:
+ *   int32 nj, ni; float64 ti, eps2, m;  float64 tj[nj], x[nj][3], v[nj][3];
+ *   int32 index[ni]; float64 xi[ni][3], vi[ni][3];
+ *   float64 acc[ni][3], jerk[ni][3], pot[ni]; int32 nn[ni].
+ * The j-particles carry zero a2, j6 and k18, so their predicted state at ti
+ * is x + (ti - tj) v, v.
  */
 #define _POSIX_C_SOURCE 199309L
 #include "grape6.h"
@@ -45,6 +53,31 @@ static double urand(void)
     rng_state ^= rng_state >> 7;
     rng_state ^= rng_state << 17;
     return (double)(rng_state >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static int dump_block(const char *path, int nj, int ni, double ti, double eps2, double m,
+                      const double *tj, const double (*x)[3], const double (*v)[3],
+                      const int *index, const double (*xi)[3], const double (*vi)[3],
+                      const double (*acc)[3], const double (*jerk)[3], const double *pot,
+                      const int *nn)
+{
+    FILE *f = fopen(path, "ab");
+    if (!f) return 1;
+    const int head[2] = {nj, ni};
+    const double par[3] = {ti, eps2, m};
+    size_t ok = fwrite(head, sizeof(int), 2, f) == 2;
+    ok &= fwrite(par, sizeof(double), 3, f) == 3;
+    ok &= fwrite(tj, sizeof(double), nj, f) == (size_t)nj;
+    ok &= fwrite(x, sizeof(double[3]), nj, f) == (size_t)nj;
+    ok &= fwrite(v, sizeof(double[3]), nj, f) == (size_t)nj;
+    ok &= fwrite(index, sizeof(int), ni, f) == (size_t)ni;
+    ok &= fwrite(xi, sizeof(double[3]), ni, f) == (size_t)ni;
+    ok &= fwrite(vi, sizeof(double[3]), ni, f) == (size_t)ni;
+    ok &= fwrite(acc, sizeof(double[3]), ni, f) == (size_t)ni;
+    ok &= fwrite(jerk, sizeof(double[3]), ni, f) == (size_t)ni;
+    ok &= fwrite(pot, sizeof(double), ni, f) == (size_t)ni;
+    ok &= fwrite(nn, sizeof(int), ni, f) == (size_t)ni;
+    return (fclose(f) != 0 || !ok) ? 1 : 0;
 }
 
 static int cmp_double(const void *a, const void *b)
@@ -108,6 +141,7 @@ int main(int argc, char **argv)
     int *nn = malloc(sizeof(int) * npipes);
     const char *h2env = getenv("ABI_BENCH_H2");
     const double h2val = h2env ? atof(h2env) : 0.0;
+    const char *dump = getenv("ABI_BENCH_DUMP");
     for (int k = 0; k < npipes; ++k) h2[k] = h2val;
     int maxlen = 4096;
     int *nbl = malloc(sizeof(int) * maxlen);
@@ -167,6 +201,14 @@ int main(int argc, char **argv)
             const double tb4 = now_us();
             for (int k = 0; k < ni; ++k)
                 if (!(pot[k] < 0) || nn[k] < 0 || nn[k] >= nj) status = 1;
+            if (dump && b == nblocks - 1 && status == 0 &&
+                dump_block(dump, nj, ni, ti, eps2, m, tj, (const double(*)[3])x,
+                           (const double(*)[3])v, index, (const double(*)[3])xi,
+                           (const double(*)[3])vi, (const double(*)[3])acc,
+                           (const double(*)[3])jerk, pot, nn)) {
+                fprintf(stderr, "cannot write %s\n", dump);
+                status = 1;
+            }
             if (b >= 0) {
                 tb[b] = tb4 - tb0;
                 ts[b] = tb1 - tb0;

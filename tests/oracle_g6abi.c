@@ -17,8 +17,12 @@
  * environment whenever a variable's text has changed since the last open: ORACLE_G6ABI_FAIL_LASTHALF=k makes the next k lasthalf2 calls
  * return 1, ORACLE_G6ABI_FAIL_READNL=k the next k read_neighbour_list calls
  * return -1, ORACLE_G6ABI_FAIL_NN=k the next k lasthalf2 calls report
- * nn = -1 for slot 0 (kira's error 42).  Only the entry points the driver
- * needs are provided.
+ * nn = -1 for slot 0 (kira's error 42).  ORACLE_G6ABI_JSPLIT=T (T > 1)
+ * sums blocks of fewer than 64 i over T j-ranges in parallel and adds the
+ * ranges in order: for long runs at large N, where one i per thread leaves
+ * most threads idle; the sums then differ from or_force's in the last bits,
+ * the nearest neighbour (smallest d2, lowest j on ties) does not.  Only the
+ * entry points the driver needs are provided.
  */
 #include "grape6.h"
 #include "g6_oracle.h"
@@ -38,6 +42,7 @@ static int *excl_now;
 static int NB_CAP = 8192;   /* ORACLE_G6ABI_NB_CAP */
 static int *nb_counts, *nb_lists;
 static int nb_valid, fail_lasthalf, fail_readnl, fail_nn, env_read;
+static int jsplit = 1;   /* ORACLE_G6ABI_JSPLIT */
 
 /* A setting is taken when its variable's text differs from the text seen
  * last (the library stays loaded across driver runs, and the re-opens of a
@@ -55,12 +60,13 @@ static int env_changed(const char *name, char *last, int *value, int dflt)
 
 static void read_env(void)
 {
-    static char s_np[64], s_fl[64], s_fr[64], s_fn[64], s_nb[64];
+    static char s_np[64], s_fl[64], s_fr[64], s_fn[64], s_nb[64], s_js[64];
     if (!env_read) {   /* first call: "" differs from an unset-variable marker */
         strcpy(s_np, "\x01"); strcpy(s_fl, "\x01"); strcpy(s_fr, "\x01");
-        strcpy(s_fn, "\x01"); strcpy(s_nb, "\x01");
+        strcpy(s_fn, "\x01"); strcpy(s_nb, "\x01"); strcpy(s_js, "\x01");
         env_read = 1;
     }
+    if (env_changed("ORACLE_G6ABI_JSPLIT", s_js, &jsplit, 1) && jsplit < 1) jsplit = 1;
     env_changed("ORACLE_G6ABI_NPIPES", s_np, &npipes, 64);
     env_changed("ORACLE_G6ABI_FAIL_LASTHALF", s_fl, &fail_lasthalf, 0);
     env_changed("ORACLE_G6ABI_FAIL_READNL", s_fr, &fail_readnl, 0);
@@ -142,6 +148,44 @@ int g6_set_j_particle_(int *c, int *address, int *index, double *tj, double *dtj
     return 0;
 }
 
+/* or_force over jsplit j-ranges at once, the ranges added in order (the
+ * ORACLE_G6ABI_JSPLIT mode); results into r_acc, r_jerk, r_pot, r_nn. */
+static void force_jsplit(int k, const double *xi, const double *vi, const int *excl, int n,
+                         double eps2)
+{
+    const int T = jsplit, chunk = (n + T - 1) / T;
+    double *pa = malloc(sizeof(double) * 3 * k * T), *pj = malloc(sizeof(double) * 3 * k * T);
+    double *pp = malloc(sizeof(double) * k * T), *pd = malloc(sizeof(double) * k * T);
+    int *pn = malloc(sizeof(int) * k * T);
+#pragma omp parallel for schedule(static, 1) num_threads(T)
+    for (int c = 0; c < T; ++c) {
+        const int j0 = c * chunk, nc = j0 < n ? (n - j0 < chunk ? n - j0 : chunk) : 0;
+        or_force(k, xi, vi, excl, nc, xp + 3 * (size_t)j0, vp + 3 * (size_t)j0, M + j0,
+                 IDX + j0, eps2, pa + 3 * k * c, pj + 3 * k * c, pp + k * c, pn + k * c,
+                 pd + k * c);
+    }
+    for (int q = 0; q < k; ++q) {
+        double a[3] = {0, 0, 0}, jk[3] = {0, 0, 0}, p = 0, best = pd[q];
+        int nn = pn[q];
+        for (int c = 0; c < T; ++c) {
+            for (int d = 0; d < 3; ++d) {
+                a[d] += pa[3 * (k * c + q) + d];
+                jk[d] += pj[3 * (k * c + q) + d];
+            }
+            p += pp[k * c + q];
+            if (c > 0 && pd[k * c + q] < best) {   /* strict <: the lower range wins ties */
+                best = pd[k * c + q];
+                nn = pn[k * c + q];
+            }
+        }
+        memcpy(r_acc + 3 * q, a, sizeof a);
+        memcpy(r_jerk + 3 * q, jk, sizeof jk);
+        r_pot[q] = p;
+        r_nn[q] = nn;
+    }
+    free(pa); free(pj); free(pp); free(pd); free(pn);
+}
+
 void g6calc_firsthalf_(int *c, int *nj, int *ni, int index[], double xi[][3],
                        double vi[][3], double aold[][3], double j6old[][3],
                        double phiold[], double *eps2, double h2[])
@@ -155,8 +199,11 @@ void g6calc_firsthalf_(int *c, int *nj, int *ni, int index[], double xi[][3],
     r_pot = realloc(r_pot, sizeof(double) * k);
     r_nn = realloc(r_nn, sizeof(int) * k);
     double *d2 = malloc(sizeof(double) * k);
-    or_force(k, &xi[0][0], &vi[0][0], index, n, xp, vp, M, IDX, *eps2, r_acc, r_jerk, r_pot,
-             r_nn, d2);
+    if (jsplit > 1 && k < 64 && n >= 64 * jsplit)
+        force_jsplit(k, &xi[0][0], &vi[0][0], index, n, *eps2);
+    else
+        or_force(k, &xi[0][0], &vi[0][0], index, n, xp, vp, M, IDX, *eps2, r_acc, r_jerk,
+                 r_pot, r_nn, d2);
     free(d2);
     last_ni = k;
     last_nj = n;

@@ -15,7 +15,13 @@ libg6 through the GRAPE-6 ABI (eps = 0, eta = 0.1).
 * every particle lands on t = 0.25 and block times are exact binary
   fractions (the power-of-two schedule);
 * the total energy (FP64 oracle pair sum) changes by |dE/E| <= 1e-4.
+
+The same model at N = 4096, a size the CPU integrates in seconds, runs on
+libg6 and on the FP64 oracle behind the same ABI: libg6's energy drift stays
+within 10x the all-CPU run's (+ 1e-8).
 """
+import os
+import subprocess
 import time
 
 import numpy as np
@@ -30,6 +36,7 @@ pytestmark = pytest.mark.gpu
 
 N = 262144
 T_END = 0.25
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_config3_king_salpeter_binaries_262144_to_quarter():
@@ -97,3 +104,39 @@ def test_config3_king_salpeter_binaries_262144_to_quarter():
     assert be.worst["pot_rel"] <= 1e-6
     assert np.percentile(np.concatenate(be.rel_errs["acc"]), 99.9) <= 1e-6
     assert de <= 1e-4
+
+
+def test_config3_model_energy_drift_against_cpu_run(tmp_path):
+    """Config 3's model -- King W0 = 6, Salpeter 1-100, 5 % binaries as their
+    centres of mass, the same generator and seed -- at N = 4096, integrated
+    to t = 0.25 twice by the native driver: forces from libg6, and from the
+    FP64 oracle behind the same GRAPE-6 ABI on the host (tests/oracle_g6abi.c,
+    bit-exact oracle build).  A CPU run at N = 262144 takes ~2400 s of force
+    alone, so the drift comparison runs at this size, the full-size run
+    above keeps its absolute bound.  libg6's |dE/E| stays within 10x the
+    all-CPU run's + 1e-8, the bound of config 1; both runs land every
+    particle on t = 0.25."""
+    lib = str(tmp_path / "liboracle_g6abi.so")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-fopenmp",
+                    f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(ROOT, 'oracle')}",
+                    os.path.join(ROOT, "tests", "oracle_g6abi.c"),
+                    os.path.join(ROOT, "oracle", "g6_oracle.c"), "-lm", "-o", lib], check=True)
+    n = 4096
+    c = ics.king_cluster(n, 42)
+    m, x, v = c["m"], c["x"], c["v"]
+    e0 = sum(oracle.energies(m, x, v, 0.0))
+    gpu = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(T_END)
+    os.environ["ORACLE_G6ABI_JSPLIT"] = "16"
+    try:
+        cpu = NativeBlockHermite(m, x, v, eta=0.1, eps2=0.0).run(T_END, grape_lib=lib)
+    finally:
+        del os.environ["ORACLE_G6ABI_JSPLIT"]
+    de_gpu = abs(sum(oracle.energies(gpu.m, gpu.x, gpu.v, 0.0)) / e0 - 1)
+    de_cpu = abs(sum(oracle.energies(cpu.m, cpu.x, cpu.v, 0.0)) / e0 - 1)
+    dx = np.linalg.norm(gpu.x - cpu.x, axis=1)
+    print(f"N={n} binaries {len(c['bin_index'])}: gpu blocks {gpu.blocks} steps "
+          f"{gpu.particle_steps} {gpu.seconds:.2f}s dE/E {de_gpu:.3e}; cpu blocks {cpu.blocks} "
+          f"steps {cpu.particle_steps} {cpu.seconds:.2f}s dE/E {de_cpu:.3e}; |x_gpu - x_cpu| "
+          f"median {np.median(dx):.2e} max {dx.max():.2e}", flush=True)
+    assert np.all(gpu.t == T_END) and np.all(cpu.t == T_END)
+    assert de_gpu <= 10 * de_cpu + 1e-8
