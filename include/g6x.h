@@ -113,6 +113,13 @@ int g6x_sync(int cluster_id);
  *                    softening predict j inside the force kernel (no
  *                    predictor launch, no JPred round trip; bitwise the
  *                    same results); 0 = predictor first   G6_FUSED_SMALL
+ *   "small_fp64"     1 (default) = ABI blocks of at most 8 i with one
+ *                    softening run as one launch: FP64 pair arithmetic with
+ *                    the j-particles across lanes, the reduction in the same
+ *                    kernel, up to 16 pending j updates carried in the launch
+ *                    arguments; results within ~1e-15 of the FP64 direct sum,
+ *                    nearest neighbours exact.  0 = the FP32 path
+ *                    G6_SMALL_FP64
  *   "fail_lasthalf", "fail_read_nl", "fail_nn"   host-side failure injection
  *                    for a caller's recovery path (kira resets, reloads and
  *                    retries, hdyn_grape6.C:405-438, 853-900, 2110-2166):
@@ -128,7 +135,8 @@ long long g6x_get_option(const char *key);
 
 /* Per-kernel event timing, accumulated since the last reset (needs
  * "timing" = 1).  kernel: 0 predictor, 1 force (with the prediction for
- * fused small blocks), 2 reduce (includes the
+ * fused small blocks, and the whole calculation of a "small_fp64" block),
+ * 2 reduce (includes the
  * FP64 nearest-neighbour rescan of ambiguous i), 3 unused (0 launches),
  * 4 neighbour list, 5 j scatter, 6 potential (g6x_potential), 7 k-nearest
  * neighbours (g6x_densities).  Returns launches; *ms = total ms.  Kernel 4
@@ -137,7 +145,9 @@ int  g6x_kernel_time(int kernel, double *ms);
 void g6x_reset_timing(void);
 
 /* Geometry of the last force launch: workgroups, j-split, j per split.
- * Returns the kernel variant (0: scalar-load kernel, 1: LDS-staged). */
+ * Returns the kernel variant (0: scalar-load kernel, 1: LDS-staged,
+ * 2: the one-launch FP64 kernel of small blocks, whose j-split is its
+ * workgroup count and whose j per split is 256 x j per lane). */
 int  g6x_last_force_geometry(int *workgroups, int *nsplit, int *jchunk);
 
 /* Number of nearest-neighbour refinements (FP64 rescans) in the last call. */

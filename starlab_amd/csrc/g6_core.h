@@ -44,6 +44,7 @@ struct Options {
     int reduce_lanes = 0;     // lanes per i in the split reduction: 0 auto, 4..64
     int wpg = 0;              // waves per force workgroup (j-splits folded in LDS): 0 auto, 1..16
     int fused_small = 1;      // ABI blocks of one i-group predict j inside the force kernel
+    int small_fp64 = 1;       // ABI blocks of at most kSmallMaxI i: one FP64 launch (k_small)
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every
                               // size so far, so off unless asked for)
@@ -128,6 +129,9 @@ struct Engine {
     // ---- partial sums -----------------------------------------------------
     size_t pbytes = 0;
     char *d_part = nullptr;
+    // k_small: its workgroup partials (small_part_doubles()), then its
+    // ticket counter (zero between launches)
+    double *d_small = nullptr;
 
     // ---- results ----------------------------------------------------------
     int rcap = 0;

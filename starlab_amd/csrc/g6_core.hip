@@ -73,6 +73,7 @@ void read_env()
     o.reduce_lanes = (int)env_ll("REDUCE_LANES", o.reduce_lanes);
     o.wpg = (int)env_ll("WPG", o.wpg);
     o.fused_small = (int)env_ll("FUSED_SMALL", o.fused_small);
+    o.small_fp64 = (int)env_ll("SMALL_FP64", o.small_fp64);
     o.fail_lasthalf = (int)env_ll("FAIL_LASTHALF", o.fail_lasthalf);
     o.fail_read_nl = (int)env_ll("FAIL_READNL", o.fail_read_nl);
     o.fail_nn = (int)env_ll("FAIL_NN", o.fail_nn);
@@ -624,6 +625,7 @@ int do_close()
     E.synced_seq = E.op_seq;
     E.icap = 0;
     dfree(E.d_part); E.pbytes = 0;
+    dfree(E.d_small);
     dfree(E.d_nblist); hfree(E.h_nbcount); hfree(E.h_nblist);
     dfree(E.d_nlscratch); E.nlscratch_cap = 0;
     hfree(E.h_nlinfo); E.nlinfo_cap = 0;
@@ -720,6 +722,138 @@ void index_map_set(int index, int address)
     }
 }
 
+// Stage the i-block in the pinned i buffer (layout of ibuf_bytes(ld)) and
+// return its stride ld, or -1.
+int stage_iblock(int ni, const double (*xi)[3], const double (*vi)[3], const int *excl,
+                 const double *h2, const double *e2)
+{
+    // the pinned i buffer may still be read by the previous upload
+    if (wait_queued(E.ibuf_seq) || ensure_i(ni)) return -1;
+    const int ld = (ni + 63) & ~63;
+    float *f = reinterpret_cast<float *>(E.h_ibuf);
+    int *ex = reinterpret_cast<int *>(f + 9 * (size_t)ld);
+    double *d = reinterpret_cast<double *>(E.h_ibuf + (size_t)ld * 40);
+    double *hh = d + 3 * (size_t)ld;
+    double *ee = d + 4 * (size_t)ld;
+    for (int k = 0; k < ld; ++k) {
+        const int s = k < ni ? k : ni - 1;
+        for (int c = 0; c < 3; ++c) {
+            const double x = xi[s][c];
+            const float hi = (float)x;
+            f[(0 + c) * (size_t)ld + k] = hi;
+            f[(3 + c) * (size_t)ld + k] = (float)(x - (double)hi);
+            f[(6 + c) * (size_t)ld + k] = (float)vi[s][c];
+            d[c * (size_t)ld + k] = x;
+            d[(5 + c) * (size_t)ld + k] = vi[s][c];
+        }
+        ex[k] = (k < ni && excl) ? excl[k] : -1;
+        hh[k] = (k < ni && h2) ? h2[k] : 0.0;
+        ee[k] = (k < ni && e2) ? e2[k] : 0.0;
+    }
+    return ld;
+}
+
+// One launch: the deferred j scatter and the upload of the staged i-block
+// (ld > 0; ld = 0 uploads nothing).
+int scatter_with_iblock(int ld)
+{
+    const int nd = E.deferred_scatter;
+    if (nd <= 0 && ld <= 0) return 0;
+    E.deferred_scatter = 0;
+    KTimer t(K_SCATTER);
+    HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex, E.h_jraw_dev,
+                          reinterpret_cast<const int *>(E.h_stage_dev), nd, E.h_ibuf_dev,
+                          E.d_ibuf, ld > 0 ? ibuf_bytes(ld) : 0));
+    t.stop();
+    const unsigned long long seq = mark_queued();
+    if (ld > 0) E.ibuf_seq = seq;
+    if (nd > 0) E.stage_seq = E.mirror_seq = seq;
+    return 0;
+}
+
+// A block of at most kSmallMaxI i with one softening and no zero-distance
+// rule: one k_small launch (FP64 pairs, the reduction inside).  Pending j
+// updates travel in its arguments when they fit and lie below nj; otherwise
+// the scatter launch runs first, as for other blocks.  With neighbour
+// spheres (lists) the i-block is uploaded too and k_small writes the FP64
+// prediction, both of which the list kernels read.
+int calc_small(int nj, int ni, const double (*xi)[3], const double (*vi)[3], const int *excl,
+               const double *h2, const double *e2, double eps2, bool lists)
+{
+    // the pinned results buffer (and the i buffer) come with the i capacity
+    if (ensure_i(ni)) return -1;
+    if (!E.h_res_dev) return fail(-1, "firsthalf: no results buffer");
+    const size_t npart = small_part_doubles();
+    if (!E.d_small) {
+        HIPCHK(hipMalloc((void **)&E.d_small, sizeof(double) * npart + 256));
+        HIPCHK(hipMemsetAsync(E.d_small + npart, 0, 256, E.stream));
+    }
+    static SmallBlock B;   // 3.6 KB, copied into the launch's arguments
+    B.ti = E.ti;
+    B.eps2 = eps2;
+    B.ni = ni;
+    B.nj = nj;
+    B.order = E.opt.predict_order;
+    B.write_pred = lists ? 1 : 0;
+    B.nd = 0;
+    for (int k = 0; k < ni; ++k) {
+        for (int c = 0; c < 3; ++c) {
+            B.xi[k][c] = xi[k][c];
+            B.vi[k][c] = vi[k][c];
+        }
+        B.excl[k] = excl ? excl[k] : -1;
+    }
+    const int nd = E.deferred_scatter;
+    const int *list = reinterpret_cast<const int *>(E.h_stage);
+    bool inline_j = !lists && nd <= kSmallMaxDirty;
+    for (int k = 0; k < nd && inline_j; ++k) inline_j = list[k] < nj;
+    if (nd > 0 && inline_j) {
+        for (int k = 0; k < nd; ++k) {
+            B.daddr[k] = list[k];
+            B.dindex[k] = list[nd + k];
+            B.drec[k] = E.h_jraw[list[k]];
+        }
+        B.nd = nd;
+        E.deferred_scatter = 0;
+    } else {
+        const int ld = lists ? stage_iblock(ni, xi, vi, excl, h2, e2) : 0;
+        if (ld < 0 || scatter_with_iblock(ld)) return -1;
+    }
+    const Results r = results_at(E.h_res_dev, ni);
+    KTimer t(K_FORCE);
+    const hipError_t e = launch_small(E.stream, B, E.d_jraw, E.d_jindex, E.d_xp64, E.d_vp64,
+                                      E.dcap, E.d_small,
+                                      reinterpret_cast<unsigned *>(E.d_small + npart), r);
+    t.stop();
+    if (e != hipSuccess) {
+        // updates carried in the arguments never reached HBM: send them again
+        for (int k = 0; k < B.nd; ++k) {
+            const int a = B.daddr[k];
+            if (!E.dirty_flag[a]) {
+                E.dirty_flag[a] = 1;
+                E.dirty.push_back(a);
+            }
+        }
+        return fail(-1, "k_small launch: %s", hipGetErrorString(e));
+    }
+    if (B.nd > 0) E.pred_valid = false;
+    E.xp_valid = lists;
+    if (lists) {
+        E.xp_ti = E.ti;
+        E.xp_nj = nj;
+    }
+    int wgs, chunks;
+    small_grid(nj, wgs, chunks);
+    E.geom_variant = 2;
+    E.geom_nsplit = wgs;
+    E.geom_jchunk = kSmallThreads * chunks;
+    E.geom_wg = wgs;
+    E.geom_tile = 0;
+    E.geom_wpg = 0;
+    E.geom_lanes = 64;
+    return 0;
+}
+
 int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const double (*vi)[3],
                const int *excl, const double *h2, const double *e2, const IGroup *groups,
                int ngroups, int include_self)
@@ -740,76 +874,53 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     if (covered != ni) return fail(-1, "%s: i-groups cover %d of %d slots", who, covered, ni);
     // the j scatter is launched below, together with the i-block upload
     if (flush_j(nj, true)) return -1;
-    // A block of at most kFusedMaxGroups i-groups with one softening predicts
-    // j inside the force kernel (the predictor launch and the JPred round
-    // trip dominate such blocks); any other block runs the predictor first.
-    const bool fused = E.opt.fused_small && ngroups == 1 && ni <= 64 * kFusedMaxGroups &&
-                       choose_geometry(ni, nj).variant == 0;
-    // the pinned i buffer may still be read by the previous upload
-    if (wait_queued(E.ibuf_seq) || ensure_i(ni)) return -1;
-    const int ld = (ni + 63) & ~63;
-    float *f = reinterpret_cast<float *>(E.h_ibuf);
-    int *ex = reinterpret_cast<int *>(f + 9 * (size_t)ld);
-    double *d = reinterpret_cast<double *>(E.h_ibuf + (size_t)ld * 40);
-    double *hh = d + 3 * (size_t)ld;
-    double *ee = d + 4 * (size_t)ld;
     // no h2 array: no neighbour spheres
     if (h2) E.last_h2.assign(h2, h2 + ni);
     else E.last_h2.assign(ni, 0.0);
-    for (int k = 0; k < ld; ++k) {
-        const int s = k < ni ? k : ni - 1;
-        for (int c = 0; c < 3; ++c) {
-            const double x = xi[s][c];
-            const float hi = (float)x;
-            f[(0 + c) * (size_t)ld + k] = hi;
-            f[(3 + c) * (size_t)ld + k] = (float)(x - (double)hi);
-            f[(6 + c) * (size_t)ld + k] = (float)vi[s][c];
-            d[c * (size_t)ld + k] = x;
-            d[(5 + c) * (size_t)ld + k] = vi[s][c];
+    bool lists = false;
+    for (int k = 0; k < ni && h2; ++k) lists = lists || h2[k] > 0;
+    if (E.opt.small_fp64 && ngroups == 1 && !groups[0].zskip && ni <= kSmallMaxI &&
+        nj <= E.dcap) {
+        if (calc_small(nj, ni, xi, vi, excl, h2, e2, groups[0].eps2, lists)) return -1;
+    } else {
+        // A block of at most kFusedMaxGroups i-groups with one softening
+        // predicts j inside the force kernel (the predictor launch and the
+        // JPred round trip dominate such blocks); any other block runs the
+        // predictor first.
+        const bool fused = E.opt.fused_small && ngroups == 1 && ni <= 64 * kFusedMaxGroups &&
+                           choose_geometry(ni, nj).variant == 0;
+        const int ld = stage_iblock(ni, xi, vi, excl, h2, e2);
+        if (ld < 0) return -1;
+        if (scatter_with_iblock(ld)) return -1;
+        if (!fused && predict(nj, false)) return -1;
+        const IBlock ib = iblock_at(E.d_ibuf, ld);
+        // the reduction writes the results straight into pinned host memory
+        const Results r = results_at(E.h_res_dev, ni);
+        for (int g = 0; g < ngroups; ++g) {
+            // a group is the block shifted by its first slot (same stride); the
+            // partial-sum buffer is reused in stream order
+            const int o = groups[g].off;
+            IBlock gb = ib;
+            gb.xh += o; gb.yh += o; gb.zh += o; gb.xl += o; gb.yl += o; gb.zl += o;
+            gb.vx += o; gb.vy += o; gb.vz += o; gb.excl += o;
+            gb.x64 += o; gb.y64 += o; gb.z64 += o;
+            gb.vx64 += o; gb.vy64 += o; gb.vz64 += o;
+            Results gr = r;
+            gr.acc += 3 * (size_t)o; gr.jerk += 3 * (size_t)o; gr.pot += o;
+            gr.nn += o; gr.flags += o;
+            const FusedPredict fp{E.d_jraw, E.ti, E.opt.predict_order, E.d_xp64, E.d_vp64,
+                                  E.dcap};
+            if (fused) {
+                // xp64 / vp64 now hold this block's prediction, JPred an older one
+                E.pred_valid = false;
+                E.xp_valid = true;
+                E.xp_ti = E.ti;
+                E.xp_nj = nj;
+            }
+            if (enqueue_force(groups[g].n, nj, groups[g].eps2, gb, gr, groups[g].zskip,
+                              fused ? &fp : nullptr))
+                return -1;
         }
-        ex[k] = (k < ni && excl) ? excl[k] : -1;
-        hh[k] = (k < ni && h2) ? h2[k] : 0.0;
-        ee[k] = (k < ni && e2) ? e2[k] : 0.0;
-    }
-    {
-        // one launch: the deferred j scatter and the i-block upload
-        const int nd = E.deferred_scatter;
-        E.deferred_scatter = 0;
-        KTimer t(K_SCATTER);
-        HIPCHK(launch_scatter(E.stream, E.d_jraw, E.d_jindex, E.h_jraw_dev,
-                              reinterpret_cast<const int *>(E.h_stage_dev), nd, E.h_ibuf_dev,
-                              E.d_ibuf, ibuf_bytes(ld)));
-        t.stop();
-        E.ibuf_seq = mark_queued();
-        if (nd > 0) E.stage_seq = E.mirror_seq = E.ibuf_seq;
-    }
-    if (!fused && predict(nj, false)) return -1;
-    const IBlock ib = iblock_at(E.d_ibuf, ld);
-    // the reduction writes the results straight into pinned host memory
-    const Results r = results_at(E.h_res_dev, ni);
-    for (int g = 0; g < ngroups; ++g) {
-        // a group is the block shifted by its first slot (same stride); the
-        // partial-sum buffer is reused in stream order
-        const int o = groups[g].off;
-        IBlock gb = ib;
-        gb.xh += o; gb.yh += o; gb.zh += o; gb.xl += o; gb.yl += o; gb.zl += o;
-        gb.vx += o; gb.vy += o; gb.vz += o; gb.excl += o;
-        gb.x64 += o; gb.y64 += o; gb.z64 += o;
-        gb.vx64 += o; gb.vy64 += o; gb.vz64 += o;
-        Results gr = r;
-        gr.acc += 3 * (size_t)o; gr.jerk += 3 * (size_t)o; gr.pot += o;
-        gr.nn += o; gr.flags += o;
-        const FusedPredict fp{E.d_jraw, E.ti, E.opt.predict_order, E.d_xp64, E.d_vp64, E.dcap};
-        if (fused) {
-            // xp64 / vp64 now hold this block's prediction, JPred an older one
-            E.pred_valid = false;
-            E.xp_valid = true;
-            E.xp_ti = E.ti;
-            E.xp_nj = nj;
-        }
-        if (enqueue_force(groups[g].n, nj, groups[g].eps2, gb, gr, groups[g].zskip,
-                          fused ? &fp : nullptr))
-            return -1;
     }
     E.res_seq = mark_queued();
     HIPCHK(hipEventRecord(E.ev_done, E.stream));

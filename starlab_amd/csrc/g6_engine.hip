@@ -276,7 +276,13 @@ int g6x_read_predicted(int, int nj, double *xp, double *vp)
 {
     if (ensure_open("g6x_read_predicted")) return -1;
     if (nj < 0) return fail(-1, "g6x_read_predicted: nj = %d", nj);
-    if (!E.xp_valid || nj > E.xp_nj) return fail(-1, "g6x_read_predicted: no prediction");
+    if (!E.xp_valid || nj > E.xp_nj) {
+        // the last calculation did not keep an FP64 prediction (a small
+        // block without neighbour spheres): predict at the current ti
+        if (E.n_addr == 0 || nj > E.n_addr)
+            return fail(-1, "g6x_read_predicted: no prediction");
+        if (flush_j(nj) || predict(nj, true)) return -1;
+    }
     std::vector<double> buf(3 * (size_t)nj);
     for (int pass = 0; pass < 2; ++pass) {
         const double *src = pass ? E.d_vp64 : E.d_xp64;
@@ -335,6 +341,7 @@ int g6x_set_option(const char *key, long long value)
     else if (k == "lds_min_ni") o.lds_min_ni = (int)value;
     else if (k == "tile") o.tile = (int)value;
     else if (k == "fused_small") o.fused_small = value != 0 ? 1 : 0;
+    else if (k == "small_fp64") o.small_fp64 = value != 0 ? 1 : 0;
     else if (k == "fail_lasthalf") o.fail_lasthalf = (int)std::max(0LL, value);
     else if (k == "fail_read_nl") o.fail_read_nl = (int)std::max(0LL, value);
     else if (k == "fail_nn") o.fail_nn = (int)std::max(0LL, value);
@@ -372,6 +379,7 @@ long long g6x_get_option(const char *key)
     if (k == "fail_read_nl") return o.fail_read_nl;
     if (k == "fail_nn") return o.fail_nn;
     if (k == "fused_small") return o.fused_small;
+    if (k == "small_fp64") return o.small_fp64;
     if (k == "last_tile") return E.geom_tile;
     if (k == "reduce_lanes") return o.reduce_lanes;
     if (k == "wpg") return o.wpg;

@@ -140,6 +140,38 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
                          const Results &r, int lanes, const NearTile &nt);
+// Small ABI blocks in one launch (k_small): at most kSmallMaxI i with one
+// softening, FP64 pair arithmetic; each wave predicts 64 j at a time (one
+// per lane) into LDS and splits its lanes into one group per i.
+// The i-block and up to kSmallMaxDirty pending j updates travel in the
+// kernel arguments: the lane that owns an updated address takes the record
+// from there and also writes it to jraw / jindex.  Each workgroup writes its
+// partial sums; the last one to finish (a ticket counter) sums them in
+// workgroup order and writes the results.  write_pred: also write the FP64
+// prediction (xp64, vp64, m64) as the predictor kernel does.
+// blocks of up to 8 i: from about 12 i on, the i-per-lane FP32 path is as
+// fast (its cost hardly grows with ni at this size)
+constexpr int kSmallMaxI = 8;
+constexpr int kSmallMaxDirty = 16;
+constexpr int kSmallMaxWG = 512;      // workgroups per launch at most
+constexpr int kSmallThreads = 256;    // threads per workgroup
+constexpr int kSmallComp = 9;         // ax ay az jx jy jz pot, nearest d2, its address
+struct SmallBlock {
+    double ti, eps2;
+    int ni, nj, nd, order, write_pred, pad;
+    double xi[kSmallMaxI][3], vi[kSmallMaxI][3];
+    int excl[kSmallMaxI];               // address excluded for each i (-1: none)
+    int daddr[kSmallMaxDirty], dindex[kSmallMaxDirty];
+    JRaw drec[kSmallMaxDirty];
+};
+static_assert(sizeof(SmallBlock) <= 3712, "kernel argument block");
+// workgroups and 64-j chunks per wave for nj j-particles
+void small_grid(int nj, int &wgs, int &chunks);
+size_t small_part_doubles();   // partial-sum buffer of one launch
+hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jindex,
+                        double *xp64, double *vp64, int cap, double *part, unsigned *ticket,
+                        const Results &r);
+
 // Potential only: pot[i] = -sum_j m_j / r_ij for the i-block (self excluded
 // by address as in the force kernel); part holds nsplit * ib.ld doubles.
 hipError_t launch_pot(hipStream_t s, const JPred *jpred, int nj, const IBlock &ib, int ni,

@@ -68,7 +68,8 @@ __device__ __forceinline__ void split_ds(double x, float &hi, float &lo)
 
 // One j-particle predicted to ti (the predictor kernel and the fused
 // small-block force kernel share it, so both round identically).
-__device__ __forceinline__ void predict_one(const JRaw &r, double ti, int order, double xp[3],
+template <class Rec>
+__device__ __forceinline__ void predict_one(const Rec &r, double ti, int order, double xp[3],
                                             double vp[3])
 {
 #pragma clang fp contract(off)
@@ -816,10 +817,10 @@ constexpr int kReduceWaves = 4;   // waves per reduction workgroup
 // y^2), each squaring the error: ~2^-46, then below an FP64 rounding.  That
 // is a quarter of the instructions of a correctly rounded divide and square
 // root, and agrees with them to a few FP64 ulps.
-__device__ __forceinline__ void pair64(double dx, double dy, double dz, double du, double dv,
-                                       double dw, double m, double eps2, double F[7])
+__device__ __forceinline__ void pair64_r2(double dx, double dy, double dz, double du,
+                                          double dv, double dw, double m, double r2,
+                                          double F[7])
 {
-    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
     double y = __builtin_amdgcn_rsq(r2);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -837,6 +838,12 @@ __device__ __forceinline__ void pair64(double dx, double dy, double dz, double d
     F[4] += mr3inv * (dv + a3 * dy);
     F[5] += mr3inv * (dw + a3 * dz);
     F[6] -= mrinv;
+}
+
+__device__ __forceinline__ void pair64(double dx, double dy, double dz, double du, double dv,
+                                       double dw, double m, double eps2, double F[7])
+{
+    pair64_r2(dx, dy, dz, du, dv, dw, m, dx * dx + dy * dy + dz * dz + eps2, F);
 }
 
 template <int LOG2L>
@@ -1046,6 +1053,338 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Small ABI blocks in one launch (at most 8 i; kira's commonest blocks hold
+// 1-7 nodes).  A block this small leaves most lanes idle in the
+// i-per-lane force kernel and pays three launches (scatter, force,
+// reduction).  Here each wave takes 64 j at a time, one per lane, predicts
+// them (the predictor's arithmetic, so the same FP64 prediction) into a
+// wave-private LDS area, and then splits its lanes into NIP groups of
+// Q = 64 / NIP, one group per i: lane q of a group adds the pairs of j
+// q, q + Q, ... of the 64 in FP64 (pair64, the pair law of
+// hdyn_ev.C:2019-2036).  The sums carry no FP32 term and the nearest
+// neighbour is the exact FP64 minimum (lowest address on ties,
+// hdyn_inline.C:44).  A group combines its Q lanes once, at the end, by DPP
+// row moves; the waves of a workgroup combine in wave order; each workgroup
+// writes one partial per i, and the last workgroup to take a ticket adds
+// them in workgroup order.  Every order is fixed by (nj, ni, grid), so the
+// results are deterministic.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void nn_min(double &bd, int &ba, double od, int oa)
+{
+    if (od < bd || (od == bd && oa < ba)) {
+        bd = od;
+        ba = oa;
+    }
+}
+
+// v from another lane of the same row, by DPP control CTRL (all four rows
+// and banks enabled; every pattern used reads a lane inside the row)
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_i32<CTRL>((int)(unsigned)u);
+    const unsigned hi = (unsigned)dpp_i32<CTRL>((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ void group_step_dpp(double F[7], double &bd, int &ba)
+{
+#pragma unroll
+    for (int c = 0; c < 7; ++c) F[c] += dpp_f64<CTRL>(F[c]);
+    const double od = dpp_f64<CTRL>(bd);
+    const int oa = dpp_i32<CTRL>(ba);
+    nn_min(bd, ba, od, oa);
+}
+
+__device__ __forceinline__ void group_step_xor(double F[7], double &bd, int &ba, int off)
+{
+#pragma unroll
+    for (int c = 0; c < 7; ++c) F[c] += __shfl_xor(F[c], off);
+    const double od = __shfl_xor(bd, off);
+    const int oa = __shfl_xor(ba, off);
+    nn_min(bd, ba, od, oa);
+}
+
+// Sum F and the nearest (bd, ba) over groups of Q consecutive lanes (Q a
+// power of two, 4..64); afterwards every lane of a group holds the group's
+// values (each step adds a lane and its partner in either order, which
+// gives the same bits).  Pairs: xor 1 and xor 2 inside quads (quad_perm),
+// then the other quad of 8 (row_half_mirror), the other half of the row of
+// 16 (row_mirror), then across rows.
+template <int Q>
+__device__ __forceinline__ void group_reduce(double F[7], double &bd, int &ba)
+{
+    group_step_dpp<0xB1>(F, bd, ba);    // quad_perm [1,0,3,2]
+    group_step_dpp<0x4E>(F, bd, ba);    // quad_perm [2,3,0,1]
+    if (Q >= 8) group_step_dpp<0x141>(F, bd, ba);   // row_half_mirror
+    if (Q >= 16) group_step_dpp<0x140>(F, bd, ba);  // row_mirror
+    if (Q >= 32) group_step_xor(F, bd, ba, 16);
+    if (Q >= 64) group_step_xor(F, bd, ba, 32);
+}
+
+struct JRawLite {   // the fields the predictor reads
+    double x[3], v[3], a2[3], j6[3], k18[3], tj, m;
+};
+
+__device__ __forceinline__ JRawLite load_lite(const JRaw &r, int order)
+{
+    JRawLite o;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o.x[k] = r.x[k];
+        o.v[k] = r.v[k];
+        o.a2[k] = r.a2[k];
+        o.j6[k] = r.j6[k];
+        o.k18[k] = order >= 5 ? r.k18[k] : 0.0;
+    }
+    o.tj = r.tj;
+    o.m = r.m;
+    return o;
+}
+
+template <int NIP>
+__global__ __launch_bounds__(kSmallThreads) void k_small(
+    const SmallBlock B, JRaw *__restrict__ jraw, int *__restrict__ jindex,
+    double *__restrict__ xp64, double *__restrict__ vp64, int cap, int chunks,
+    double *__restrict__ part, unsigned *__restrict__ ticket, Results R)
+{
+#pragma clang fp contract(off)
+    constexpr int NW = kSmallThreads / 64;
+    constexpr int Q = 64 / NIP;
+    __shared__ double sj[NW][7][64];   // per wave: predicted x y z vx vy vz, m of its 64 j
+    __shared__ double red[NW][NIP][kSmallComp];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const int nj = B.nj, ni = B.ni;
+    const int slot = lane / Q, q = lane - slot * Q;
+    const int ic = slot < ni ? slot : ni - 1;
+    const double xi = B.xi[ic][0], yi = B.xi[ic][1], zi = B.xi[ic][2];
+    const double ui = B.vi[ic][0], vi = B.vi[ic][1], wi = B.vi[ic][2];
+    const int ex = B.excl[ic];
+
+    double F[7] = {0, 0, 0, 0, 0, 0, 0};
+    double bd = __builtin_inf();
+    int ba = INT_MAX;
+    // chunk c of this wave: j in [jbase(c), jbase(c) + 64); the next chunk's
+    // record is loaded while this one is summed
+    auto jbase = [&](int c) { return ((c * G + (int)blockIdx.x) * NW + wave) * 64; };
+    JRawLite nxt{};
+    if (jbase(0) + lane < nj) nxt = load_lite(jraw[jbase(0) + lane], B.order);
+    for (int c = 0; c < chunks; ++c) {
+        const int j0 = jbase(c);
+        if (j0 >= nj) break;   // uniform in the wave
+        const int jn = min(64, nj - j0);
+        const int j = j0 + lane;
+        JRawLite cur = nxt;
+        if (c + 1 < chunks && jbase(c + 1) + lane < nj)
+            nxt = load_lite(jraw[jbase(c + 1) + lane], B.order);
+        // pending updates inside this wave's 64 j: lane k tests update k, one
+        // ballot names those in range (usually none), and for each the lane
+        // that owns the address takes the record and stores it to HBM
+        {
+            const unsigned mine = lane < B.nd ? (unsigned)(B.daddr[lane] - j0) : ~0u;
+            unsigned long long hits = __ballot(mine < (unsigned)jn);
+            while (hits) {
+                const int k = __builtin_ctzll(hits);
+                hits &= hits - 1;
+                if (lane == B.daddr[k] - j0) {
+                    const JRaw &d = B.drec[k];
+                    cur = load_lite(d, B.order);
+                    jraw[j] = d;
+                    jindex[j] = B.dindex[k];
+                }
+            }
+        }
+        if (lane < jn) {
+            double xp[3], vp[3];
+            predict_one(cur, B.ti, B.order, xp, vp);
+            if (B.write_pred) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    xp64[(size_t)k * cap + j] = xp[k];
+                    vp64[(size_t)k * cap + j] = vp[k];
+                }
+                vp64[3 * (size_t)cap + j] = cur.m;   // m64
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                sj[wave][k][lane] = xp[k];
+                sj[wave][3 + k][lane] = vp[k];
+            }
+            sj[wave][6][lane] = cur.m;
+        }
+        wave_lds_sync();
+#pragma unroll 2
+        for (int t = q; t < jn; t += Q) {
+            const int jj = j0 + t;
+            if (jj != ex) {
+                const double dx = sj[wave][0][t] - xi;
+                const double dy = sj[wave][1][t] - yi;
+                const double dz = sj[wave][2][t] - zi;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                pair64_r2(dx, dy, dz, sj[wave][3][t] - ui, sj[wave][4][t] - vi,
+                          sj[wave][5][t] - wi, sj[wave][6][t], d2 + B.eps2, F);
+                nn_min(bd, ba, d2, jj);
+            }
+        }
+        wave_lds_sync();   // the next chunk overwrites the staging area
+    }
+    group_reduce<Q>(F, bd, ba);
+    if (q == 0 && slot < ni) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) red[wave][slot][c] = F[c];
+        red[wave][slot][7] = bd;
+        red[wave][slot][8] = (double)ba;
+    }
+    __syncthreads();
+    // wave 0 writes this workgroup's partial per i (waves added in order),
+    // publishes it device-wide and takes a ticket; the last workgroup sees
+    // every other's partials
+    if (wave == 0) {
+        for (int v = lane; v < ni * 8; v += 64) {
+            const int i = v >> 3, c = v & 7;
+            double *col = part + ((size_t)i * kSmallComp + c) * G + blockIdx.x;
+            if (c < 7) {
+                double s = red[0][i][c];
+                for (int w = 1; w < NW; ++w) s += red[w][i][c];
+                col[0] = s;
+            } else {
+                double d = red[0][i][7];
+                int a = (int)red[0][i][8];
+                for (int w = 1; w < NW; ++w) nn_min(d, a, red[w][i][7], (int)red[w][i][8]);
+                col[0] = d;
+                col[G] = (double)a;   // component 8
+            }
+        }
+        __threadfence();
+        if (lane == 0) is_last = atomicAdd(ticket, 1u) == (unsigned)(G - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    // The last workgroup: NIP groups of QF threads, one group per i; thread
+    // fq of a group adds workgroups fq, fq + QF, ... in order, then the group
+    // combines its lanes (DPP) and, when it spans several waves, their sums
+    // in wave order through LDS.
+    constexpr int QF = kSmallThreads / NIP;
+    const int fs = threadIdx.x / QF, fq = threadIdx.x - fs * QF;
+    double S[7] = {0, 0, 0, 0, 0, 0, 0};
+    double sd = __builtin_inf();
+    int sa = INT_MAX;
+    if (fs < ni) {
+        // U workgroups' partials per round, all loads issued before the adds
+        // (the partials come from other XCDs: every round is a trip to
+        // memory); the adds keep workgroup order fq, fq + QF, ...  Groups of
+        // 256 threads or more see at most two rounds and keep the registers.
+        constexpr int U = QF >= 256 ? 1 : 4;
+        const double *col = part + (size_t)fs * kSmallComp * G;
+        for (int b0 = fq; b0 < G; b0 += U * QF) {
+            double v[U][kSmallComp];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int b = b0 + u * QF;
+#pragma unroll
+                for (int c = 0; c < kSmallComp; ++c)
+                    v[u][c] = b < G ? col[(size_t)c * G + b] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (b0 + u * QF >= G) break;
+#pragma unroll
+                for (int c = 0; c < 7; ++c) S[c] += v[u][c];
+                nn_min(sd, sa, v[u][7], (int)v[u][8]);
+            }
+        }
+    }
+    group_reduce<(QF < 64 ? QF : 64)>(S, sd, sa);
+    if (QF > 64) {
+        // red[][0] is free again: every wave's partials were read above
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < 7; ++c) red[wave][0][c] = S[c];
+            red[wave][0][7] = sd;
+            red[wave][0][8] = (double)sa;
+        }
+        __syncthreads();
+        if (fq == 0) {
+#pragma unroll 1
+            for (int w = 1; w < QF / 64; ++w) {
+#pragma unroll
+                for (int c = 0; c < 7; ++c) S[c] += red[wave + w][0][c];
+                nn_min(sd, sa, red[wave + w][0][7], (int)red[wave + w][0][8]);
+            }
+        }
+    }
+    if (fq == 0 && fs < ni) {
+        const int i = fs;
+        int nn = 0;
+        if (sa != INT_MAX) {
+            nn = jindex[sa];
+            // an index this launch wrote may not be visible here yet
+            for (int k = 0; k < B.nd; ++k)
+                if (B.daddr[k] == sa) nn = B.dindex[k];
+        }
+        R.acc[3 * i + 0] = S[0];
+        R.acc[3 * i + 1] = S[1];
+        R.acc[3 * i + 2] = S[2];
+        R.jerk[3 * i + 0] = S[3];
+        R.jerk[3 * i + 1] = S[4];
+        R.jerk[3 * i + 2] = S[5];
+        R.pot[i] = S[6];
+        R.nn[i] = nn;
+        R.flags[i] = 0;
+    }
+    if (threadIdx.x == 0) atomicExch(ticket, 0u);   // ready for the next launch
+}
+
+// workgroups and 64-j chunks per wave for nj j-particles
+void small_grid(int nj, int &wgs, int &chunks)
+{
+    wgs = std::min(kSmallMaxWG, std::max(1, (nj + kSmallThreads - 1) / kSmallThreads));
+    chunks = (nj + wgs * kSmallThreads - 1) / (wgs * kSmallThreads);
+}
+
+size_t small_part_doubles() { return (size_t)kSmallMaxI * kSmallComp * kSmallMaxWG; }
+
+hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jindex,
+                        double *xp64, double *vp64, int cap, double *part, unsigned *ticket,
+                        const Results &r)
+{
+    if (b.ni < 1 || b.ni > kSmallMaxI || b.nj < 1 || b.nj > cap || b.nd < 0 ||
+        b.nd > kSmallMaxDirty)
+        return hipErrorInvalidValue;
+    for (int k = 0; k < b.nd; ++k)
+        if (b.daddr[k] < 0 || b.daddr[k] >= b.nj) return hipErrorInvalidValue;
+    int wgs, chunks;
+    small_grid(b.nj, wgs, chunks);
+    const dim3 grid(wgs), block(kSmallThreads);
+    int nip = 1;
+    while (nip < b.ni) nip *= 2;
+#define G6_SMALL_LAUNCH(N)                                                                  \
+    hipLaunchKernelGGL(k_small<N>, grid, block, 0, s, b, jraw, jindex, xp64, vp64, cap,     \
+                       chunks, part, ticket, r)
+    switch (nip) {
+    case 1: G6_SMALL_LAUNCH(1); break;
+    case 2: G6_SMALL_LAUNCH(2); break;
+    case 4: G6_SMALL_LAUNCH(4); break;
+    case 8: G6_SMALL_LAUNCH(8); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef G6_SMALL_LAUNCH
     return hipGetLastError();
 }
 

@@ -374,6 +374,7 @@ def test_small_blocks_many_splits(g, nsplit):
     s = plummer_state(20000, seed=29)
     load_j(g, s)
     g.option("nsplit", nsplit)
+    g.option("small_fp64", 0)   # the FP32 kernels' shapes (blocks of <= 16 i default to FP64)
     try:
         rng = np.random.default_rng(5)
         for ni in (1, 3, 64, 65):
@@ -382,6 +383,7 @@ def test_small_blocks_many_splits(g, nsplit):
             _check(errs, f"ni={ni} nsplit={nsplit} tile={g.option('last_tile')}")
     finally:
         g.option("nsplit", 0)
+        g.option("small_fp64", 1)
 
 
 @pytest.mark.parametrize("tile", [16, 32])
@@ -393,6 +395,7 @@ def test_short_tiles(g, tile):
     load_j(g, s)
     g.option("force_variant", 0)
     g.option("tile", tile)
+    g.option("small_fp64", 0)
     try:
         rng = np.random.default_rng(tile)
         for ni, eps2 in ((1, 0.0), (40, 1e-8), (300, 0.0)):
@@ -403,6 +406,7 @@ def test_short_tiles(g, tile):
     finally:
         g.option("tile", 0)
         g.option("force_variant", -1)
+        g.option("small_fp64", 1)
 
 
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
@@ -515,6 +519,7 @@ def test_fused_small_blocks_equal_the_predictor_path(g, nj, eps2):
     rng = np.random.default_rng(nj)
     ti = 0.0
     xp, vp = _predict(s, ti)
+    g.option("small_fp64", 0)   # blocks of up to 16 i would take the FP64 kernel
     try:
         for ni in (1, 13, 64, 65, 200, 256, 257):
             isel = np.sort(rng.choice(nj, ni, replace=False))
@@ -541,6 +546,7 @@ def test_fused_small_blocks_equal_the_predictor_path(g, nj, eps2):
                            isel), f"fused nj={nj} ni={ni}")
     finally:
         g.option("fused_small", 1)
+        g.option("small_fp64", 1)
 
 
 def test_large_abi_block_equals_self_sweep(g):
