@@ -3,7 +3,11 @@
  * systems of uneven N integrated together on one MI355X.
  *
  * No counterpart in the reference's C ABI.  The integrator restates kira's
- * Hermite scheme for each system with one shared, power-of-two time step:
+ * Hermite scheme for each system:
+ *   block steps      every particle keeps its own power-of-two step; a block
+ *                    is every particle with the smallest t + dt
+ *                    (hdyn_schedule.C:641-745); all particles are predicted
+ *                    to the block time, only the block's are corrected
  *   predictor        include/_dyn_.h:290-295 (predict_loworder)
  *   force            hdyn_ev.C:2019-2036, in FP64 (default) or with libg6's
  *                    FP32 pair law (option "precision")
@@ -11,11 +15,12 @@
  *   step criterion   hdyn_ev.C:627-655 (Aarseth), :855-887 (halve at most
  *                    once, double only when commensurate)
  *   first step       hdyn_ev.C:377-410 (set_first_timestep, eta/16)
- * The system step is the minimum over its particles of kira's per-particle
- * step.  Each engine step advances every resident system by its own step;
- * systems finish independently at their t_end.  Results of a system are
- * bitwise identical whatever else is resident (its arithmetic depends only on
- * its own particles and N).
+ * With option "block_steps" = 0 a system instead takes one shared step, the
+ * minimum over its particles of kira's per-particle step, and every particle
+ * moves at every step.  Each engine step advances every resident system by
+ * one block (or one shared step); systems finish independently at their
+ * t_end.  Results of a system are bitwise identical whatever else is
+ * resident (its arithmetic depends only on its own particles and N).
  *
  * One engine per process, not re-entrant: call it from one thread at a time.
  *
@@ -55,7 +60,8 @@ int g6e_run(int steps);
 int g6e_poll(int max, long long *job_ids);
 
 /* Final state of a finished job: positions, velocities (double[n][3]), and
- * diagnostics: final time, steps taken, total energy at t = 0 and at the end
+ * diagnostics: final time, steps taken (blocks in block mode), total energy
+ * at t = 0 and at the end
  * (kinetic + potential with the job's softening).  Frees the result. */
 int g6e_result(long long job_id, int n, double *x, double *v, double *t_final,
                long long *steps, double *e0, double *e1);
@@ -71,6 +77,8 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * "precision" (pair arithmetic of jobs submitted afterwards: 64 = FP64 in the
  * CPU oracle's operation order, the default; 32 = FP32 pairs on
  * double-single positions with a noise-floored step criterion),
+ * "block_steps" (time steps of jobs submitted afterwards: 1 = kira's
+ * individual block steps, the default; 0 = one shared step per system),
  * "max_steps" (per system; beyond it the job fails), "dt_max_bits" (largest
  * block step 2^bits, default -3),
  * "light_max" (systems with n <= light_max run in the light lane, which
@@ -80,7 +88,8 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * step graphs there too; default 256), "wg_steps" (> 0: systems with
  * light_max < n <= wg_max_n run in a workgroup-fused kernel, one workgroup
  * per system (256 threads up to wg_small_max = 512, else 1024), up to this
- * many steps per launch, bitwise equal to the step graphs; default 0 = step
+ * many steps per launch, bitwise equal to the step graphs; shared-step
+ * systems only; default 0 = step
  * graphs, which measured faster on mixed cluster batches), "wg_small_max",
  * "wg_max_n" (<= 2048); "light_max", "fused_steps", "wg_steps", "max_split",
  * "wg_small_max" and "wg_max_n" are fixed while open (set them before

@@ -101,6 +101,12 @@ def _prototypes(L):
                                     C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.or_hermite_shared.restype = C.c_int
+    L.or_hermite_block.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_longlong,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_longlong), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    L.or_hermite_block.restype = C.c_int
     return L
 
 
@@ -198,6 +204,19 @@ def hermite_shared(m, x, v, t_end, eta, eps2, dt_max=0.125, max_steps=10**8, L=N
                                         max_steps, C.byref(t), C.byref(ns), C.byref(e0),
                                         C.byref(e1))
     return x, v, t.value, ns.value, e0.value, e1.value, rc
+
+
+def hermite_block(m, x, v, t_end, eta, eps2, dt_max=0.125, max_blocks=10**8, L=None):
+    """Integrate one system with kira's block-step Hermite rules; returns
+    (x, v, t_final, blocks, particle_steps, e0, e1, rc)."""
+    x = np.array(x, dtype=np.float64, order="C")
+    v = np.array(v, dtype=np.float64, order="C")
+    t = C.c_double(); nb = C.c_longlong(); ps = C.c_longlong()
+    e0 = C.c_double(); e1 = C.c_double()
+    rc = (L or lib()).or_hermite_block(len(m), _d(m), x, v, t_end, eta, eps2, dt_max,
+                                       max_blocks, C.byref(t), C.byref(nb), C.byref(ps),
+                                       C.byref(e0), C.byref(e1))
+    return x, v, t.value, nb.value, ps.value, e0.value, e1.value, rc
 
 
 def threads() -> int:

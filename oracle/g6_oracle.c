@@ -16,7 +16,13 @@
  * big-integer restatement of its two LCGs and shuffle, and Plummer
  * invariants.  The shared-step integrator (or_hermite_shared) is pinned in
  * tests/test_ens_cpu.py: a Kepler orbit closes after one period, the error
- * falls as eta^4, momentum is conserved.
+ * falls as eta^4, momentum is conserved.  The block-step integrator
+ * (or_hermite_block) is pinned there too: bit for bit against kira's
+ * block-step rules restated independently in vectorised numpy
+ * (starlab_amd/blockstep.py on an FP64 host backend), bit for bit against
+ * the shared-step integrator where every particle keeps the same step (two
+ * equal masses; a step cap below every natural step), a Kepler orbit closes,
+ * and the error falls as eta^4.
  */
 #include "g6_oracle.h"
 
@@ -574,5 +580,137 @@ int or_hermite_shared(int n, const double *m, double *x, double *v, double t_end
     *t_final = t;
     *steps = ns;
     free(a); free(jk); free(a1); free(j1); free(xp); free(vp); free(pot);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Block-step Hermite integrator (checker of the ensemble engine's block     */
+/* mode): kira's individual power-of-two steps.                              */
+/* ------------------------------------------------------------------------ */
+
+/* One system from t = 0 to t_end.  Each block is every particle with the
+ * smallest t + dt (fast_get_nodes_to_move, hdyn_schedule.C:641-745); all
+ * particles are predicted to the block time with predict_loworder
+ * (_dyn_.h:290-295), the block's forces are summed over every other particle
+ * in index order with the pair law of hs_force, and each block particle is
+ * corrected (hdyn_ev.C:1235-1250, :1717-1718) and given its own next step:
+ * the Aarseth criterion (:627-655), halved once if it falls short, doubled
+ * when the block time is commensurate with the doubled step (:866-887),
+ * never above min(dt_max, t_end).  First steps: set_first_timestep
+ * (:377-410) per particle.  A step that would pass t_end is cut to end
+ * there. */
+int or_hermite_block(int n, const double *m, double *x, double *v, double t_end,
+                     double eta, double eps2, double dt_max, long long max_blocks,
+                     double *t_final, long long *blocks, long long *psteps,
+                     double *e0, double *e1)
+{
+    const double ONE3 = 1.0 / 3.0, ONE12 = 1.0 / 12.0;
+    double *a = calloc(3 * (size_t)n, sizeof(double)), *jk = calloc(3 * (size_t)n, sizeof(double));
+    double *a1 = calloc(3 * (size_t)n, sizeof(double)), *j1 = calloc(3 * (size_t)n, sizeof(double));
+    double *xp = calloc(3 * (size_t)n, sizeof(double)), *vp = calloc(3 * (size_t)n, sizeof(double));
+    double *pot = calloc((size_t)n, sizeof(double)), *p1 = calloc((size_t)n, sizeof(double));
+    double *t = calloc((size_t)n, sizeof(double)), *dt = calloc((size_t)n, sizeof(double));
+    int *act = calloc((size_t)n, sizeof(int));
+    int rc = 0;
+    hs_force(n, m, x, v, eps2, a, jk, pot);
+    *e0 = hs_energy(n, m, v, pot);
+    const double dt_cap = fmin(dt_max, t_end);
+    for (int i = 0; i < n; i++) {
+        double eta_init = eta / 16;
+        double asq = a[3*i]*a[3*i] + a[3*i+1]*a[3*i+1] + a[3*i+2]*a[3*i+2];
+        double jsq = jk[3*i]*jk[3*i] + jk[3*i+1]*jk[3*i+1] + jk[3*i+2]*jk[3*i+2];
+        double dt_adot = 1e300, dt_ff = 1e300;
+        if (jsq > 0) dt_adot = asq / jsq;
+        if (pot[i] < 0 && asq > 0) dt_ff = -pot[i] / asq;
+        double c = eta_init * sqrt(fmin(dt_adot, dt_ff));
+        c = fmin(c, eta_init);
+        dt[i] = hs_pow2_below(fmin(c, dt_cap));
+        if (!(dt[i] > 1e-30)) rc = 1;
+    }
+    long long nb = 0, ps = 0;
+    double tlast = 0;
+    while (!rc) {
+        double tn = INFINITY;
+        for (int i = 0; i < n; i++)
+            if (t[i] < t_end && t[i] + dt[i] < tn) tn = t[i] + dt[i];
+        if (tn == INFINITY) break;                 /* every particle is at t_end */
+        int na = 0;
+        for (int i = 0; i < n; i++)
+            if (t[i] < t_end && t[i] + dt[i] == tn) act[na++] = i;
+        for (int i = 0; i < n; i++) {             /* predict_loworder to the block time */
+            double d = tn - t[i], d3 = d * ONE3, d2 = d * 0.5;
+            for (int k = 0; k < 3; k++) {
+                int q = 3 * i + k;
+                xp[q] = ((jk[q] * d3 + a[q]) * d2 + v[q]) * d + x[q];
+                vp[q] = (jk[q] * d2 + a[q]) * d + v[q];
+            }
+        }
+        for (int b = 0; b < na; b++) {            /* forces on the block */
+            int i = act[b];
+            double ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0, p = 0;
+            for (int k = 0; k < n; k++) {
+                if (k == i) continue;
+                double dx = xp[3*k] - xp[3*i], dy = xp[3*k+1] - xp[3*i+1], dz = xp[3*k+2] - xp[3*i+2];
+                double du = vp[3*k] - vp[3*i], dv = vp[3*k+1] - vp[3*i+1], dw = vp[3*k+2] - vp[3*i+2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                double r2inv = 1.0 / (d2 + eps2);
+                double a3 = -3.0 * (dx * du + dy * dv + dz * dw) * r2inv;
+                double mrinv = m[k] * sqrt(r2inv);
+                p -= mrinv;
+                double mr3inv = mrinv * r2inv;
+                ax += mr3inv * dx; ay += mr3inv * dy; az += mr3inv * dz;
+                jx += mr3inv * (du + a3 * dx);
+                jy += mr3inv * (dv + a3 * dy);
+                jz += mr3inv * (dw + a3 * dz);
+            }
+            a1[3*b] = ax; a1[3*b+1] = ay; a1[3*b+2] = az;
+            j1[3*b] = jx; j1[3*b+1] = jy; j1[3*b+2] = jz;
+            p1[b] = p;
+        }
+        for (int b = 0; b < na; b++) {            /* correct, next step */
+            int i = act[b];
+            double d = dt[i];
+            double bt2[3], at3[3];
+            for (int k = 0; k < 3; k++) {
+                int q = 3 * i + k, r = 3 * b + k;
+                bt2[k] = -3 * (a[q] - a1[r]) - d * (2 * jk[q] + j1[r]);
+                at3[k] = 2 * (a[q] - a1[r]) + d * (jk[q] + j1[r]);
+            }
+            for (int k = 0; k < 3; k++) {
+                int q = 3 * i + k, r = 3 * b + k;
+                x[q] = xp[q] + (0.05 * at3[k] + ONE12 * bt2[k]) * d * d;
+                v[q] = vp[q] + (0.25 * at3[k] + ONE3 * bt2[k]) * d;
+                a[q] = a1[r];
+                jk[q] = j1[r];
+            }
+            pot[i] = p1[b];
+            double d2 = d * d, d4 = d2 * d2, d6 = d4 * d2;
+            double l2 = 36 * (at3[0]*at3[0] + at3[1]*at3[1] + at3[2]*at3[2]) / d6;
+            double k2 = 4 * (bt2[0]*bt2[0] + bt2[1]*bt2[1] + bt2[2]*bt2[2]) / d4;
+            const double *jn = &j1[3*b], *an = &a1[3*b];
+            double j2 = jn[0]*jn[0] + jn[1]*jn[1] + jn[2]*jn[2];
+            double asq = an[0]*an[0] + an[1]*an[1] + an[2]*an[2];
+            double nat = eta * sqrt((sqrt(asq * k2) + j2) / (sqrt(j2 * l2) + k2));
+            if (!(nat > 0)) nat = INFINITY;
+            double T = d, nd = T;
+            if (nat < T) nd = 0.5 * T;
+            else if (nat >= 2 * T && fmod(tn, 2 * T) == 0.0 && 2 * T <= dt_cap) nd = 2 * T;
+            if (tn + nd > t_end) nd = t_end - tn;
+            t[i] = tn;
+            dt[i] = nd;
+            if (tn < t_end && !(nd > 1e-30)) rc = 1;
+        }
+        tlast = tn;
+        nb++;
+        ps += na;
+        if (nb > max_blocks) rc = 1;
+    }
+    hs_force(n, m, x, v, eps2, a1, j1, pot);
+    *e1 = hs_energy(n, m, v, pot);
+    *t_final = tlast;
+    *blocks = nb;
+    *psteps = ps;
+    free(a); free(jk); free(a1); free(j1); free(xp); free(vp); free(pot); free(p1);
+    free(t); free(dt); free(act);
     return rc;
 }

@@ -111,6 +111,20 @@ int or_hermite_shared(int n, const double *m, double *x, double *v, double t_end
                       double eta, double eps2, double dt_max, long long max_steps,
                       double *t_final, long long *steps, double *e0, double *e1);
 
+/* --- block-step Hermite integrator (ensemble engine checker, block mode) -- */
+/* The same rules with kira's individual power-of-two steps: each block is
+ * every particle with the smallest t + dt; all particles are predicted to
+ * the block time, the block's forces summed over every other particle in
+ * index order, and each block particle corrected and given its own next step
+ * (halved once when the criterion falls short, doubled when commensurate,
+ * at most min(dt_max, t_end); first steps per particle).  Returns 0, or 1 on
+ * step underflow or more than max_blocks blocks.  blocks = block times,
+ * psteps = particle steps, t_final = the last block time. */
+int or_hermite_block(int n, const double *m, double *x, double *v, double t_end,
+                     double eta, double eps2, double dt_max, long long max_blocks,
+                     double *t_final, long long *blocks, long long *psteps,
+                     double *e0, double *e1);
+
 #ifdef __cplusplus
 }
 #endif

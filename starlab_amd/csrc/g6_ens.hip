@@ -1,29 +1,39 @@
 // g6_ens.hip -- batched ensemble engine (include/g6ens.h): many independent
-// N-body systems integrated together, one shared power-of-two step per
-// system, kira's Hermite scheme.
+// N-body systems integrated together with kira's Hermite scheme, each system
+// either on kira's individual power-of-two block steps (the default) or on
+// one shared power-of-two step.
 //
 // Device state lives in one particle pool (capacity from the HBM budget);
 // each resident system owns a contiguous range [off, off + n) and a header
 // slot.  One engine step is four kernels, captured in a HIP graph:
-//   k_ens_predict   thread per particle: kira's predictor to t + dt
-//                   (include/_dyn_.h:290-295), double-single copy for pairs
+//   k_ens_predict   thread per particle: kira's predictor to the system's
+//                   next time (include/_dyn_.h:290-295), double-single copy
+//                   for pairs; in block mode it also lists the particles of
+//                   the block (t + dt = the block time, hdyn_schedule.C:641-745)
 //   k_ens_force     one wave per (system, 64-i chunk, j-split), against the
-//                   system's own j.  Precision per system:
+//                   system's own j; in block mode the i are the block's
+//                   particles.  Precision per system:
 //                   64 (default): FP64 pair law written in the CPU oracle's
 //                   exact operation order (hdyn_ev.C:2019-2036; IEEE division
 //                   and square root, no contraction), so with one j-split a
 //                   trajectory is bitwise the oracle's;
 //                   32: the libg6 pair law (FP32 pairs on double-single
 //                   positions, 256-j FP32 tiles folded into FP64)
-//   k_ens_correct   thread per particle: sum j-splits in order, Hermite
-//                   corrector (hdyn_ev.C:1235-1250, 1717-1718), Aarseth step
-//                   (hdyn_ev.C:627-655) -> atomic min into the system header
-//   k_ens_finalize  thread per slot: t += dt and kira's power-of-two step rule
-//                   (hdyn_ev.C:855-887); first step as set_first_timestep
-//                   (hdyn_ev.C:377-410)
+//   k_ens_correct   thread per particle (of the block): sum j-splits in order,
+//                   Hermite corrector (hdyn_ev.C:1235-1250, 1717-1718),
+//                   Aarseth step (hdyn_ev.C:627-655).  Shared mode: atomic
+//                   min of the step candidate into the system header.  Block
+//                   mode: the particle's own next step by kira's power-of-two
+//                   rule (hdyn_ev.C:866-887), atomic min of its next time.
+//   k_ens_finalize  thread per slot: shared mode t += dt and the power-of-two
+//                   rule on the system's step; block mode t = the block time,
+//                   next block time = the smallest t + dt.  First steps as
+//                   set_first_timestep (hdyn_ev.C:377-410).
 // Atomic min of a positive double through its bit pattern is order-free, and
 // every sum runs in a fixed order, so a system's trajectory does not depend
-// on what else is resident.
+// on what else is resident.  In block mode the block's particles are listed
+// in whatever order the waves append them; each particle's arithmetic does
+// not depend on its place in the list.
 //
 // FP32 pairs carry ~1e-7 relative noise per force evaluation.  The Aarseth
 // criterion differences successive accelerations (at3 = 2(a0 - a1) + ...,
@@ -55,10 +65,20 @@ enum Phase : int { FREE = 0, INIT = 1, STEP = 2, FINAL = 3, DONE = 4, FAILED = 5
 struct SysHdr {
     double t, dt, t_end, eta, eps2, dt_max;
     unsigned long long pmin;      // bits of the smallest candidate step (positive double)
-    long long steps;
+    long long steps;              // engine steps (shared mode) or blocks (block mode)
     int off, n, phase, nsplit, jchunk, prec;   // prec: 32 or 64
-    int lane, pad;                             // 0 heavy, 1 light
+    int lane;                                  // 0 heavy, 1 light
+    int mode;                                  // 0 shared step, 1 block steps
+    // block mode: the next block time, the bits of the smallest t + dt among
+    // the particles (atomic min, +inf: none left), the block's size, the
+    // particle steps so far, and a flag raised when a particle's step underflows
+    double tnext;
+    unsigned long long tmin;
+    long long psteps;
+    int nact, fail;
 };
+static_assert(sizeof(SysHdr) == 128, "SysHdr layout");
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;   // +inf as bits
 
 // FP64 predicted state for the precision-64 pair law (one scalar-cache line).
 struct JPred64 {
@@ -72,6 +92,8 @@ struct Pool {
     double *x = nullptr, *v = nullptr, *a = nullptr, *j = nullptr;   // [3][cap]
     double *xp = nullptr, *vp = nullptr;                             // [3][cap]
     double *m = nullptr, *pot = nullptr, *pot0 = nullptr;            // [cap]
+    double *pt = nullptr, *pdt = nullptr;                            // [cap] block mode: t, dt
+    int *act = nullptr;                                              // [cap] block mode: the block
     JPred *jp = nullptr;                                             // [cap]
     JPred64 *jp64 = nullptr;                                         // [cap]
     double *part = nullptr;                                          // [max_split][7][cap]
@@ -124,7 +146,23 @@ __device__ __forceinline__ JPred64 make_jpred64(const double xp[3], const double
     return d;
 }
 
-__global__ __launch_bounds__(64) void k_ens_predict(Pool P, const SysHdr *__restrict__ H,
+// Wave-wide minimum of a non-negative double (+inf where none), then one
+// atomic min of its bit pattern into *dst (order-free for non-negative values).
+__device__ __forceinline__ void wave_atomic_min(unsigned long long *dst, double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0 && v < __builtin_inf())
+        atomicMin(dst, (unsigned long long)__double_as_longlong(v));
+}
+
+// Block mode, STEP: is particle p in the block at the system's next time?
+__device__ __forceinline__ bool in_block(const Pool &P, const SysHdr &h, size_t p)
+{
+    const double ti = P.pt[p];
+    return ti < h.t_end && ti + P.pdt[p] == h.tnext;
+}
+
+__global__ __launch_bounds__(64) void k_ens_predict(Pool P, SysHdr *H,
                                                     const Chunk *__restrict__ chunks,
                                                     const int *__restrict__ nchunks)
 {
@@ -132,13 +170,29 @@ __global__ __launch_bounds__(64) void k_ens_predict(Pool P, const SysHdr *__rest
     const int c = blockIdx.x;
     if (c >= *nchunks) return;
     const Chunk ch = chunks[c];
-    const SysHdr &h = H[ch.slot];
-    if (h.phase < INIT || h.phase > FINAL) return;
+    SysHdr &h = H[ch.slot];
+    if (h.phase < INIT || h.phase > FINAL) return;   // uniform in the wave
     const int i = ch.i0 + threadIdx.x;
-    if (i >= h.n) return;
-    const size_t p = (size_t)h.off + i;
+    const bool valid = i < h.n;
+    const size_t p = (size_t)h.off + (valid ? i : 0);
     const size_t cap = P.cap;
-    const double dt = h.phase == STEP ? h.dt : 0.0;
+    double dt = h.phase == STEP ? h.dt : 0.0;
+    if (h.mode == 1 && h.phase == STEP) {
+        // every particle is predicted to the block time; the block's
+        // particles are appended to the block list, the others' next times
+        // go into the system's minimum
+        const bool blk = valid && in_block(P, h, p);
+        const double ti = P.pt[p];
+        dt = h.tnext - ti;
+        const unsigned long long bal = __ballot(blk);
+        int base = 0;
+        if (threadIdx.x == 0 && bal) base = atomicAdd(&h.nact, __popcll(bal));
+        base = __shfl(base, 0);
+        if (blk) P.act[(size_t)h.off + base + __popcll(bal & ((1ull << threadIdx.x) - 1))] = i;
+        const double nxt = (valid && !blk && ti < h.t_end) ? ti + P.pdt[p] : __builtin_inf();
+        wave_atomic_min(&h.tmin, nxt);
+    }
+    if (!valid) return;
     double x[3], v[3], a[3], jk[3], xp[3], vp[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -238,14 +292,34 @@ __device__ __forceinline__ void ens_pair64(const JPred64 &J, int jj, int self, c
     a.jz += mr3inv * (dw + a3 * dz);
 }
 
-__device__ void ens_force64(const Pool &P, const SysHdr &h, const FItem &it)
+// The i of lane `lane` of work item chunk i0: position pos = i0 + lane of the
+// system's i-list (its particles in order, or in block mode the block list).
+// Lanes past the list's end get i = -1 and read particle 0.
+struct ISel {
+    int pos, i, ic;
+};
+
+__device__ __forceinline__ ISel select_i(const Pool &P, const SysHdr &h, bool blk, int cnt,
+                                         int i0, int lane)
+{
+    ISel s;
+    s.pos = i0 + lane;
+    s.i = s.pos < cnt ? (blk ? P.act[(size_t)h.off + s.pos] : s.pos) : -1;
+    s.ic = s.i >= 0 ? s.i : 0;
+    return s;
+}
+
+// block mode, STEP: the i-list is the block; otherwise every particle
+__device__ __forceinline__ bool block_step(const SysHdr &h) { return h.mode == 1 && h.phase == STEP; }
+
+__device__ void ens_force64(const Pool &P, const SysHdr &h, const FItem &it, const ISel &sel)
 {
     const int n = h.n;
-    const int i = it.i0 + threadIdx.x;
-    const int ic = i < n ? i : n - 1;
+    const int i = sel.i;
+    const int ic = sel.ic;
     const JPred64 *__restrict__ base = P.jp64 + h.off;
     const JPred64 I = base[ic];
-    const int self = i < n ? i : -1;
+    const int self = i;
     const int jbeg = it.split * h.jchunk;
     const int jend = min(jbeg + h.jchunk, n);
     const double eps2 = h.eps2;
@@ -260,9 +334,9 @@ __device__ void ens_force64(const Pool &P, const SysHdr &h, const FItem &it)
             for (int jj = 0; jj < tn; ++jj) ens_pair64<true>(base[t + jj], t + jj, self, I, eps2, a);
         }
     }
-    if (i < n) {
+    if (i >= 0) {
         const size_t cap = P.cap;
-        const size_t p = (size_t)h.off + i;
+        const size_t p = (size_t)h.off + sel.pos;   // partials by list position
         double *out = P.part + (size_t)it.split * 7 * cap;
         out[0 * cap + p] = a.ax;
         out[1 * cap + p] = a.ay;
@@ -283,17 +357,21 @@ __global__ __launch_bounds__(64) void k_ens_force(Pool P, const SysHdr *__restri
     const FItem it = items[w];
     const SysHdr &h = H[it.slot];
     if (h.phase < INIT || h.phase > FINAL) return;
+    const bool blk = block_step(h);
+    const int cnt = blk ? h.nact : h.n;
+    if (it.i0 >= cnt) return;                      // uniform in the wave
+    const int lane = threadIdx.x;
+    const ISel sel = select_i(P, h, blk, cnt, it.i0, lane);
     if (h.prec == 64) {
-        ens_force64(P, h, it);
+        ens_force64(P, h, it, sel);
         return;
     }
     const int n = h.n;
-    const int lane = threadIdx.x;
-    const int i = it.i0 + lane;
-    const int ic = i < n ? i : n - 1;
+    const int i = sel.i;
+    const int ic = sel.ic;
     const JPred *__restrict__ base = P.jp + h.off;
     const JPred I = base[ic];
-    const int self = i < n ? i : -1;
+    const int self = i;
     const float eps2 = (float)h.eps2;
     const int jbeg = it.split * h.jchunk;
     const int jend = min(jbeg + h.jchunk, n);
@@ -315,9 +393,9 @@ __global__ __launch_bounds__(64) void k_ens_force(Pool P, const SysHdr *__restri
         A[5] += (double)a.jz - 3.0 * (double)a.kz;
         A[6] += (double)a.p;
     }
-    if (i < n) {
+    if (i >= 0) {
         const size_t cap = P.cap;
-        const size_t p = (size_t)h.off + i;
+        const size_t p = (size_t)h.off + sel.pos;   // partials by list position
         double *out = P.part + (size_t)it.split * 7 * cap;
 #pragma unroll
         for (int c = 0; c < 7; ++c) out[c * cap + p] = A[c];
@@ -395,9 +473,53 @@ __device__ __forceinline__ double ens_correct_one(PState &st, const double xp[3]
     return cand;
 }
 
+__device__ __forceinline__ double pow2_below(double x)
+{
+    int e;
+    const double f = frexp(x, &e);   // x = f * 2^e, f in [0.5, 1)
+    (void)f;
+    return ldexp(1.0, e - 1);
+}
+
+// Block mode: the particle's next step after a corrected step of length T
+// ending at block time tb, given the criterion's value nat: halved once if
+// nat falls short, doubled when tb is commensurate with 2T and 2T stays within
+// dt_max (hdyn_ev.C:866-887), cut to end at t_end.
+__device__ __forceinline__ double block_next_step(double nat, double T, double tb, double dt_max,
+                                                  double t_end)
+{
+#pragma clang fp contract(off)
+    double nd = T;
+    if (nat < T) nd = 0.5 * T;
+    else if (nat >= 2 * T && fmod(tb, 2 * T) == 0.0 && 2 * T <= dt_max) nd = 2 * T;
+    if (tb + nd > t_end) nd = t_end - tb;
+    return nd;
+}
+
+// Block mode: a particle's time and step after its correction (INIT: the
+// first step from the set_first_timestep candidate; STEP: the next step);
+// returns its next time, +inf once it has reached t_end.  *fail is set when
+// a step underflows dt_min.
+__device__ __forceinline__ double block_advance(double &ti, double &di, double cand, int phase,
+                                                const SysHdr &h, double dt_min, int *fail)
+{
+    if (phase == INIT) {
+        ti = 0.0;
+        di = pow2_below(fmin(cand, h.dt_max));
+        if (!(di > dt_min)) *fail = 1;
+    } else if (phase == STEP) {
+        const double tb = h.tnext;
+        di = block_next_step(cand, di, tb, h.dt_max, h.t_end);
+        ti = tb;
+        if (tb < h.t_end && !(di > dt_min)) *fail = 1;
+    }
+    return ti < h.t_end ? ti + di : __builtin_inf();
+}
+
 __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__ H,
                                                     const Chunk *__restrict__ chunks,
-                                                    const int *__restrict__ nchunks)
+                                                    const int *__restrict__ nchunks,
+                                                    double dt_min)
 {
 #pragma clang fp contract(off)
     const int c = blockIdx.x;
@@ -406,24 +528,30 @@ __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__
     SysHdr &h = H[ch.slot];
     const int phase = h.phase;
     if (phase < INIT || phase > FINAL) return;
-    const int i = ch.i0 + threadIdx.x;
-    double cand = __builtin_inf();
-    if (i < h.n) {
+    const bool blk = block_step(h);
+    const int cnt = blk ? h.nact : h.n;
+    if (ch.i0 >= cnt) return;                      // uniform in the wave
+    const ISel sel = select_i(P, h, blk, cnt, ch.i0, threadIdx.x);
+    const int i = sel.i;
+    double cand = __builtin_inf(), next = __builtin_inf();
+    int fail = 0;
+    if (i >= 0) {
         const size_t cap = P.cap;
         const size_t p = (size_t)h.off + i;
+        const size_t q = (size_t)h.off + sel.pos;   // partials by list position
         // split 0 is taken as is (no "0 +" rounding), later splits added in order
         double a1[3], j1[3], pot;
         {
             const double *in = P.part;
-            a1[0] = in[0 * cap + p]; a1[1] = in[1 * cap + p]; a1[2] = in[2 * cap + p];
-            j1[0] = in[3 * cap + p]; j1[1] = in[4 * cap + p]; j1[2] = in[5 * cap + p];
-            pot = in[6 * cap + p];
+            a1[0] = in[0 * cap + q]; a1[1] = in[1 * cap + q]; a1[2] = in[2 * cap + q];
+            j1[0] = in[3 * cap + q]; j1[1] = in[4 * cap + q]; j1[2] = in[5 * cap + q];
+            pot = in[6 * cap + q];
         }
         for (int s = 1; s < h.nsplit; ++s) {
             const double *in = P.part + (size_t)s * 7 * cap;
-            a1[0] += in[0 * cap + p]; a1[1] += in[1 * cap + p]; a1[2] += in[2 * cap + p];
-            j1[0] += in[3 * cap + p]; j1[1] += in[4 * cap + p]; j1[2] += in[5 * cap + p];
-            pot += in[6 * cap + p];
+            a1[0] += in[0 * cap + q]; a1[1] += in[1 * cap + q]; a1[2] += in[2 * cap + q];
+            j1[0] += in[3 * cap + q]; j1[1] += in[4 * cap + q]; j1[2] += in[5 * cap + q];
+            pot += in[6 * cap + q];
         }
         PState st;
         double xp[3], vp[3];
@@ -433,7 +561,15 @@ __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__
             xp[k] = P.xp[k * cap + p]; vp[k] = P.vp[k * cap + p];
         }
         st.pot = 0; st.pot0 = 0;
-        cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, h.dt, h.eta, h.prec);
+        double ti = 0, di = 0;
+        if (h.mode == 1) { ti = P.pt[p]; di = P.pdt[p]; }
+        cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, h.mode == 1 ? di : h.dt, h.eta,
+                               h.prec);
+        if (h.mode == 1 && phase != FINAL) {
+            next = block_advance(ti, di, cand, phase, h, dt_min, &fail);
+            P.pt[p] = ti;
+            P.pdt[p] = di;
+        }
         if (phase == INIT) {
             for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = st.a[k]; P.j[k * cap + p] = st.j[k]; }
             P.pot[p] = st.pot;
@@ -450,18 +586,15 @@ __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__
             P.pot[p] = st.pot;
         }
     }
+    if (h.mode == 1) {
+        // the block's next times join the system's minimum; a failed step
+        // marks the system
+        wave_atomic_min(&h.tmin, next);
+        if (__ballot(fail) && threadIdx.x == 0) atomicOr(&h.fail, 1);
+        return;
+    }
     // wave minimum, then one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) cand = fmin(cand, __shfl_xor(cand, off));
-    if (threadIdx.x == 0 && phase != FINAL && cand < __builtin_inf())
-        atomicMin(&h.pmin, (unsigned long long)__double_as_longlong(cand));
-}
-
-__device__ __forceinline__ double pow2_below(double x)
-{
-    int e;
-    const double f = frexp(x, &e);   // x = f * 2^e, f in [0.5, 1)
-    (void)f;
-    return ldexp(1.0, e - 1);
+    if (phase != FINAL) wave_atomic_min(&h.pmin, cand);
 }
 
 // t += dt and kira's step rules for one system, given the smallest step
@@ -497,6 +630,35 @@ __device__ __forceinline__ void ens_finalize_hdr(SysHdr &h, double nat, long lon
     }
 }
 
+// Block mode: the system moves to the block just done (STEP) and on to the
+// next block time tmin, the smallest t + dt of its particles (+inf once every
+// particle is at t_end: then the final potential pass, FINAL, and DONE).
+// nact = the block's size, fail = a particle's step underflowed.
+__device__ __forceinline__ void ens_finalize_block(SysHdr &h, double tmin, int nact, int fail,
+                                                   long long max_steps)
+{
+    if (h.phase == INIT) {
+        h.t = 0.0;
+        h.tnext = tmin;
+        h.dt = tmin;
+        h.phase = (fail || !(tmin < __builtin_inf())) ? FAILED : STEP;
+    } else if (h.phase == STEP) {
+        h.dt = h.tnext - h.t;
+        h.t = h.tnext;
+        h.steps += 1;
+        h.psteps += nact;
+        if (fail || h.steps > max_steps) {
+            h.phase = FAILED;
+        } else if (!(tmin < __builtin_inf())) {
+            h.phase = FINAL;
+        } else {
+            h.tnext = tmin;
+        }
+    } else if (h.phase == FINAL) {
+        h.phase = DONE;
+    }
+}
+
 __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restrict__ phase_out,
                                long long max_steps, double dt_min, int lane)
 {
@@ -504,9 +666,15 @@ __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restri
     if (s >= nslots) return;
     SysHdr &h = H[s];
     if (h.lane != lane || h.phase < INIT || h.phase > FINAL) return;
-    const double nat = __longlong_as_double((long long)h.pmin);
-    ens_finalize_hdr(h, nat, max_steps, dt_min);
-    h.pmin = 0x7FF0000000000000ull;   // +inf
+    if (h.mode == 1) {
+        ens_finalize_block(h, __longlong_as_double((long long)h.tmin), h.nact, h.fail, max_steps);
+        h.tmin = kInfBits;
+        h.nact = 0;
+    } else {
+        const double nat = __longlong_as_double((long long)h.pmin);
+        ens_finalize_hdr(h, nat, max_steps, dt_min);
+    }
+    h.pmin = kInfBits;
     phase_out[s] = h.phase;
 }
 
@@ -517,7 +685,9 @@ __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restri
 // system is done.  Predicted j live in LDS; state stays in registers and is
 // written back at the end.  The arithmetic is the step kernels' (shared
 // helpers above, one j-split, j ascending), so a system's trajectory is the
-// same bit for bit on either path.
+// same bit for bit on either path.  In block mode every lane predicts its
+// particle to the block time and sums its force; only the block's lanes
+// correct and take new steps.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_ens_fused(Pool P, SysHdr *__restrict__ H,
                                                   const Chunk *__restrict__ list,
@@ -549,10 +719,19 @@ __global__ __launch_bounds__(64) void k_ens_fused(Pool P, SysHdr *__restrict__ H
     st.pot = P.pot[p];
     st.pot0 = P.pot0[p];
     const int self = act ? i : -1;
+    const bool bm = h.mode == 1;
+    double ti = 0, di = 0;   // block mode: the particle's time and step
+    if (bm) { ti = P.pt[p]; di = P.pdt[p]; }
     double xp[3], vp[3];
     for (int it = 0; it < max_iters; ++it) {
         const int phase = h.phase;
-        ens_predict_one(st.x, st.v, st.a, st.j, phase == STEP ? h.dt : 0.0, xp, vp);
+        double dpred = phase == STEP ? h.dt : 0.0;
+        bool inb = act;
+        if (bm && phase == STEP) {
+            inb = act && ti < h.t_end && ti + di == h.tnext;
+            dpred = h.tnext - ti;
+        }
+        ens_predict_one(st.x, st.v, st.a, st.j, dpred, xp, vp);
         if (act) {
             s64[i] = make_jpred64(xp, vp, m);
             s32[i] = make_jpred(xp, vp, m);
@@ -585,6 +764,22 @@ __global__ __launch_bounds__(64) void k_ens_fused(Pool P, SysHdr *__restrict__ H
             pot = A[6];
         }
         __syncthreads();   // LDS is rewritten by the next step's prediction
+        if (bm) {
+            double next = __builtin_inf();
+            int fail = 0;
+            if (inb) {
+                const double c = ens_correct_one(st, xp, vp, a1, j1, pot, phase, di, h.eta, h.prec);
+                if (phase != FINAL) next = block_advance(ti, di, c, phase, h, dt_min, &fail);
+            } else if (act && ti < h.t_end) {
+                next = ti + di;
+            }
+            for (int off = 32; off > 0; off >>= 1) next = fmin(next, __shfl_xor(next, off));
+            const int nact = phase == STEP ? __popcll(__ballot(inb)) : 0;
+            const int anyfail = __ballot(fail) != 0;
+            ens_finalize_block(h, next, nact, anyfail, max_steps);   // every lane, same values
+            if (h.phase == DONE || h.phase == FAILED) break;
+            continue;
+        }
         double cand = __builtin_inf();
         if (act) cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, h.dt, h.eta, h.prec);
         for (int off = 32; off > 0; off >>= 1) cand = fmin(cand, __shfl_xor(cand, off));
@@ -599,9 +794,12 @@ __global__ __launch_bounds__(64) void k_ens_fused(Pool P, SysHdr *__restrict__ H
         }
         P.pot[p] = st.pot;
         P.pot0[p] = st.pot0;
+        if (bm) { P.pt[p] = ti; P.pdt[p] = di; }
     }
     if (i == 0) {
-        h.pmin = 0x7FF0000000000000ull;
+        h.pmin = kInfBits;
+        h.tmin = kInfBits;
+        h.nact = 0;
         H[slot] = h;
         phase_out[slot] = h.phase;
     }
@@ -779,7 +977,7 @@ __global__ __launch_bounds__(NT) void k_ens_wg(Pool P, SysHdr *__restrict__ H,
         if (sh.phase == DONE || sh.phase == FAILED) break;
     }
     if (tid == 0) {
-        sh.pmin = 0x7FF0000000000000ull;
+        sh.pmin = kInfBits;
         H[slot] = sh;
         phase_out[slot] = sh.phase;
     }
@@ -804,6 +1002,8 @@ __global__ void k_ens_unpack(Pool P, const double *__restrict__ rec, int n)
     }
     P.pot[p] = 0.0;
     P.pot0[p] = 0.0;
+    P.pt[p] = 0.0;
+    P.pdt[p] = 0.0;
 }
 
 __global__ void k_ens_set_headers(SysHdr *__restrict__ H, const SysHdr *__restrict__ staged,
@@ -849,7 +1049,7 @@ struct Job {
     int n;
     std::vector<double> m, x, v;
     double t_end, eta, eps2;
-    int prec;
+    int prec, mode;
 };
 
 struct SlotInfo {
@@ -858,7 +1058,7 @@ struct SlotInfo {
     int off = 0, n = 0, nsplit = 1, lane = 0;
     std::vector<double> m, x0, v0;
     double eps2 = 0, t_end = 0;
-    int prec = 64;
+    int prec = 64, mode = 1;
 };
 
 struct Result {
@@ -930,6 +1130,7 @@ struct Engine {
     int max_split = 16;
     int lookahead = 64;
     int precision = 64;
+    int block_steps = 1;              // 1: kira's individual block steps, 0: one shared step
     int light_max = 64;
     int fused_steps = 256;            // light lane: steps per fused launch (0: graphs)
     int wg_steps = 0;                 // medium lanes: steps per workgroup-fused launch (0: graphs;
@@ -972,15 +1173,16 @@ template <class T> void hfree(T *&p) { if (p) { (void)hipHostFree(p); p = nullpt
 
 size_t bytes_per_particle(int max_split)
 {
-    return sizeof(double) * (3 * 6 + 3) + sizeof(JPred) + sizeof(JPred64) +
+    return sizeof(double) * (3 * 6 + 3 + 2) + sizeof(int) + sizeof(JPred) + sizeof(JPred64) +
            sizeof(double) * 7 * (size_t)max_split;
 }
 
-int lane_for(int n)
+// The workgroup-fused lanes step shared-step systems only.
+int lane_for(int n, int mode)
 {
     if (n <= X.light_max) return 1;
-    if (X.wg_steps > 0 && n <= X.wg_small_max) return 2;
-    if (X.wg_steps > 0 && n <= X.wg_max_n) return 3;
+    if (mode == 0 && X.wg_steps > 0 && n <= X.wg_small_max) return 2;
+    if (mode == 0 && X.wg_steps > 0 && n <= X.wg_max_n) return 3;
     return 0;
 }
 
@@ -1136,7 +1338,10 @@ int retire()
             r.e1 = 0.5 * kin + 0.5 * pe;
             r.e0 = 0.5 * kin0 + 0.5 * pe0;
         }
-        X.pairs += (double)(r.steps + 2) * si.n * (double)(si.n - 1);
+        // pair interactions: every particle at the start and at the end, plus
+        // each step's particles (all of them in shared mode, the block's in block mode)
+        const double psteps = si.mode == 1 ? (double)hdr[q].psteps : (double)r.steps * si.n;
+        X.pairs += (psteps + 2.0 * si.n) * (double)(si.n - 1);
         X.results[si.job] = std::move(r);
         X.finished.push_back(si.job);
         X.n_finished++;
@@ -1146,7 +1351,7 @@ int retire()
     }
     // mark retired slots FREE on the device
     std::vector<SysHdr> zero(done.size());
-    for (auto &z : zero) { memset(&z, 0, sizeof z); z.phase = FREE; z.pmin = 0x7FF0000000000000ull; }
+    for (auto &z : zero) { memset(&z, 0, sizeof z); z.phase = FREE; z.pmin = z.tmin = kInfBits; }
     for (size_t q = 0; q < done.size(); ++q) X.h_hslots[q] = done[q];
     memcpy(X.h_hstage, zero.data(), sizeof(SysHdr) * done.size());
     ECHK(hipMemcpyAsync(X.d_hstage, X.h_hstage, sizeof(SysHdr) * done.size(), hipMemcpyHostToDevice,
@@ -1184,8 +1389,9 @@ int admit()
         si.off = off;
         si.n = it->n;
         si.nsplit = nsplit_for(it->n);
-        si.lane = lane_for(it->n);
+        si.lane = lane_for(it->n, it->mode);
         si.prec = it->prec;
+        si.mode = it->mode;
         si.m = it->m;
         si.x0 = it->x;
         si.v0 = it->v;
@@ -1195,10 +1401,12 @@ int admit()
         memset(&h, 0, sizeof h);
         h.t = 0; h.dt = 0; h.t_end = it->t_end; h.eta = it->eta; h.eps2 = it->eps2;
         h.dt_max = std::min(X.dt_max, it->t_end);
-        h.pmin = 0x7FF0000000000000ull;
+        h.pmin = kInfBits;
+        h.tmin = kInfBits;
         h.steps = 0;
         h.off = off; h.n = it->n; h.phase = INIT; h.nsplit = si.nsplit; h.prec = it->prec;
         h.lane = si.lane;
+        h.mode = it->mode;
         h.jchunk = ((it->n + si.nsplit - 1) / si.nsplit + kTile - 1) / kTile * kTile;
         if (si.nsplit > 1) {
             // split boundaries on whole tiles; recompute the split count
@@ -1328,7 +1536,7 @@ void enqueue_step(int ln, int cgrid, int igrid)
     hipLaunchKernelGGL(k_ens_force, dim3(igrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
                        L.d_items, L.d_counts + 1);
     hipLaunchKernelGGL(k_ens_correct, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
-                       L.d_chunks, L.d_counts);
+                       L.d_chunks, L.d_counts, 1e-30);
     hipLaunchKernelGGL(k_ens_finalize, dim3((X.max_slots + 255) / 256), dim3(256), 0, L.stream,
                        X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30, ln);
 }
@@ -1391,6 +1599,7 @@ void release_all()
     }
     dfree(X.P.x); dfree(X.P.v); dfree(X.P.a); dfree(X.P.j); dfree(X.P.xp); dfree(X.P.vp);
     dfree(X.P.m); dfree(X.P.pot); dfree(X.P.pot0); dfree(X.P.jp); dfree(X.P.jp64); dfree(X.P.part);
+    dfree(X.P.pt); dfree(X.P.pdt); dfree(X.P.act);
     dfree(X.d_hdr); dfree(X.d_phase); hfree(X.h_phase);
     hfree(X.h_hstage); dfree(X.d_hstage); hfree(X.h_hslots); dfree(X.d_hslots);
     hfree(X.h_stage); dfree(X.d_stage); X.stage_cap = 0;
@@ -1445,6 +1654,9 @@ int open_engine(long long hbm_budget_bytes, int max_systems)
     ECHK(hipMalloc((void **)&X.P.m, sizeof(double) * c));
     ECHK(hipMalloc((void **)&X.P.pot, sizeof(double) * c));
     ECHK(hipMalloc((void **)&X.P.pot0, sizeof(double) * c));
+    ECHK(hipMalloc((void **)&X.P.pt, sizeof(double) * c));
+    ECHK(hipMalloc((void **)&X.P.pdt, sizeof(double) * c));
+    ECHK(hipMalloc((void **)&X.P.act, sizeof(int) * c));
     ECHK(hipMalloc((void **)&X.P.jp, sizeof(JPred) * c));
     ECHK(hipMalloc((void **)&X.P.jp64, sizeof(JPred64) * c));
     ECHK(hipMalloc((void **)&X.P.part, sizeof(double) * 7 * c * (size_t)X.max_split));
@@ -1460,7 +1672,7 @@ int open_engine(long long hbm_budget_bytes, int max_systems)
     for (int s = 0; s < max_systems; ++s) {
         memset(&X.h_hstage[s], 0, sizeof(SysHdr));
         X.h_hstage[s].phase = FREE;
-        X.h_hstage[s].pmin = 0x7FF0000000000000ull;
+        X.h_hstage[s].pmin = X.h_hstage[s].tmin = kInfBits;
         X.h_hslots[s] = s;
         X.h_phase[s] = FREE;
     }
@@ -1525,6 +1737,7 @@ long long g6e_submit(int n, const double *m, const double *x, const double *v, d
     j.eta = eta;
     j.eps2 = eps2;
     j.prec = X.precision;
+    j.mode = X.block_steps;
     X.queue.push_back(std::move(j));
     return X.next_id - 1;
 }
@@ -1721,6 +1934,10 @@ int g6e_set_option(const char *key, long long value)
     else if (k == "precision") {
         if (value != 32 && value != 64) return efail(-1, "precision must be 32 or 64");
         X.precision = (int)value;
+    } else if (k == "block_steps") {
+        // taken by each job at submission, like the precision
+        if (value != 0 && value != 1) return efail(-1, "block_steps must be 0 or 1");
+        X.block_steps = (int)value;
     } else return -1;
     return 0;
 }
@@ -1735,6 +1952,7 @@ long long g6e_get_option(const char *key)
     if (k == "dt_max_bits") return std::ilogb(X.dt_max);
     if (k == "max_steps") return X.max_steps;
     if (k == "precision") return X.precision;
+    if (k == "block_steps") return X.block_steps;
     if (k == "light_max") return X.light_max;
     if (k == "fused_steps") return X.fused_steps;
     if (k == "wg_steps") return X.wg_steps;

@@ -1,4 +1,5 @@
-"""GPU: the batched ensemble engine (include/g6ens.h).
+"""GPU: the batched ensemble engine (include/g6ens.h) in shared-step mode
+(option block_steps = 0; block mode is tested in test_gpu_ens_block.py).
 
 * precision 64 with one j-split: every system's trajectory, step count and
   energies are bitwise those of the FP64 shared-step oracle
@@ -38,11 +39,12 @@ def _jobs(seed, sizes, t_end, eta=0.05):
 
 
 DEFAULTS = dict(jsplit_target=512, precision=64, max_steps=50_000_000, steps_per_graph=8,
-                fused_steps=256, light_max=64, wg_steps=0)
+                fused_steps=256, light_max=64, wg_steps=0, block_steps=1)
 PATHS = pytest.mark.parametrize("fused", [256, 0], ids=["fused", "graphs"])
 
 
 def _run(jobs, budget=2 * GB, max_systems=4096, **opt):
+    opt.setdefault("block_steps", 0)
     e = Ensemble(budget, max_systems, **opt)
     try:
         ids = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in jobs]
@@ -109,7 +111,7 @@ def test_alone_equals_mixed_bitwise(prec, fused):
     target3 = _jobs(4, [3], t_end=2.0)[0]
     alone, _ = _run([target, target3], precision=prec, fused_steps=fused)
     mix = _jobs(5, [3, 64, 2048, 3, 700, 3, 128, 1500, 3, 3, 40], t_end=1.0 / 8)
-    e = Ensemble(2 * GB, 64, precision=prec, fused_steps=fused)
+    e = Ensemble(2 * GB, 64, precision=prec, fused_steps=fused, block_steps=0)
     try:
         others = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in mix[:6]]
         e.run(16)                                   # the mix is already running
@@ -282,10 +284,10 @@ def test_config4_sample_fp32_to_t1_against_fp64():
     batch = serve.cluster_batch(4096, 32, 2048, 4, 1.0, prec=32)
     order = np.argsort([j["n"] for j in batch], kind="stable")
     sample = [batch[k] for k in order[::16]]
-    r32 = serve.run_batch(sample, keep=True)["results"]
+    r32 = serve.run_batch(sample, keep=True, block_steps=0)["results"]
     for j in sample:
         j["prec"] = 64
-    r64 = serve.run_batch(sample, keep=True)["results"]
+    r64 = serve.run_batch(sample, keep=True, block_steps=0)["results"]
     de32 = np.array([abs(r["e1"] / r["e0"] - 1) for r in r32])
     de64 = np.array([abs(r["e1"] / r["e0"] - 1) for r in r64])
     ratio = np.array([a["steps"] / b["steps"] for a, b in zip(r32, r64)])
@@ -314,7 +316,7 @@ def test_alone_equals_mixed_at_the_largest_serving_cluster(prec):
     target = _jobs(13, [4096], t_end=1.0 / 64)[0]
     alone, _ = _run([target], precision=prec)
     mix = _jobs(14, [3, 3, 300, 3, 2048, 3, 3, 1000, 3], t_end=1.0 / 64)
-    e = Ensemble(2 * GB, 64, precision=prec)
+    e = Ensemble(2 * GB, 64, precision=prec, block_steps=0)
     try:
         first = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in mix[:5]]
         e.run(8)
