@@ -660,9 +660,11 @@ __device__ __forceinline__ void ens_finalize_block(SysHdr &h, double tmin, int n
 }
 
 __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restrict__ phase_out,
-                               long long max_steps, double dt_min, int lane)
+                               long long max_steps, double dt_min, int lane,
+                               int *__restrict__ bcount)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) bcount[0] = bcount[1] = 0;   // the lane's block lists, for the next step
     if (s >= nslots) return;
     SysHdr &h = H[s];
     if (h.lane != lane || h.phase < INIT || h.phase > FINAL) return;
@@ -676,6 +678,250 @@ __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restri
     }
     h.pmin = kInfBits;
     phase_out[s] = h.phase;
+}
+
+// ---------------------------------------------------------------------------
+// Block-step path with prediction inside the force kernel (option
+// block_fusion = 1, the default).  Per engine step:
+//   k_ens_bselect  thread per particle: reads only t and dt, lists the block
+//                  (t + dt = the block time) and takes the others' next times
+//                  into the system's minimum;
+//   k_ens_bitems   thread per block-mode system: appends the system's work to
+//                  the lane's device lists, (system, 64-i chunk) for the
+//                  corrector and (system, chunk, j-split) for the force;
+//   k_ens_bforce   grid-stride over the force items: each wave predicts its
+//                  j-split 64 j at a time (one per lane) into LDS and sums the
+//                  pairs from there, in the same order as k_ens_force on
+//                  predicted records, so results are bitwise equal;
+//   k_ens_bcorrect grid-stride over the chunks: sums the splits, recomputes
+//                  the particle's own prediction, corrects it and takes its
+//                  next step;
+//   k_ens_finalize as above; it also clears the lane's list counters.
+// No prediction is written to HBM, and waves never start on empty work.
+// ---------------------------------------------------------------------------
+struct BLists {
+    Chunk *chunks;
+    FItem *items;
+    int *count;        // [0] chunks, [1] items of this step
+};
+
+__global__ __launch_bounds__(64) void k_ens_bselect(Pool P, SysHdr *H,
+                                                    const Chunk *__restrict__ chunks,
+                                                    const int *__restrict__ nchunks)
+{
+    const int c = blockIdx.x;
+    if (c >= *nchunks) return;
+    const Chunk ch = chunks[c];
+    SysHdr &h = H[ch.slot];
+    if (h.mode != 1 || h.phase != STEP) return;   // INIT / FINAL: every particle, in order
+    const int i = ch.i0 + threadIdx.x;
+    const bool valid = i < h.n;
+    const size_t p = (size_t)h.off + (valid ? i : 0);
+    const double ti = P.pt[p];
+    const bool blk = valid && ti < h.t_end && ti + P.pdt[p] == h.tnext;
+    const unsigned long long bal = __ballot(blk);
+    int base = 0;
+    if (threadIdx.x == 0 && bal) base = atomicAdd(&h.nact, __popcll(bal));
+    base = __shfl(base, 0);
+    if (blk) P.act[(size_t)h.off + base + __popcll(bal & ((1ull << threadIdx.x) - 1))] = i;
+    const double nxt = (valid && !blk && ti < h.t_end) ? ti + P.pdt[p] : __builtin_inf();
+    wave_atomic_min(&h.tmin, nxt);
+}
+
+__global__ void k_ens_bitems(const SysHdr *__restrict__ H, const int *__restrict__ sys,
+                             const int *__restrict__ nsys, BLists L)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= *nsys) return;
+    const int slot = sys[s];
+    const SysHdr &h = H[slot];
+    if (h.mode != 1 || h.phase < INIT || h.phase > FINAL) return;
+    const int cnt = h.phase == STEP ? h.nact : h.n;
+    const int nc = (cnt + 63) / 64;
+    if (nc == 0) return;
+    const int c0 = atomicAdd(&L.count[0], nc);
+    const int w0 = atomicAdd(&L.count[1], nc * h.nsplit);
+    for (int k = 0; k < nc; ++k) {
+        L.chunks[c0 + k] = Chunk{slot, 64 * k};
+        for (int sp = 0; sp < h.nsplit; ++sp)
+            L.items[w0 + k * h.nsplit + sp] = FItem{slot, 64 * k, sp, 0};
+    }
+}
+
+// Particle p's state predicted to the system's current evaluation time
+// (the block time in STEP, its own time otherwise).
+__device__ __forceinline__ void bpredict(const Pool &P, const SysHdr &h, size_t p, double xp[3],
+                                         double vp[3])
+{
+#pragma clang fp contract(off)
+    const size_t cap = P.cap;
+    double x[3], v[3], a[3], jk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = P.x[k * cap + p]; v[k] = P.v[k * cap + p];
+        a[k] = P.a[k * cap + p]; jk[k] = P.j[k * cap + p];
+    }
+    const double dt = h.phase == STEP ? h.tnext - P.pt[p] : 0.0;
+    ens_predict_one(x, v, a, jk, dt, xp, vp);
+}
+
+__global__ __launch_bounds__(64) void k_ens_bforce(Pool P, const SysHdr *__restrict__ H,
+                                                   const FItem *__restrict__ items,
+                                                   const int *__restrict__ nitems)
+{
+#pragma clang fp contract(off)
+    __shared__ double rec[64 * 8];                 // 64 records of 64 B (JPred or JPred64)
+    JPred64 *r64 = reinterpret_cast<JPred64 *>(rec);
+    JPred *r32 = reinterpret_cast<JPred *>(rec);
+    const int lane = threadIdx.x;
+    const int total = *nitems;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const FItem it = items[w];
+        const SysHdr &h = H[it.slot];
+        const bool blk = h.phase == STEP;
+        const int cnt = blk ? h.nact : h.n;
+        const ISel sel = select_i(P, h, blk, cnt, it.i0, lane);
+        const int self = sel.i;
+        const size_t off = (size_t)h.off;
+        double xi[3], vi[3];
+        bpredict(P, h, off + sel.ic, xi, vi);
+        const double mi = P.m[off + sel.ic];
+        const int jbeg = it.split * h.jchunk;
+        const int jend = min(jbeg + h.jchunk, h.n);
+        double A[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (h.prec == 64) {
+            const JPred64 I = make_jpred64(xi, vi, mi);
+            DAcc a;
+            a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.p = 0.0;
+            for (int t = jbeg; t < jend; t += 64) {
+                const int tn = min(64, jend - t);
+                __syncthreads();                   // the previous records have been read
+                if (lane < tn) {
+                    double xp[3], vp[3];
+                    bpredict(P, h, off + t + lane, xp, vp);
+                    r64[lane] = make_jpred64(xp, vp, P.m[off + t + lane]);
+                }
+                __syncthreads();
+                const unsigned rel = (unsigned)(self - t);
+                if (__ballot(rel < (unsigned)tn) == 0) {
+                    for (int jj = 0; jj < tn; ++jj) ens_pair64<false>(r64[jj], t + jj, self, I, h.eps2, a);
+                } else {
+                    for (int jj = 0; jj < tn; ++jj) ens_pair64<true>(r64[jj], t + jj, self, I, h.eps2, a);
+                }
+            }
+            A[0] = a.ax; A[1] = a.ay; A[2] = a.az;
+            A[3] = a.jx; A[4] = a.jy; A[5] = a.jz; A[6] = a.p;
+        } else {
+            const JPred I = make_jpred(xi, vi, mi);
+            const float eps2 = (float)h.eps2;
+            for (int t = jbeg; t < jend; t += kTile) {
+                const int tn = min(kTile, jend - t);
+                EAcc e;
+                e.ax = e.ay = e.az = e.jx = e.jy = e.jz = e.kx = e.ky = e.kz = e.p = 0.f;
+                for (int s0 = 0; s0 < tn; s0 += 64) {
+                    const int sn = min(64, tn - s0);
+                    __syncthreads();
+                    if (lane < sn) {
+                        double xp[3], vp[3];
+                        const size_t q = off + t + s0 + lane;
+                        bpredict(P, h, q, xp, vp);
+                        r32[lane] = make_jpred(xp, vp, P.m[q]);
+                    }
+                    __syncthreads();
+                    // pairs of the 256-j tile in ascending order, as ens_tile
+                    const unsigned rel = (unsigned)(self - (t + s0));
+                    if (__ballot(rel < (unsigned)sn) == 0)
+                        ens_tile<false>(r32, sn, rel, I, eps2, e);
+                    else
+                        ens_tile<true>(r32, sn, rel, I, eps2, e);
+                }
+                A[0] += (double)e.ax;
+                A[1] += (double)e.ay;
+                A[2] += (double)e.az;
+                A[3] += (double)e.jx - 3.0 * (double)e.kx;
+                A[4] += (double)e.jy - 3.0 * (double)e.ky;
+                A[5] += (double)e.jz - 3.0 * (double)e.kz;
+                A[6] += (double)e.p;
+            }
+        }
+        if (self >= 0) {
+            const size_t cap = P.cap;
+            const size_t q = off + sel.pos;   // partials by list position
+            double *out = P.part + (size_t)it.split * 7 * cap;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) out[c * cap + q] = A[c];
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_ens_bcorrect(Pool P, SysHdr *H,
+                                                     const Chunk *__restrict__ chunks,
+                                                     const int *__restrict__ nchunks,
+                                                     double dt_min)
+{
+#pragma clang fp contract(off)
+    const int total = *nchunks;
+    for (int c = blockIdx.x; c < total; c += gridDim.x) {
+        const Chunk ch = chunks[c];
+        SysHdr &h = H[ch.slot];
+        const int phase = h.phase;
+        const bool blk = phase == STEP;
+        const int cnt = blk ? h.nact : h.n;
+        const ISel sel = select_i(P, h, blk, cnt, ch.i0, threadIdx.x);
+        const int i = sel.i;
+        double next = __builtin_inf();
+        int fail = 0;
+        if (i >= 0) {
+            const size_t cap = P.cap;
+            const size_t p = (size_t)h.off + i;
+            const size_t q = (size_t)h.off + sel.pos;
+            double a1[3], j1[3], pot;
+            {
+                const double *in = P.part;
+                a1[0] = in[0 * cap + q]; a1[1] = in[1 * cap + q]; a1[2] = in[2 * cap + q];
+                j1[0] = in[3 * cap + q]; j1[1] = in[4 * cap + q]; j1[2] = in[5 * cap + q];
+                pot = in[6 * cap + q];
+            }
+            for (int s = 1; s < h.nsplit; ++s) {
+                const double *in = P.part + (size_t)s * 7 * cap;
+                a1[0] += in[0 * cap + q]; a1[1] += in[1 * cap + q]; a1[2] += in[2 * cap + q];
+                j1[0] += in[3 * cap + q]; j1[1] += in[4 * cap + q]; j1[2] += in[5 * cap + q];
+                pot += in[6 * cap + q];
+            }
+            PState st;
+            double xp[3], vp[3];
+            bpredict(P, h, p, xp, vp);
+            for (int k = 0; k < 3; ++k) {
+                st.x[k] = P.x[k * cap + p]; st.v[k] = P.v[k * cap + p];
+                st.a[k] = P.a[k * cap + p]; st.j[k] = P.j[k * cap + p];
+            }
+            st.pot = 0; st.pot0 = 0;
+            double ti = P.pt[p], di = P.pdt[p];
+            const double cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, di, h.eta, h.prec);
+            if (phase != FINAL) {
+                next = block_advance(ti, di, cand, phase, h, dt_min, &fail);
+                P.pt[p] = ti;
+                P.pdt[p] = di;
+            }
+            if (phase == INIT) {
+                for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = st.a[k]; P.j[k * cap + p] = st.j[k]; }
+                P.pot[p] = st.pot;
+                P.pot0[p] = st.pot0;
+            } else if (phase == STEP) {
+                for (int k = 0; k < 3; ++k) {
+                    P.x[k * cap + p] = st.x[k];
+                    P.v[k * cap + p] = st.v[k];
+                    P.a[k * cap + p] = st.a[k];
+                    P.j[k * cap + p] = st.j[k];
+                }
+                P.pot[p] = st.pot;
+            } else {
+                P.pot[p] = st.pot;
+            }
+        }
+        wave_atomic_min(&h.tmin, next);
+        if (__ballot(fail) && threadIdx.x == 0) atomicOr(&h.fail, 1);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1072,6 +1318,7 @@ struct Result {
 struct GraphExec {
     hipGraphExec_t exec = nullptr;
     int chunk_grid = 0, item_grid = 0, steps = 0;
+    int sel_grid = 0, sys_grid = 0;   // block path: select chunks, systems
 };
 
 // Systems run in lanes with their own work lists and streams:
@@ -1098,6 +1345,17 @@ struct Lane {
     long long steps = 0;
     int nmax = 0;                 // largest system of a workgroup-fused lane
     int n32 = 0;                  // its systems of precision 32 (listed first)
+    // block-step systems on the fused block path: every 64-particle chunk (for
+    // the select kernel) and the slots; the step's chunk and item lists are
+    // built on the device (d_bchunks, d_bitems, counters d_bcount)
+    Chunk *d_schunks = nullptr, *h_schunks = nullptr;
+    int *d_bsys = nullptr, *h_bsys = nullptr;
+    Chunk *d_bchunks = nullptr;
+    FItem *d_bitems = nullptr;
+    int *d_bcount = nullptr;
+    int *d_nsel = nullptr, *h_nsel = nullptr;   // [0] select chunks, [1] systems
+    int bcap_chunks = 0, bcap_items = 0, bcap_sys = 0;
+    int n_schunks = 0, n_bsys = 0, n_bitems_max = 0;
 };
 
 struct Engine {
@@ -1131,6 +1389,8 @@ struct Engine {
     int lookahead = 64;
     int precision = 64;
     int block_steps = 1;              // 1: kira's individual block steps, 0: one shared step
+    int block_fusion = 1;             // block steps: 1 = prediction inside the force kernel,
+                                      // device-built work lists; 0 = the four step kernels
     int light_max = 64;
     int fused_steps = 256;            // light lane: steps per fused launch (0: graphs)
     int wg_steps = 0;                 // medium lanes: steps per workgroup-fused launch (0: graphs;
@@ -1241,6 +1501,37 @@ int ensure_items(Lane &L, int chunks, int items)
         ECHK(hipHostMalloc((void **)&L.h_items, sizeof(FItem) * cap, hipHostMallocDefault));
         ECHK(hipMalloc((void **)&L.d_items, sizeof(FItem) * cap));
         L.item_cap = cap;
+        destroy_graphs(L);
+    }
+    return 0;
+}
+
+// Capacity of the lane's block-path lists: select chunks (= the most chunks a
+// step can list), systems, and force items.
+int ensure_blists(Lane &L, int chunks, int nsys, int items)
+{
+    if (chunks > L.bcap_chunks) {
+        int cap = std::max(chunks, L.bcap_chunks * 2);
+        hfree(L.h_schunks); dfree(L.d_schunks); dfree(L.d_bchunks);
+        ECHK(hipHostMalloc((void **)&L.h_schunks, sizeof(Chunk) * cap, hipHostMallocDefault));
+        ECHK(hipMalloc((void **)&L.d_schunks, sizeof(Chunk) * cap));
+        ECHK(hipMalloc((void **)&L.d_bchunks, sizeof(Chunk) * cap));
+        L.bcap_chunks = cap;
+        destroy_graphs(L);
+    }
+    if (nsys > L.bcap_sys) {
+        int cap = std::max(nsys, L.bcap_sys * 2);
+        hfree(L.h_bsys); dfree(L.d_bsys);
+        ECHK(hipHostMalloc((void **)&L.h_bsys, sizeof(int) * cap, hipHostMallocDefault));
+        ECHK(hipMalloc((void **)&L.d_bsys, sizeof(int) * cap));
+        L.bcap_sys = cap;
+        destroy_graphs(L);
+    }
+    if (items > L.bcap_items) {
+        int cap = std::max(items, L.bcap_items * 2);
+        dfree(L.d_bitems);
+        ECHK(hipMalloc((void **)&L.d_bitems, sizeof(FItem) * cap));
+        L.bcap_items = cap;
         destroy_graphs(L);
     }
     return 0;
@@ -1486,20 +1777,39 @@ int rebuild_items()
         ECHK(hipMemcpyAsync(L.d_counts, L.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice,
                             X.stream));
     }
+    const bool fused_light = X.fused_steps > 0 && X.light_max <= 64;
     for (int ln = 0; ln < 2; ++ln) {
         Lane &L = X.lanes[ln];
-        int nc = 0, ni = 0;
-        for (int s = 0; s < X.max_slots; ++s)
-            if (X.slots[s].busy && X.slots[s].lane == ln) {
-                const int ch = (X.slots[s].n + 63) / 64;
-                nc += ch;
-                ni += ch * X.slots[s].nsplit;
-            }
-        if (ensure_items(L, std::max(nc, 1), std::max(ni, 1))) return -1;
-        int c = 0, w = 0;
+        // block-step systems take the fused block path, except in the fused
+        // light-system kernel, which steps both modes itself
+        auto on_bpath = [&](const SlotInfo &si) {
+            return si.mode == 1 && X.block_fusion && !(ln == 1 && fused_light);
+        };
+        int nc = 0, ni = 0, bc = 0, bs = 0, bi = 0;
         for (int s = 0; s < X.max_slots; ++s) {
             const SlotInfo &si = X.slots[s];
             if (!si.busy || si.lane != ln) continue;
+            const int ch = (si.n + 63) / 64;
+            if (on_bpath(si)) {
+                bc += ch;
+                bs += 1;
+                bi += ch * si.nsplit;
+            } else {
+                nc += ch;
+                ni += ch * si.nsplit;
+            }
+        }
+        if (ensure_items(L, std::max(nc, 1), std::max(ni, 1))) return -1;
+        if (ensure_blists(L, std::max(bc, 1), std::max(bs, 1), std::max(bi, 1))) return -1;
+        int c = 0, w = 0, q = 0, y = 0;
+        for (int s = 0; s < X.max_slots; ++s) {
+            const SlotInfo &si = X.slots[s];
+            if (!si.busy || si.lane != ln) continue;
+            if (on_bpath(si)) {
+                L.h_bsys[y++] = s;
+                for (int i0 = 0; i0 < si.n; i0 += 64) L.h_schunks[q++] = Chunk{s, i0};
+                continue;
+            }
             for (int i0 = 0; i0 < si.n; i0 += 64) {
                 L.h_chunks[c++] = Chunk{s, i0};
                 for (int sp = 0; sp < si.nsplit; ++sp) L.h_items[w++] = FItem{s, i0, sp, 0};
@@ -1507,13 +1817,24 @@ int rebuild_items()
         }
         L.n_chunks = nc;
         L.n_items = ni;
+        L.n_schunks = bc;
+        L.n_bsys = bs;
+        L.n_bitems_max = bi;
         L.h_counts[0] = nc;
         L.h_counts[1] = ni;
+        L.h_nsel[0] = bc;
+        L.h_nsel[1] = bs;
         if (nc) ECHK(hipMemcpyAsync(L.d_chunks, L.h_chunks, sizeof(Chunk) * nc,
                                     hipMemcpyHostToDevice, X.stream));
         if (ni) ECHK(hipMemcpyAsync(L.d_items, L.h_items, sizeof(FItem) * ni,
                                     hipMemcpyHostToDevice, X.stream));
         ECHK(hipMemcpyAsync(L.d_counts, L.h_counts, sizeof(int) * 2, hipMemcpyHostToDevice,
+                            X.stream));
+        if (bc) ECHK(hipMemcpyAsync(L.d_schunks, L.h_schunks, sizeof(Chunk) * bc,
+                                    hipMemcpyHostToDevice, X.stream));
+        if (bs) ECHK(hipMemcpyAsync(L.d_bsys, L.h_bsys, sizeof(int) * bs, hipMemcpyHostToDevice,
+                                    X.stream));
+        ECHK(hipMemcpyAsync(L.d_nsel, L.h_nsel, sizeof(int) * 2, hipMemcpyHostToDevice,
                             X.stream));
     }
     ECHK(hipStreamSynchronize(X.stream));
@@ -1521,24 +1842,44 @@ int rebuild_items()
     return 0;
 }
 
+// Grid bucket: 0 for no work, else the next power of two from 64.
 int bucket(int n)
 {
+    if (n <= 0) return 0;
     int b = 64;
     while (b < n) b *= 2;
     return b;
 }
 
-void enqueue_step(int ln, int cgrid, int igrid)
+// Waves of the grid-stride block-path kernels: enough to fill the chip,
+// never more than the work can use.
+constexpr int kBForceWaves = 8192;
+constexpr int kBCorrectWaves = 2048;
+
+void enqueue_step(int ln, int cgrid, int igrid, int sgrid, int ygrid)
 {
     Lane &L = X.lanes[ln];
-    hipLaunchKernelGGL(k_ens_predict, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
-                       L.d_chunks, L.d_counts);
-    hipLaunchKernelGGL(k_ens_force, dim3(igrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
-                       L.d_items, L.d_counts + 1);
-    hipLaunchKernelGGL(k_ens_correct, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
-                       L.d_chunks, L.d_counts, 1e-30);
+    if (cgrid) {
+        hipLaunchKernelGGL(k_ens_predict, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                           L.d_chunks, L.d_counts);
+        hipLaunchKernelGGL(k_ens_force, dim3(igrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                           L.d_items, L.d_counts + 1);
+        hipLaunchKernelGGL(k_ens_correct, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                           L.d_chunks, L.d_counts, 1e-30);
+    }
+    if (sgrid) {
+        const BLists B{L.d_bchunks, L.d_bitems, L.d_bcount};
+        hipLaunchKernelGGL(k_ens_bselect, dim3(sgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                           L.d_schunks, L.d_nsel);
+        hipLaunchKernelGGL(k_ens_bitems, dim3((ygrid + 255) / 256), dim3(256), 0, L.stream,
+                           X.d_hdr, L.d_bsys, L.d_nsel + 1, B);
+        hipLaunchKernelGGL(k_ens_bforce, dim3(std::min(kBForceWaves, L.n_bitems_max)), dim3(64),
+                           0, L.stream, X.P, X.d_hdr, L.d_bitems, L.d_bcount + 1);
+        hipLaunchKernelGGL(k_ens_bcorrect, dim3(std::min(kBCorrectWaves, L.n_schunks)), dim3(64),
+                           0, L.stream, X.P, X.d_hdr, L.d_bchunks, L.d_bcount, 1e-30);
+    }
     hipLaunchKernelGGL(k_ens_finalize, dim3((X.max_slots + 255) / 256), dim3(256), 0, L.stream,
-                       X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30, ln);
+                       X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30, ln, L.d_bcount);
 }
 
 // A graph of steps_per_graph engine steps of lane ln for its grid buckets.
@@ -1546,18 +1887,22 @@ int get_graph(int ln, hipGraphExec_t &exec)
 {
     Lane &L = X.lanes[ln];
     const int cg = bucket(L.n_chunks), ig = bucket(L.n_items);
+    const int sg = bucket(L.n_schunks), yg = bucket(L.n_bsys);
     for (auto &g : L.graphs)
-        if (g.chunk_grid == cg && g.item_grid == ig && g.steps == X.steps_per_graph) {
+        if (g.chunk_grid == cg && g.item_grid == ig && g.sel_grid == sg && g.sys_grid == yg &&
+            g.steps == X.steps_per_graph) {
             exec = g.exec;
             return 0;
         }
     hipGraph_t graph;
     ECHK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < X.steps_per_graph; ++s) enqueue_step(ln, cg, ig);
+    for (int s = 0; s < X.steps_per_graph; ++s) enqueue_step(ln, cg, ig, sg, yg);
     ECHK(hipStreamEndCapture(L.stream, &graph));
     GraphExec ge;
     ge.chunk_grid = cg;
     ge.item_grid = ig;
+    ge.sel_grid = sg;
+    ge.sys_grid = yg;
     ge.steps = X.steps_per_graph;
     ECHK(hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
@@ -1591,6 +1936,8 @@ void release_all()
         destroy_graphs(L);
         hfree(L.h_chunks); dfree(L.d_chunks); hfree(L.h_items); dfree(L.d_items);
         dfree(L.d_counts); hfree(L.h_counts);
+        hfree(L.h_schunks); dfree(L.d_schunks); hfree(L.h_bsys); dfree(L.d_bsys);
+        dfree(L.d_bchunks); dfree(L.d_bitems); dfree(L.d_bcount); dfree(L.d_nsel); hfree(L.h_nsel);
         L.chunk_cap = L.item_cap = 0;
         if (L.ev0) (void)hipEventDestroy(L.ev0);
         if (L.ev1) (void)hipEventDestroy(L.ev1);
@@ -1624,6 +1971,10 @@ int open_engine(long long hbm_budget_bytes, int max_systems)
         ECHK(hipEventCreate(&L.ev1));
         ECHK(hipMalloc((void **)&L.d_counts, sizeof(int) * 2));
         ECHK(hipHostMalloc((void **)&L.h_counts, sizeof(int) * 2, hipHostMallocDefault));
+        ECHK(hipMalloc((void **)&L.d_bcount, sizeof(int) * 2));
+        ECHK(hipMemset(L.d_bcount, 0, sizeof(int) * 2));
+        ECHK(hipMalloc((void **)&L.d_nsel, sizeof(int) * 2));
+        ECHK(hipHostMalloc((void **)&L.h_nsel, sizeof(int) * 2, hipHostMallocDefault));
         L.steps = 0;
     }
     // the large workgroup-fused lane needs more than the default dynamic LDS
@@ -1763,7 +2114,8 @@ int g6e_run(int steps)
         if (busy_count() == 0 || done >= steps) break;
         t0 = clk::now();
         Lane &H = X.lanes[0], &Lt = X.lanes[1];
-        const bool heavy = H.n_chunks > 0, light = Lt.n_chunks > 0;
+        const bool heavy = H.n_chunks > 0 || H.n_schunks > 0;
+        const bool light = Lt.n_chunks > 0 || Lt.n_schunks > 0;
         const bool med[4] = {false, false, X.lanes[2].n_chunks > 0, X.lanes[3].n_chunks > 0};
         hipGraphExec_t eh = nullptr, el = nullptr;
         if (heavy && get_graph(0, eh)) return -1;
@@ -1916,6 +2268,11 @@ int g6e_set_option(const char *key, long long value)
     } else if (k == "lookahead") X.lookahead = (int)std::max(1LL, value);
     else if (k == "dt_max_bits") X.dt_max = std::ldexp(1.0, (int)value);
     else if (k == "max_steps") X.max_steps = std::max(1LL, value);
+    else if (k == "block_fusion") {
+        // decides the lists a resident block-step system is stepped from
+        if (X.open) return efail(-1, "%s is fixed while open", k.c_str());
+        X.block_fusion = value ? 1 : 0;
+    }
     else if (k == "light_max" || k == "fused_steps" || k == "wg_steps") {
         // they decide which lane and kernel each resident system steps in:
         // changed under resident systems, a lane could stop advancing
@@ -1953,6 +2310,7 @@ long long g6e_get_option(const char *key)
     if (k == "max_steps") return X.max_steps;
     if (k == "precision") return X.precision;
     if (k == "block_steps") return X.block_steps;
+    if (k == "block_fusion") return X.block_fusion;
     if (k == "light_max") return X.light_max;
     if (k == "fused_steps") return X.fused_steps;
     if (k == "wg_steps") return X.wg_steps;

@@ -6,6 +6,10 @@ default): kira's individual power-of-two block steps per system.
   against kira's rules restated in starlab_amd/blockstep.py), on the step
   graphs and in the fused light-system kernel;
 * with j-splits: agreement to rounding;
+* the fused block path (prediction inside the force kernel, work lists built
+  on the device, option block_fusion = 1, the default) is bitwise the four
+  step kernels on predicted records (block_fusion = 0), in both lanes, both
+  precisions, with and without j-splits;
 * a system's result is bitwise the same alone and in a mix that holds
   block-step and shared-step systems of both precisions, admitted at other
   times;
@@ -23,7 +27,7 @@ pytestmark = pytest.mark.gpu
 
 GB = 1 << 30
 DEFAULTS = dict(jsplit_target=512, precision=64, max_steps=50_000_000, steps_per_graph=8,
-                fused_steps=256, light_max=64, wg_steps=0, block_steps=1)
+                fused_steps=256, light_max=64, wg_steps=0, block_steps=1, block_fusion=1)
 PATHS = pytest.mark.parametrize("fused", [256, 0], ids=["fused", "graphs"])
 
 
@@ -90,6 +94,22 @@ def test_fp64_block_split_j_agrees_to_rounding():
         size = np.abs(j["x"]).max()
         assert r["steps"] == nb
         assert np.abs(r["x"] - xf).max() <= 1e-12 * size
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("jsplit", [0, 256])
+@pytest.mark.parametrize("fused", [256, 0], ids=["light-fused", "light-graphs"])
+def test_fused_block_path_equals_step_kernels_bitwise(prec, jsplit, fused):
+    jobs = _jobs(15, [3, 40, 64, 65, 300, 700, 1300, 2100], t_end=1.0 / 16)
+    a, sa = _run(jobs, precision=prec, jsplit_target=jsplit, fused_steps=fused, block_fusion=1)
+    b, sb = _run(jobs, precision=prec, jsplit_target=jsplit, fused_steps=fused, block_fusion=0)
+    for j, f, g in zip(jobs, a, b):
+        assert not f["failed"] and f["t"] == g["t"] == 1.0 / 16, len(j["m"])
+        assert f["steps"] == g["steps"], len(j["m"])
+        np.testing.assert_array_equal(f["x"], g["x"])
+        np.testing.assert_array_equal(f["v"], g["v"])
+        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
+    assert sa["pairs"] == sb["pairs"]
 
 
 @pytest.mark.parametrize("prec", [64, 32])
