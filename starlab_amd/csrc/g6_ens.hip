@@ -688,11 +688,21 @@ __global__ void k_ens_finalize(SysHdr *__restrict__ H, int nslots, int *__restri
 //                  into the system's minimum;
 //   k_ens_bitems   thread per block-mode system: appends the system's work to
 //                  the lane's device lists, (system, 64-i chunk) for the
-//                  corrector and (system, chunk, j-split) for the force;
+//                  corrector and (system, chunk, j-split) for the force, with
+//                  one atomic per wave;
 //   k_ens_bforce   grid-stride over the force items: each wave predicts its
 //                  j-split 64 j at a time (one per lane) into LDS and sums the
-//                  pairs from there, in the same order as k_ens_force on
-//                  predicted records, so results are bitwise equal;
+//                  pairs from there.  Precision 64, and precision 32 for
+//                  blocks of more than 32 i: one lane per i, in the same order
+//                  as k_ens_force on predicted records, so results are bitwise
+//                  equal.  Precision 32 blocks of at most 32 i (most blocks
+//                  hold a few particles): NIP i-slots (the next power of two)
+//                  of Q = 64 / NIP lanes each; lane q of a slot takes the j
+//                  congruent to q mod Q, folds its FP32 sums of each 256-j
+//                  tile into FP64, and the slot's lanes add their sums in a
+//                  fixed butterfly.  The order depends only on the block's
+//                  size, so a system's trajectory still depends only on its
+//                  own particles;
 //   k_ens_bcorrect grid-stride over the chunks: sums the splits, recomputes
 //                  the particle's own prediction, corrects it and takes its
 //                  next step;
@@ -714,37 +724,62 @@ __global__ __launch_bounds__(64) void k_ens_bselect(Pool P, SysHdr *H,
     const Chunk ch = chunks[c];
     SysHdr &h = H[ch.slot];
     if (h.mode != 1 || h.phase != STEP) return;   // INIT / FINAL: every particle, in order
-    const int i = ch.i0 + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int i = ch.i0 + lane;
     const bool valid = i < h.n;
     const size_t p = (size_t)h.off + (valid ? i : 0);
     const double ti = P.pt[p];
     const bool blk = valid && ti < h.t_end && ti + P.pdt[p] == h.tnext;
     const unsigned long long bal = __ballot(blk);
     int base = 0;
-    if (threadIdx.x == 0 && bal) base = atomicAdd(&h.nact, __popcll(bal));
+    if (lane == 0 && bal) base = atomicAdd(&h.nact, __popcll(bal));
     base = __shfl(base, 0);
-    if (blk) P.act[(size_t)h.off + base + __popcll(bal & ((1ull << threadIdx.x) - 1))] = i;
+    if (blk) P.act[(size_t)h.off + base + __popcll(bal & ((1ull << lane) - 1))] = i;
     const double nxt = (valid && !blk && ti < h.t_end) ? ti + P.pdt[p] : __builtin_inf();
     wave_atomic_min(&h.tmin, nxt);
 }
 
-__global__ void k_ens_bitems(const SysHdr *__restrict__ H, const int *__restrict__ sys,
-                             const int *__restrict__ nsys, BLists L)
+// Inclusive prefix sum over the wave.
+__device__ __forceinline__ int wave_scan(int v)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= *nsys) return;
-    const int slot = sys[s];
-    const SysHdr &h = H[slot];
-    if (h.mode != 1 || h.phase < INIT || h.phase > FINAL) return;
-    const int cnt = h.phase == STEP ? h.nact : h.n;
-    const int nc = (cnt + 63) / 64;
-    if (nc == 0) return;
-    const int c0 = atomicAdd(&L.count[0], nc);
-    const int w0 = atomicAdd(&L.count[1], nc * h.nsplit);
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Thread per block-mode system (64-thread workgroups): the system's work
+// items, placed by a prefix sum over the wave and one atomic per wave.
+__global__ __launch_bounds__(64) void k_ens_bitems(const SysHdr *__restrict__ H,
+                                                   const int *__restrict__ sys,
+                                                   const int *__restrict__ nsys, BLists L)
+{
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const int lane = threadIdx.x;
+    int slot = -1, nc = 0, ns = 0;
+    if (s < *nsys) {
+        slot = sys[s];
+        const SysHdr &h = H[slot];
+        if (h.mode == 1 && h.phase >= INIT && h.phase <= FINAL) {
+            const int cnt = h.phase == STEP ? h.nact : h.n;
+            nc = (cnt + 63) / 64;
+            ns = h.nsplit;
+        }
+    }
+    const int pc = wave_scan(nc), pi = wave_scan(nc * ns);
+    int bc = 0, bi = 0;
+    if (lane == 63) {
+        if (pc) bc = atomicAdd(&L.count[0], pc);
+        if (pi) bi = atomicAdd(&L.count[1], pi);
+    }
+    bc = __shfl(bc, 63);
+    bi = __shfl(bi, 63);
+    const int c0 = bc + pc - nc, w0 = bi + pi - nc * ns;
     for (int k = 0; k < nc; ++k) {
         L.chunks[c0 + k] = Chunk{slot, 64 * k};
-        for (int sp = 0; sp < h.nsplit; ++sp)
-            L.items[w0 + k * h.nsplit + sp] = FItem{slot, 64 * k, sp, 0};
+        for (int sp = 0; sp < ns; ++sp) L.items[w0 + k * ns + sp] = FItem{slot, 64 * k, sp, 0};
     }
 }
 
@@ -770,9 +805,14 @@ __global__ __launch_bounds__(64) void k_ens_bforce(Pool P, const SysHdr *__restr
                                                    const int *__restrict__ nitems)
 {
 #pragma clang fp contract(off)
-    __shared__ double rec[64 * 8];                 // 64 records of 64 B (JPred or JPred64)
+    // 64 predicted j: JPred64 records (FP64), JPred records (FP32, one lane
+    // per i: every lane reads the same record), or the ten JPred fields as
+    // separate arrays (FP32 packed: lanes read different j, which then sit
+    // in different banks)
+    __shared__ double rec[64 * 8];
     JPred64 *r64 = reinterpret_cast<JPred64 *>(rec);
     JPred *r32 = reinterpret_cast<JPred *>(rec);
+    float (*f32)[64] = reinterpret_cast<float (*)[64]>(rec);
     const int lane = threadIdx.x;
     const int total = *nitems;
     for (int w = blockIdx.x; w < total; w += gridDim.x) {
@@ -780,7 +820,18 @@ __global__ __launch_bounds__(64) void k_ens_bforce(Pool P, const SysHdr *__restr
         const SysHdr &h = H[it.slot];
         const bool blk = h.phase == STEP;
         const int cnt = blk ? h.nact : h.n;
-        const ISel sel = select_i(P, h, blk, cnt, it.i0, lane);
+        // i-slots: one per lane, or for FP32 blocks of at most 32 i NIP slots
+        // of Q lanes.  The choice depends on the block's size, not on the
+        // chunk: the block list is filled in no fixed order, so a particle's
+        // chunk is not reproducible, while its arithmetic must be.
+        int nip = 64;
+        if (h.prec != 64 && cnt <= 32) {
+            nip = 1;
+            while (nip < cnt) nip *= 2;
+        }
+        const int Q = 64 / nip;
+        const int slot = lane / Q, q = lane - slot * Q;
+        const ISel sel = select_i(P, h, blk, cnt, it.i0, slot);
         const int self = sel.i;
         const size_t off = (size_t)h.off;
         double xi[3], vi[3];
@@ -823,17 +874,42 @@ __global__ __launch_bounds__(64) void k_ens_bforce(Pool P, const SysHdr *__restr
                     __syncthreads();
                     if (lane < sn) {
                         double xp[3], vp[3];
-                        const size_t q = off + t + s0 + lane;
-                        bpredict(P, h, q, xp, vp);
-                        r32[lane] = make_jpred(xp, vp, P.m[q]);
+                        const size_t qj = off + t + s0 + lane;
+                        bpredict(P, h, qj, xp, vp);
+                        const JPred r = make_jpred(xp, vp, P.m[qj]);
+                        if (Q == 1) {
+                            r32[lane] = r;
+                        } else {
+                            f32[0][lane] = r.xh; f32[1][lane] = r.yh; f32[2][lane] = r.zh;
+                            f32[3][lane] = r.xl; f32[4][lane] = r.yl; f32[5][lane] = r.zl;
+                            f32[6][lane] = r.vx; f32[7][lane] = r.vy; f32[8][lane] = r.vz;
+                            f32[9][lane] = r.m;
+                        }
                     }
                     __syncthreads();
-                    // pairs of the 256-j tile in ascending order, as ens_tile
+                    // one lane per i: the tile's pairs in ascending order, as
+                    // ens_tile; Q lanes per i: lane q takes the j congruent to
+                    // q mod Q
                     const unsigned rel = (unsigned)(self - (t + s0));
-                    if (__ballot(rel < (unsigned)sn) == 0)
-                        ens_tile<false>(r32, sn, rel, I, eps2, e);
-                    else
-                        ens_tile<true>(r32, sn, rel, I, eps2, e);
+                    const bool mask = __ballot(rel < (unsigned)sn) != 0;
+                    if (Q == 1) {
+                        if (!mask) ens_tile<false>(r32, sn, rel, I, eps2, e);
+                        else ens_tile<true>(r32, sn, rel, I, eps2, e);
+                    } else {
+                        auto rec_at = [&](int jj) {
+                            JPred r;
+                            r.xh = f32[0][jj]; r.yh = f32[1][jj]; r.zh = f32[2][jj];
+                            r.xl = f32[3][jj]; r.yl = f32[4][jj]; r.zl = f32[5][jj];
+                            r.vx = f32[6][jj]; r.vy = f32[7][jj]; r.vz = f32[8][jj];
+                            r.m = f32[9][jj];
+                            return r;
+                        };
+                        if (!mask) {
+                            for (int jj = q; jj < sn; jj += Q) ens_pair<false>(rec_at(jj), jj, rel, I, eps2, e);
+                        } else {
+                            for (int jj = q; jj < sn; jj += Q) ens_pair<true>(rec_at(jj), jj, rel, I, eps2, e);
+                        }
+                    }
                 }
                 A[0] += (double)e.ax;
                 A[1] += (double)e.ay;
@@ -843,13 +919,19 @@ __global__ __launch_bounds__(64) void k_ens_bforce(Pool P, const SysHdr *__restr
                 A[5] += (double)e.jz - 3.0 * (double)e.kz;
                 A[6] += (double)e.p;
             }
+            // butterfly over a slot's Q lanes (a + b and b + a round alike, so
+            // every lane of the slot ends with the same sums)
+            for (int o = 1; o < Q; o <<= 1) {
+#pragma unroll
+                for (int c = 0; c < 7; ++c) A[c] += __shfl_xor(A[c], o);
+            }
         }
-        if (self >= 0) {
+        if (q == 0 && self >= 0) {
             const size_t cap = P.cap;
-            const size_t q = off + sel.pos;   // partials by list position
+            const size_t qo = off + sel.pos;   // partials by list position
             double *out = P.part + (size_t)it.split * 7 * cap;
 #pragma unroll
-            for (int c = 0; c < 7; ++c) out[c * cap + q] = A[c];
+            for (int c = 0; c < 7; ++c) out[c * cap + qo] = A[c];
         }
     }
 }
@@ -1346,8 +1428,8 @@ struct Lane {
     int nmax = 0;                 // largest system of a workgroup-fused lane
     int n32 = 0;                  // its systems of precision 32 (listed first)
     // block-step systems on the fused block path: every 64-particle chunk (for
-    // the select kernel) and the slots; the step's chunk and item lists are
-    // built on the device (d_bchunks, d_bitems, counters d_bcount)
+    // the select kernel); the step's chunk and item lists are built on the
+    // device (d_bchunks, d_bitems, counters d_bcount)
     Chunk *d_schunks = nullptr, *h_schunks = nullptr;
     int *d_bsys = nullptr, *h_bsys = nullptr;
     Chunk *d_bchunks = nullptr;
@@ -1854,7 +1936,7 @@ int bucket(int n)
 // Waves of the grid-stride block-path kernels: enough to fill the chip,
 // never more than the work can use.
 constexpr int kBForceWaves = 8192;
-constexpr int kBCorrectWaves = 2048;
+constexpr int kBCorrectWaves = 8192;
 
 void enqueue_step(int ln, int cgrid, int igrid, int sgrid, int ygrid)
 {
@@ -1871,8 +1953,8 @@ void enqueue_step(int ln, int cgrid, int igrid, int sgrid, int ygrid)
         const BLists B{L.d_bchunks, L.d_bitems, L.d_bcount};
         hipLaunchKernelGGL(k_ens_bselect, dim3(sgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
                            L.d_schunks, L.d_nsel);
-        hipLaunchKernelGGL(k_ens_bitems, dim3((ygrid + 255) / 256), dim3(256), 0, L.stream,
-                           X.d_hdr, L.d_bsys, L.d_nsel + 1, B);
+        hipLaunchKernelGGL(k_ens_bitems, dim3(ygrid / 64), dim3(64), 0, L.stream, X.d_hdr,
+                           L.d_bsys, L.d_nsel + 1, B);
         hipLaunchKernelGGL(k_ens_bforce, dim3(std::min(kBForceWaves, L.n_bitems_max)), dim3(64),
                            0, L.stream, X.P, X.d_hdr, L.d_bitems, L.d_bcount + 1);
         hipLaunchKernelGGL(k_ens_bcorrect, dim3(std::min(kBCorrectWaves, L.n_schunks)), dim3(64),

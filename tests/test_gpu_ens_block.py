@@ -8,8 +8,16 @@ default): kira's individual power-of-two block steps per system.
 * with j-splits: agreement to rounding;
 * the fused block path (prediction inside the force kernel, work lists built
   on the device, option block_fusion = 1, the default) is bitwise the four
-  step kernels on predicted records (block_fusion = 0), in both lanes, both
-  precisions, with and without j-splits;
+  step kernels on predicted records (block_fusion = 0) in FP64, in both
+  lanes, with and without j-splits.  In FP32 the fused path sums chunks of
+  at most 32 i with several lanes per i (a different FP32 order), so there
+  the two agree to FP32 rounding: positions within 1e-9 of the system size
+  (measured up to 6e-11 against FP64 for either) and the same energy
+  error to 1e-6.  Block counts are not compared between FP32 orders: kira's
+  criterion differences successive forces, so over a short span FP32 noise
+  can halve a few steps more in one order than the other (one 1300-body
+  system took 665 blocks packed, 4837 unpacked, 694 in FP64).  Over config
+  4 to t = 1 the FP32 block counts stay within 6 % of FP64 (last test);
 * a system's result is bitwise the same alone and in a mix that holds
   block-step and shared-step systems of both precisions, admitted at other
   times;
@@ -43,6 +51,25 @@ def _jobs(seed, sizes, t_end, eta=0.05):
             eps2 = (4.0 / n) ** 2
         out.append(dict(m=m, x=x, v=v, t_end=t_end, eta=eta, eps2=eps2))
     return out
+
+
+def _agree(f, g, prec, job, xtol=1e-9):
+    """Two runs of one job: bitwise in FP64; in FP32 to the pair law's
+    rounding (positions within xtol of the system size, relative energy
+    errors within 1e-6 of each other: the FP32 potential differs in the last
+    digits between sum orders, and a three-body scattering's total energy is
+    a small difference of kinetic and potential energy)."""
+    if prec == 64:
+        assert f["steps"] == g["steps"], len(job["m"])
+        np.testing.assert_array_equal(f["x"], g["x"])
+        np.testing.assert_array_equal(f["v"], g["v"])
+        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
+    else:
+        size = np.abs(job["x"]).max()
+        print(len(job["m"]), "blocks", f["steps"], g["steps"], "max|dx|/size",
+              np.abs(f["x"] - g["x"]).max() / size, flush=True)
+        assert np.abs(f["x"] - g["x"]).max() <= xtol * size
+        assert abs(f["e1"] / f["e0"] - g["e1"] / g["e0"]) <= 1e-6
 
 
 def _run(jobs, budget=2 * GB, max_systems=4096, **opt):
@@ -99,29 +126,30 @@ def test_fp64_block_split_j_agrees_to_rounding():
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("jsplit", [0, 256])
 @pytest.mark.parametrize("fused", [256, 0], ids=["light-fused", "light-graphs"])
-def test_fused_block_path_equals_step_kernels_bitwise(prec, jsplit, fused):
+def test_fused_block_path_equals_step_kernels(prec, jsplit, fused):
     jobs = _jobs(15, [3, 40, 64, 65, 300, 700, 1300, 2100], t_end=1.0 / 16)
     a, sa = _run(jobs, precision=prec, jsplit_target=jsplit, fused_steps=fused, block_fusion=1)
     b, sb = _run(jobs, precision=prec, jsplit_target=jsplit, fused_steps=fused, block_fusion=0)
     for j, f, g in zip(jobs, a, b):
         assert not f["failed"] and f["t"] == g["t"] == 1.0 / 16, len(j["m"])
-        assert f["steps"] == g["steps"], len(j["m"])
-        np.testing.assert_array_equal(f["x"], g["x"])
-        np.testing.assert_array_equal(f["v"], g["v"])
-        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
-    assert sa["pairs"] == sb["pairs"]
+        _agree(f, g, prec, j)
+    if prec == 64:
+        assert sa["pairs"] == sb["pairs"]
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-def test_fused_kernel_equals_step_graphs_bitwise(prec):
+def test_fused_light_kernel_equals_step_graphs(prec):
+    """Light systems in the fused light-system kernel against the same
+    systems on the block path's step graphs: bitwise in FP64, to FP32
+    rounding in FP32 (where the graphs pack several lanes per i; over a whole
+    time unit of three-body encounters positions agree to 1e-7 of the system
+    size, measured 4e-9)."""
     jobs = _jobs(9, [3, 3, 3, 5, 17, 40, 64], t_end=1.0)
     fused, _ = _run(jobs, precision=prec, fused_steps=256)
     graphs, _ = _run(jobs, precision=prec, fused_steps=0)
-    for f, g in zip(fused, graphs):
-        assert f["steps"] == g["steps"] and f["t"] == g["t"] == 1.0
-        np.testing.assert_array_equal(f["x"], g["x"])
-        np.testing.assert_array_equal(f["v"], g["v"])
-        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
+    for j, f, g in zip(jobs, fused, graphs):
+        assert f["t"] == g["t"] == 1.0
+        _agree(f, g, prec, j, xtol=1e-7)
 
 
 @PATHS
