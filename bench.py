@@ -122,8 +122,9 @@ def cpu_leg(inp, gpu, target_s=10.0, clusters=None, kira=None):
     contraction) is timed beside it.  Config 3: the kira leg's pair
     interactions at that rate (the CPU force path of kira, one N-body unit).
     Config 4: a size-stratified sample of the ensemble's clusters integrated
-    with the same shared-step Hermite rules in FP64, one system per host
-    thread.  Errors against FP64 for configs 1-3 from the kept GPU samples."""
+    with the same block-step Hermite rules in FP64 (oracle.hermite_block),
+    one system per host thread.  Errors against FP64 for configs 1-3 from the
+    kept GPU samples."""
     import oracle
     from concurrent.futures import ThreadPoolExecutor
     n = len(inp["m"])
@@ -171,9 +172,12 @@ def cpu_leg(inp, gpu, target_s=10.0, clusters=None, kira=None):
         sample = clusters["sample"]
 
         def one(j):
-            _, _, _, steps, _, _, _ = oracle.hermite_shared(j["m"], j["x"], j["v"], j["t_end"],
-                                                            j["eta"], j["eps2"], L=L)
-            return steps * len(j["m"]) * (len(j["m"]) - 1)
+            # pair interactions counted as the engine counts them: each
+            # particle step, plus every particle at the start and the end
+            n = len(j["m"])
+            _, _, _, _, psteps, _, _, _ = oracle.hermite_block(j["m"], j["x"], j["v"], j["t_end"],
+                                                               j["eta"], j["eps2"], L=L)
+            return (psteps + 2 * n) * (n - 1)
 
         t0 = time.perf_counter()
         with ThreadPoolExecutor(threads) as ex:
@@ -182,7 +186,8 @@ def cpu_leg(inp, gpu, target_s=10.0, clusters=None, kira=None):
         out["config4"] = {"pairs_per_s": pairs / dt4, "systems_per_s": len(sample) / dt4,
                           "seconds": round(dt4, 3), "threads": threads,
                           "sample": f"{len(sample)} of the 4096 clusters (every "
-                                    f"{clusters['stride']}th), FP64, to t = {sample[0]['t_end']}",
+                                    f"{clusters['stride']}th), FP64, block steps, to "
+                                    f"t = {sample[0]['t_end']}",
                           "gpu_batch_pairs_per_s": clusters.get("gpu_pairs_per_s"),
                           "gpu_batch_t1_s_at_cpu_rate":
                               clusters["gpu_pairs_t1"] / (pairs / dt4)
@@ -202,26 +207,31 @@ def ensemble_legs(seconds=2.0):
     """Batched ensembles and the served stream.
 
     batch_fp32_t1: 4096 Plummer clusters, N ~ U[32, 2048], FP32 pairs,
-    integrated to t = 1 (the batched-ensemble configuration as specified);
-    batch_fp64: the same clusters in FP64 to t = 1/32; both with engine shapes
-    from the ensemble autotuner.  serving: a Poisson stream of 2000 req/s,
-    70 % binary-single scatterings (FP64) to t = 32, 30 % clusters
-    N ~ U[256, 4096] (FP32) to t = 1/32."""
+    integrated to t = 1 on kira's block steps (the batched-ensemble
+    configuration as specified); batch_fp32_t1_shared: the same on one shared
+    step per system (every particle moves at every step); batch_fp64: the
+    clusters in FP64 to t = 1/32 on block steps; all with engine shapes from
+    the ensemble autotuner.  serving: a Poisson stream of 2000 req/s, 70 %
+    binary-single scatterings (FP64) to t = 32, 30 % clusters N ~ U[256,
+    4096] (FP32) to t = 1/32, on block steps."""
     from starlab_amd import serve
-    keys = ("jobs", "wall_s", "systems_per_s", "pairs_per_s", "p50_s", "p99_s", "de_median",
-            "de_max", "failed", "tuned")
+    keys = ("jobs", "wall_s", "systems_per_s", "pairs", "pairs_per_s", "engine_steps", "p50_s",
+            "p99_s", "de_median", "de_max", "failed", "tuned")
     out = {}
-    for name, prec, t_end in (("batch_fp32_t1", 32, 1.0), ("batch_fp64", 64, 1.0 / 32)):
+    for name, prec, t_end, block in (("batch_fp32_t1", 32, 1.0, 1),
+                                     ("batch_fp32_t1_shared", 32, 1.0, 0),
+                                     ("batch_fp64", 64, 1.0 / 32, 1)):
         batch = serve.cluster_batch(4096, 32, 2048, 4, t_end, prec=prec)
-        r = serve.run_batch(batch, tune=True)
+        r = serve.run_batch(batch, tune=True, block_steps=block)
         out[name] = {k: r[k] for k in keys}
         out[name]["t_end"] = t_end
+        out[name]["steps"] = "block" if block else "shared"
     jobs = serve.serving_mix(int(2000 * seconds), 5)
     r = serve.run_poisson(jobs, 2000.0)
     out["serving"] = r
     # error against FP64: the same 256 clusters (every 16th of the batch) in
-    # both precisions to t = 1/32; the FP64 engine is the oracle integrator
-    # bit for bit (tests/test_gpu_ens.py)
+    # both precisions to t = 1/32 on block steps; the FP64 engine is the
+    # block-step oracle integrator bit for bit (tests/test_gpu_ens_block.py)
     sample = serve.cluster_batch(4096, 32, 2048, 4, 1.0 / 32, prec=64)[::16]
     r64 = serve.run_batch(sample, keep=True)["results"]
     for j in sample:
@@ -597,8 +607,7 @@ def main():
         clusters = {"sample": serve.cluster_batch(4096, 32, 2048, 4, 1.0 / 32)[::stride],
                     "stride": stride,
                     "gpu_pairs_per_s": ens["batch_fp32_t1"]["pairs_per_s"],
-                    "gpu_pairs_t1": ens["batch_fp32_t1"]["pairs_per_s"]
-                    * ens["batch_fp32_t1"]["wall_s"]}
+                    "gpu_pairs_t1": ens["batch_fp32_t1"]["pairs"]}
     if not a.no_cpu:
         log("bench: CPU baselines and errors against FP64")
         cb, errs = cpu_leg(inp, gpu, clusters=clusters, kira=kira)
@@ -606,8 +615,8 @@ def main():
         if "ensemble" in out:
             e = out["ensemble"]["error_vs_fp64"]
             errs["config4"] = e
-            errs["config5"] = {"3body": "FP64 pairs: the oracle integrator bit for bit "
-                                        "(tests/test_gpu_ens.py)",
+            errs["config5"] = {"3body": "FP64 pairs on block steps: the block-step oracle "
+                                        "integrator bit for bit (tests/test_gpu_ens_block.py)",
                                "clusters": "FP32 pairs, as config4"}
         out["error_vs_fp64"] = errs
     print(json.dumps(out), flush=True)

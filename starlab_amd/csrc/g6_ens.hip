@@ -317,11 +317,13 @@ __device__ void ens_force64(const Pool &P, const SysHdr &h, const FItem &it, con
     const int n = h.n;
     const int i = sel.i;
     const int ic = sel.ic;
-    const JPred64 *__restrict__ base = P.jp64 + h.off;
+    // the record address is the same in every lane: readfirstlane says so,
+    // which keeps the j records on scalar loads into SGPRs
+    const JPred64 *__restrict__ base = P.jp64 + __builtin_amdgcn_readfirstlane(h.off);
     const JPred64 I = base[ic];
     const int self = i;
-    const int jbeg = it.split * h.jchunk;
-    const int jend = min(jbeg + h.jchunk, n);
+    const int jbeg = __builtin_amdgcn_readfirstlane(it.split * h.jchunk);
+    const int jend = __builtin_amdgcn_readfirstlane(min(jbeg + h.jchunk, n));
     const double eps2 = h.eps2;
     DAcc a;
     a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.p = 0.0;
@@ -369,12 +371,13 @@ __global__ __launch_bounds__(64) void k_ens_force(Pool P, const SysHdr *__restri
     const int n = h.n;
     const int i = sel.i;
     const int ic = sel.ic;
-    const JPred *__restrict__ base = P.jp + h.off;
+    // uniform record address (readfirstlane): the j records stay on scalar loads
+    const JPred *__restrict__ base = P.jp + __builtin_amdgcn_readfirstlane(h.off);
     const JPred I = base[ic];
     const int self = i;
     const float eps2 = (float)h.eps2;
-    const int jbeg = it.split * h.jchunk;
-    const int jend = min(jbeg + h.jchunk, n);
+    const int jbeg = __builtin_amdgcn_readfirstlane(it.split * h.jchunk);
+    const int jend = __builtin_amdgcn_readfirstlane(min(jbeg + h.jchunk, n));
     double A[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int t = jbeg; t < jend; t += kTile) {
         const int tn = min(kTile, jend - t);

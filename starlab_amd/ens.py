@@ -125,8 +125,10 @@ class Ensemble:
 
 # Engine options the tuner chooses among: j per split within a system (the
 # split-j shape of the step kernels), engine steps per graph replay, and
-# whether medium systems run in the workgroup-fused kernel.
+# whether medium systems run in the workgroup-fused kernel (shared-step
+# systems only: block-step systems always take the step graphs).
 TUNE_SPACE = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32), wg_steps=(0, 32))
+TUNE_SPACE_BLOCK = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32))
 
 
 def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192,
@@ -135,13 +137,19 @@ def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192
 
     Takes a size-stratified sample of `jobs` (dicts with m, x, v, eta, eps2
     and optionally prec), runs copies of it to t = t_probe once per candidate
-    option set (the product of `space`, default TUNE_SPACE) on an idle engine,
+    option set (the product of `space`; default TUNE_SPACE, or
+    TUNE_SPACE_BLOCK when the jobs run on block steps) on an idle engine,
     and returns (best options, {options tuple: seconds per pair interaction},
     seconds spent).  The jobs themselves are not touched.  `fixed` options
-    are applied to every candidate."""
+    are applied to every candidate (block_steps among them: the jobs' step
+    mode, else the engine's current one)."""
     import itertools
     import time as _time
-    space = TUNE_SPACE if space is None else space
+    if space is None:
+        block = (fixed or {}).get("block_steps")
+        if block is None:
+            block = load().g6e_get_option(b"block_steps")
+        space = TUNE_SPACE_BLOCK if block else TUNE_SPACE
     order = sorted(range(len(jobs)), key=lambda k: len(jobs[k]["m"]))
     step = max(1, len(order) // max(1, sample))
     probe = [jobs[k] for k in order[::step]][:sample]

@@ -65,7 +65,8 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep
     result (x, v, t, steps, e0, e1, failed) in job order."""
     tuned = None
     if tune:
-        best, timings, secs = autotune(jobs, max_systems=max_systems)
+        mode = {k: v for k, v in opt.items() if k in ("block_steps", "block_fusion")}
+        best, timings, secs = autotune(jobs, max_systems=max_systems, fixed=mode)
         opt = {**opt, **best}
         chunk = best.get("steps_per_graph", chunk)
         tuned = dict(options=best, seconds=round(secs, 3), candidates=len(timings))
