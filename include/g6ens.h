@@ -79,6 +79,14 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * double-single positions with a noise-floored step criterion),
  * "block_steps" (time steps of jobs submitted afterwards: 1 = kira's
  * individual block steps, the default; 0 = one shared step per system),
+ * "block_fusion" (block-step systems: 1 = prediction inside the force kernel
+ * and work lists built on the device, the default; 0 = the four step
+ * kernels; fixed while open), "kernel_timing" (1: the step kernels run one
+ * at a time between HIP events instead of as graphs, for diagnosis; read
+ * the device microseconds per kernel since g6e_open as "kernel_us_predict",
+ * "kernel_us_force", "kernel_us_correct", "kernel_us_bselect",
+ * "kernel_us_bitems", "kernel_us_bforce", "kernel_us_bcorrect",
+ * "kernel_us_finalize", "kernel_us_fused"),
  * "max_steps" (per system; beyond it the job fails), "dt_max_bits" (largest
  * block step 2^bits, default -3),
  * "light_max" (systems with n <= light_max run in the light lane, which

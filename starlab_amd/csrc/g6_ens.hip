@@ -1490,7 +1490,18 @@ struct Engine {
     // host time of g6e_run by phase (seconds): retire, admit, rebuild, graphs,
     // waiting for the lanes, the phase download
     double host_s[6] = {0, 0, 0, 0, 0, 0};
+    // option kernel_timing: lanes 0 and 1 launch their step kernels one by one
+    // (no graphs) between HIP events; device milliseconds per kernel
+    int kernel_timing = 0;
+    double kernel_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t kev[2] = {nullptr, nullptr};
 };
+
+// kernel_ms slots, named kernel_us_<name> by g6e_get_option
+enum KernelSlot { KS_PREDICT, KS_FORCE, KS_CORRECT, KS_BSELECT, KS_BITEMS, KS_BFORCE, KS_BCORRECT,
+                  KS_FINALIZE, KS_FUSED, KS_N };
+const char *const kKernelNames[KS_N] = {"predict", "force", "correct", "bselect", "bitems",
+                                        "bforce", "bcorrect", "finalize", "fused"};
 
 Engine X;
 
@@ -1967,6 +1978,66 @@ void enqueue_step(int ln, int cgrid, int igrid, int sgrid, int ygrid)
                        X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30, ln, L.d_bcount);
 }
 
+// One engine step of lane ln launched kernel by kernel, each between two
+// events whose elapsed time is added to kernel_ms (option kernel_timing).
+int timed_step(int ln)
+{
+    Lane &L = X.lanes[ln];
+    const int cgrid = bucket(L.n_chunks), igrid = bucket(L.n_items);
+    const int sgrid = bucket(L.n_schunks), ygrid = bucket(L.n_bsys);
+    auto timed = [&](int slot, auto launch) -> int {
+        ECHK(hipEventRecord(X.kev[0], L.stream));
+        launch();
+        ECHK(hipGetLastError());
+        ECHK(hipEventRecord(X.kev[1], L.stream));
+        ECHK(hipEventSynchronize(X.kev[1]));
+        float ms = 0;
+        ECHK(hipEventElapsedTime(&ms, X.kev[0], X.kev[1]));
+        X.kernel_ms[slot] += ms;
+        return 0;
+    };
+    if (cgrid) {
+        if (timed(KS_PREDICT, [&] {
+                hipLaunchKernelGGL(k_ens_predict, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                                   L.d_chunks, L.d_counts);
+            }) ||
+            timed(KS_FORCE, [&] {
+                hipLaunchKernelGGL(k_ens_force, dim3(igrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                                   L.d_items, L.d_counts + 1);
+            }) ||
+            timed(KS_CORRECT, [&] {
+                hipLaunchKernelGGL(k_ens_correct, dim3(cgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                                   L.d_chunks, L.d_counts, 1e-30);
+            }))
+            return -1;
+    }
+    if (sgrid) {
+        const BLists B{L.d_bchunks, L.d_bitems, L.d_bcount};
+        if (timed(KS_BSELECT, [&] {
+                hipLaunchKernelGGL(k_ens_bselect, dim3(sgrid), dim3(64), 0, L.stream, X.P,
+                                   X.d_hdr, L.d_schunks, L.d_nsel);
+            }) ||
+            timed(KS_BITEMS, [&] {
+                hipLaunchKernelGGL(k_ens_bitems, dim3(ygrid / 64), dim3(64), 0, L.stream, X.d_hdr,
+                                   L.d_bsys, L.d_nsel + 1, B);
+            }) ||
+            timed(KS_BFORCE, [&] {
+                hipLaunchKernelGGL(k_ens_bforce, dim3(std::min(kBForceWaves, L.n_bitems_max)),
+                                   dim3(64), 0, L.stream, X.P, X.d_hdr, L.d_bitems, L.d_bcount + 1);
+            }) ||
+            timed(KS_BCORRECT, [&] {
+                hipLaunchKernelGGL(k_ens_bcorrect, dim3(std::min(kBCorrectWaves, L.n_schunks)),
+                                   dim3(64), 0, L.stream, X.P, X.d_hdr, L.d_bchunks, L.d_bcount,
+                                   1e-30);
+            }))
+            return -1;
+    }
+    return timed(KS_FINALIZE, [&] {
+        hipLaunchKernelGGL(k_ens_finalize, dim3((X.max_slots + 255) / 256), dim3(256), 0, L.stream,
+                           X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30, ln, L.d_bcount);
+    });
+}
+
 // A graph of steps_per_graph engine steps of lane ln for its grid buckets.
 int get_graph(int ln, hipGraphExec_t &exec)
 {
@@ -2036,6 +2107,8 @@ void release_all()
     hfree(X.h_hstage); dfree(X.d_hstage); hfree(X.h_hslots); dfree(X.d_hslots);
     hfree(X.h_stage); dfree(X.d_stage); X.stage_cap = 0;
     hfree(X.h_out); dfree(X.d_out); hfree(X.h_src); dfree(X.d_src); X.out_cap = 0;
+    for (hipEvent_t &e : X.kev)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (X.stream) (void)hipStreamDestroy(X.stream);
     X.stream = nullptr;
     X.slots.clear();
@@ -2050,6 +2123,9 @@ int open_engine(long long hbm_budget_bytes, int max_systems)
     if (max_systems < 1 || hbm_budget_bytes <= 0)
         return efail(-1, "g6e_open: bad budget %lld / systems %d", hbm_budget_bytes, max_systems);
     ECHK(hipStreamCreate(&X.stream));
+    ECHK(hipEventCreate(&X.kev[0]));
+    ECHK(hipEventCreate(&X.kev[1]));
+    for (double &k : X.kernel_ms) k = 0;
     for (Lane &L : X.lanes) {
         ECHK(hipStreamCreate(&L.stream));
         ECHK(hipEventCreate(&L.ev0));
@@ -2203,14 +2279,21 @@ int g6e_run(int steps)
         const bool light = Lt.n_chunks > 0 || Lt.n_schunks > 0;
         const bool med[4] = {false, false, X.lanes[2].n_chunks > 0, X.lanes[3].n_chunks > 0};
         hipGraphExec_t eh = nullptr, el = nullptr;
-        if (heavy && get_graph(0, eh)) return -1;
-        if (light && !(X.fused_steps > 0 && X.light_max <= 64) && get_graph(1, el)) return -1;
+        const bool timing = X.kernel_timing != 0;
+        if (heavy && !timing && get_graph(0, eh)) return -1;
+        if (light && !timing && !(X.fused_steps > 0 && X.light_max <= 64) && get_graph(1, el))
+            return -1;
         X.host_s[3] += since(t0);
         t0 = clk::now();
         double ms_total = 0;
         if (heavy) {
             ECHK(hipEventRecord(H.ev0, H.stream));
-            ECHK(hipGraphLaunch(eh, H.stream));
+            if (timing) {
+                for (int s = 0; s < X.steps_per_graph; ++s)
+                    if (timed_step(0)) return -1;
+            } else {
+                ECHK(hipGraphLaunch(eh, H.stream));
+            }
             ECHK(hipEventRecord(H.ev1, H.stream));
         }
         for (int ln = 2; ln < 4; ++ln) {
@@ -2259,12 +2342,17 @@ int g6e_run(int steps)
                                        X.d_hdr, Lt.d_chunks, Lt.d_counts, X.d_phase, X.max_steps,
                                        1e-30, X.fused_steps);
                     ECHK(hipGetLastError());
+                } else if (timing) {
+                    for (int s = 0; s < X.steps_per_graph; ++s)
+                        if (timed_step(1)) return -1;
                 } else {
                     ECHK(hipGraphLaunch(el, Lt.stream));
                 }
                 ECHK(hipEventRecord(Lt.ev1, Lt.stream));
                 ECHK(hipStreamSynchronize(Lt.stream));
                 float ms = 0;
+                if (timing && fused && hipEventElapsedTime(&ms, Lt.ev0, Lt.ev1) == hipSuccess)
+                    X.kernel_ms[KS_FUSED] += ms;
                 if (!any_other && hipEventElapsedTime(&ms, Lt.ev0, Lt.ev1) == hipSuccess)
                     ms_total += ms;
                 Lt.steps += X.steps_per_graph;
@@ -2376,6 +2464,8 @@ int g6e_set_option(const char *key, long long value)
     else if (k == "precision") {
         if (value != 32 && value != 64) return efail(-1, "precision must be 32 or 64");
         X.precision = (int)value;
+    } else if (k == "kernel_timing") {
+        X.kernel_timing = value ? 1 : 0;
     } else if (k == "block_steps") {
         // taken by each job at submission, like the precision
         if (value != 0 && value != 1) return efail(-1, "block_steps must be 0 or 1");
@@ -2396,6 +2486,9 @@ long long g6e_get_option(const char *key)
     if (k == "precision") return X.precision;
     if (k == "block_steps") return X.block_steps;
     if (k == "block_fusion") return X.block_fusion;
+    if (k == "kernel_timing") return X.kernel_timing;
+    for (int q = 0; q < KS_N; ++q)
+        if (k == std::string("kernel_us_") + kKernelNames[q]) return (long long)(X.kernel_ms[q] * 1e3);
     if (k == "light_max") return X.light_max;
     if (k == "fused_steps") return X.fused_steps;
     if (k == "wg_steps") return X.wg_steps;

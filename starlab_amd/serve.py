@@ -15,6 +15,9 @@ import numpy as np
 from .ens import Ensemble, autotune, binary_single, plummer_cluster
 
 GB = 1 << 30
+# the engine's step kernels, as named by its kernel_us_<name> counters
+KERNELS = ("predict", "force", "correct", "bselect", "bitems", "bforce", "bcorrect", "finalize",
+           "fused")
 
 
 def cluster_job(n, rng, t_end, eta=0.1, prec=32):
@@ -86,6 +89,10 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep
                 break
         wall = time.perf_counter() - t0
         st = eng.stats()
+        kernel_us = None
+        if opt.get("kernel_timing"):
+            kernel_us = {k: eng.option("kernel_us_" + k) for k in KERNELS}
+            kernel_us = {k: v for k, v in kernel_us.items() if v}
     finally:
         eng.close()
     lat = [done_at[i] for i in ids]
@@ -96,6 +103,8 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep
                p50_s=_pct(lat, 50), p99_s=_pct(lat, 99),
                failed=sum(results[i]["failed"] for i in ids),
                de_median=_pct(de, 50), de_max=float(np.max(de)), tuned=tuned)
+    if kernel_us is not None:
+        out["kernel_us"] = kernel_us
     if keep:
         out["results"] = [results[i] for i in ids]
     return out
@@ -161,7 +170,8 @@ def main(argv=None):
     stream on the ensemble engine and print one JSON line.
 
         python -m starlab_amd.serve batch [--count 4096 --nmin 32 --nmax 2048
-                                           --t-end 0.03125 --prec 32 --tune]
+                                           --t-end 0.03125 --prec 32 --tune
+                                           --block-steps 1 --kernel-timing]
         python -m starlab_amd.serve poisson [--rate 2000 --seconds 2]
     """
     import argparse
@@ -176,6 +186,10 @@ def main(argv=None):
     b.add_argument("--prec", type=int, choices=(32, 64), default=32)
     b.add_argument("--seed", type=int, default=4)
     b.add_argument("--tune", action="store_true", help="engine shapes from ens.autotune")
+    b.add_argument("--block-steps", type=int, choices=(0, 1), default=1,
+                   help="1: kira's individual block steps; 0: one shared step per system")
+    b.add_argument("--kernel-timing", action="store_true",
+                   help="step kernels one at a time between HIP events; adds kernel_us")
     p = sub.add_parser("poisson", help="70 %% three-body scatterings, 30 %% clusters")
     p.add_argument("--rate", type=float, default=2000.0)
     p.add_argument("--seconds", type=float, default=2.0)
@@ -183,7 +197,10 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "batch":
         jobs = cluster_batch(a.count, a.nmin, a.nmax, a.seed, a.t_end, prec=a.prec)
-        r = run_batch(jobs, tune=a.tune)
+        opt = {"block_steps": a.block_steps}
+        if a.kernel_timing:
+            opt["kernel_timing"] = 1
+        r = run_batch(jobs, tune=a.tune, **opt)
     else:
         r = run_poisson(serving_mix(int(a.rate * a.seconds), a.seed), a.rate)
     print(json.dumps(r), flush=True)

@@ -243,3 +243,27 @@ def test_config4_sample_to_t1_block_fp32_fp64_and_shared():
         assert rc == 0 and t == 1.0
         np.testing.assert_array_equal(r64[k]["x"], x)
         assert r64[k]["steps"] == nb
+
+
+def test_kernel_timing_runs_the_same_steps():
+    """Option kernel_timing launches the step kernels one at a time between
+    HIP events instead of as graphs: the same results bit for bit, and a
+    device time for every kernel of the block path and the light lane."""
+    jobs = _jobs(16, [3, 40, 300, 1300], t_end=1.0 / 16)
+    a, _ = _run(jobs, precision=32)
+    e = Ensemble(2 * GB, 64, block_steps=1, precision=32, kernel_timing=1)
+    try:
+        ids = [e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]) for j in jobs]
+        res = e.run_until_idle()
+        us = {k: e.option("kernel_us_" + k) for k in ("bselect", "bitems", "bforce", "bcorrect",
+                                                       "finalize", "fused")}
+    finally:
+        e.close()
+        e.option("kernel_timing", 0)
+        for k, v in DEFAULTS.items():
+            e.option(k, v)
+    print(us, flush=True)
+    for r0, i in zip(a, ids):
+        np.testing.assert_array_equal(res[i]["x"], r0["x"])
+        assert res[i]["steps"] == r0["steps"]
+    assert all(v > 0 for v in us.values())
