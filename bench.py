@@ -405,6 +405,7 @@ def kira_leg(n=262144, t_end=0.25, samples=None):
                         "per-particle g6_set_j_particle_",
             "n": n, "t_end": t_end, "blocks": b.blocks, "particle_steps": b.particle_steps,
             "seconds": round(wall, 3), "seconds_in_abi": round(b.abi_seconds, 3),
+            "phase_seconds": {k: round(v, 3) for k, v in b.phase_seconds.items()},
             "pairs_per_s": b.particle_steps * (n - 1) / wall,
             "de_over_e": abs((e1 - e0) / e0)}, c
 

@@ -193,6 +193,14 @@ int bs_run_kira(const char *grape_lib, int n, const double *m, double *x, double
 
 const char *bs_last_error(void);
 
+/* Wall time of the last bs_run by phase, in seconds: seconds[0] the j sends
+ * (g6_set_j_particle_ or the bulk call, with the records' packing),
+ * [1] g6calc_firsthalf_, [2] g6calc_lasthalf2_ (the wait for the device
+ * included), [3] the host prediction of each block's particles, [4] the
+ * corrector, new steps and rescheduling, [5] the whole run.  Returns 0, -1
+ * for a null array. */
+int bs_timers(double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

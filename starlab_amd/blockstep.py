@@ -292,6 +292,8 @@ def _load_kb():
                                   _C.c_double, _C.c_double, _C.POINTER(_KiraOpts), _DP, _DP,
                                   _DP, _DP, _DP, _CALL_FN, _C.c_void_p]
         L.bs_last_error.restype = _C.c_char_p
+        L.bs_timers.restype = _C.c_int
+        L.bs_timers.argtypes = [_DP]
         _kb = L
     return _kb
 
@@ -360,6 +362,11 @@ class NativeBlockHermite:
         self.t[:] = t_end
         self.blocks, self.particle_steps = int(stats[0]), int(stats[1])
         self.abi_seconds, self.seconds = float(stats[2]), float(stats[3])
+        ph = np.zeros(6)
+        L.bs_timers(_dp(ph))
+        # wall seconds by phase (include/kira_blockstep.h: bs_timers)
+        self.phase_seconds = dict(zip(("send_j", "firsthalf", "lasthalf", "predict",
+                                       "correct", "total"), (float(q) for q in ph)))
         return self
 
 

@@ -24,6 +24,12 @@ namespace {
 
 std::string g_err;
 
+// Wall time of the last bs_run by phase (seconds): the j sends, firsthalf,
+// lasthalf2, the host prediction of the block, correction and scheduling,
+// and the whole run.
+enum Phase { PH_SEND, PH_FIRST, PH_LAST, PH_PREDICT, PH_CORRECT, PH_TOTAL, PH_N };
+double g_phase[PH_N];
+
 int bs_fail(int code, const char *msg, const char *detail = "")
 {
     g_err = std::string(msg) + detail;
@@ -171,9 +177,13 @@ struct Run {
             auto V = reinterpret_cast<double(*)[3]>(const_cast<double *>(vi) + 3 * (size_t)s);
             auto A = reinterpret_cast<double(*)[3]>(acc + 3 * (size_t)s);
             auto J = reinterpret_cast<double(*)[3]>(jrk + 3 * (size_t)s);
+            const double tf = now_s();
             abi.firsthalf(&cl, &nj, &nii, ix, X, V, A, J, pt + s, &e2, h2.data());
+            const double tl = now_s();
             const int rc = abi.lasthalf2(&cl, &nj, &nii, ix, X, V, &e2, h2.data(), A, J, pt + s,
                                          nn + s);
+            g_phase[PH_FIRST] += tl - tf;
+            g_phase[PH_LAST] += now_s() - tl;
             if (rc) {
                 abi_s += now_s() - t0;
                 char buf[64];
@@ -1015,6 +1025,13 @@ extern "C" {
 
 const char *bs_last_error(void) { return g_err.c_str(); }
 
+int bs_timers(double *seconds)
+{
+    if (!seconds) return bs_fail(-1, "bs_timers: null array");
+    for (int k = 0; k < PH_N; ++k) seconds[k] = g_phase[k];
+    return 0;
+}
+
 int bs_run(const char *grape_lib, int n, const double *m, double *x, double *v, double t_end,
            double eta, double eps2, double dt_max, int bulk, double *acc, double *jerk,
            double *pot, double *dt, double *stats, bs_block_fn observer, void *user)
@@ -1026,6 +1043,7 @@ int bs_run(const char *grape_lib, int n, const double *m, double *x, double *v, 
         !std::isfinite(eta) || !std::isfinite(eps2) || !std::isfinite(dt_max))
         return bs_fail(-1, "bs_run: need t_end > 0, eta > 0, eps2 >= 0, dt_max > 0 (finite)");
     const double t_start = now_s();
+    for (double &p : g_phase) p = 0;
     Run R;
     if (R.abi.load(grape_lib)) return -1;
     R.n = n;
@@ -1080,7 +1098,10 @@ int bs_run(const char *grape_lib, int n, const double *m, double *x, double *v, 
             sched.erase(sched.begin());
             std::sort(blk.begin(), blk.end());
             // the previous block's particles go to j-memory (hdyn_grape6.C:3112-3113)
+            double tp = now_s();
             if ((status = R.send_j(prev))) break;
+            g_phase[PH_SEND] += now_s() - tp;
+            tp = now_s();
             const int ni = (int)blk.size();
             xi.resize(3 * (size_t)ni); vi.resize(3 * (size_t)ni);
             a1.resize(3 * (size_t)ni); j1.resize(3 * (size_t)ni);
@@ -1095,9 +1116,11 @@ int bs_run(const char *grape_lib, int n, const double *m, double *x, double *v, 
                     vi[3 * q + c] = (R.j[o] * d2 + R.a[o]) * d + R.v[o];
                 }
             }
+            g_phase[PH_PREDICT] += now_s() - tp;
             if ((status = R.force(t_next, blk, xi.data(), vi.data(), a1.data(), j1.data(),
                                   p1.data(), nn.data())))
                 break;
+            tp = now_s();
             if (observer) {
                 // j-memory now equals the current state: every particle was last
                 // sent as it is (the previous block just now)
@@ -1117,9 +1140,11 @@ int bs_run(const char *grape_lib, int n, const double *m, double *x, double *v, 
             prev.swap(blk);
             ++blocks;
             steps += ni;
+            g_phase[PH_CORRECT] += now_s() - tp;
             if (t_next == t_end) break;
         }
     } while (false);
+    g_phase[PH_TOTAL] = now_s() - t_start;
 
     R.abi.close(&cl);
     if (status) return status;

@@ -39,6 +39,13 @@ def test_native_equals_python_driver_bitwise(shim, npipes, monkeypatch):
         np.testing.assert_array_equal(getattr(nat, name), getattr(py, name), err_msg=name)
     assert np.all(nat.t == 0.25)
     assert nat.abi_seconds > 0 and nat.seconds >= nat.abi_seconds
+    # bs_timers: every phase ran, the parts fit in the whole, the ABI ones in
+    # the ABI time
+    ph = nat.phase_seconds
+    assert all(ph[k] > 0 for k in ("send_j", "firsthalf", "lasthalf", "predict", "correct"))
+    parts = sum(ph[k] for k in ("send_j", "firsthalf", "lasthalf", "predict", "correct"))
+    assert parts <= ph["total"] and abs(ph["total"] - nat.seconds) < 0.05 * nat.seconds + 1e-3
+    assert ph["firsthalf"] + ph["lasthalf"] <= nat.abi_seconds
 
 
 def test_observer_sees_blocks_and_jmemory(shim):
