@@ -98,8 +98,12 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * per system (256 threads up to wg_small_max = 512, else 1024), up to this
  * many steps per launch, bitwise equal to the step graphs; shared-step
  * systems only; default 0 = step
- * graphs, which measured faster on mixed cluster batches), "wg_small_max",
- * "wg_max_n" (<= 2048); "light_max", "fused_steps", "wg_steps", "max_split",
+ * graphs, which measured faster on mixed cluster batches), "wgb_steps" (> 0:
+ * block-step systems of light_max < n <= wg_max_n run in a workgroup kernel,
+ * one workgroup per system (256 threads up to wg_small_max, else 512), up to
+ * this many block steps per launch, bitwise equal to the block path; default
+ * 0: faster on batches of more than a few dozen systems), "wg_small_max",
+ * "wg_max_n" (<= 2048); "light_max", "fused_steps", "wg_steps", "wgb_steps", "max_split",
  * "wg_small_max" and "wg_max_n" are fixed while open (set them before
  * g6e_open: they decide the lane each resident system steps in); read-only
  * "medium_steps", and the

@@ -939,71 +939,80 @@ __global__ __launch_bounds__(64) void k_ens_bforce(Pool P, const SysHdr *__restr
     }
 }
 
+// Block path: correct the particle at position pos of the system's i-list
+// (particle i): sum its split partials in order, recompute its prediction,
+// correct it, give it its next step.  Returns its next time (+inf: at t_end
+// or FINAL); *fail is set when its step underflows.
+__device__ __forceinline__ double bcorrect_one(const Pool &P, const SysHdr &h, int i, int pos,
+                                               double dt_min, int *fail)
+{
+#pragma clang fp contract(off)
+    const int phase = h.phase;
+    const size_t cap = P.cap;
+    const size_t p = (size_t)h.off + i;
+    const size_t q = (size_t)h.off + pos;
+    double a1[3], j1[3], pot;
+    {
+        const double *in = P.part;
+        a1[0] = in[0 * cap + q]; a1[1] = in[1 * cap + q]; a1[2] = in[2 * cap + q];
+        j1[0] = in[3 * cap + q]; j1[1] = in[4 * cap + q]; j1[2] = in[5 * cap + q];
+        pot = in[6 * cap + q];
+    }
+    for (int s = 1; s < h.nsplit; ++s) {
+        const double *in = P.part + (size_t)s * 7 * cap;
+        a1[0] += in[0 * cap + q]; a1[1] += in[1 * cap + q]; a1[2] += in[2 * cap + q];
+        j1[0] += in[3 * cap + q]; j1[1] += in[4 * cap + q]; j1[2] += in[5 * cap + q];
+        pot += in[6 * cap + q];
+    }
+    PState st;
+    double xp[3], vp[3];
+    bpredict(P, h, p, xp, vp);
+    for (int k = 0; k < 3; ++k) {
+        st.x[k] = P.x[k * cap + p]; st.v[k] = P.v[k * cap + p];
+        st.a[k] = P.a[k * cap + p]; st.j[k] = P.j[k * cap + p];
+    }
+    st.pot = 0; st.pot0 = 0;
+    double ti = P.pt[p], di = P.pdt[p];
+    const double cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, di, h.eta, h.prec);
+    double next = __builtin_inf();
+    if (phase != FINAL) {
+        next = block_advance(ti, di, cand, phase, h, dt_min, fail);
+        P.pt[p] = ti;
+        P.pdt[p] = di;
+    }
+    if (phase == INIT) {
+        for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = st.a[k]; P.j[k * cap + p] = st.j[k]; }
+        P.pot[p] = st.pot;
+        P.pot0[p] = st.pot0;
+    } else if (phase == STEP) {
+        for (int k = 0; k < 3; ++k) {
+            P.x[k * cap + p] = st.x[k];
+            P.v[k * cap + p] = st.v[k];
+            P.a[k * cap + p] = st.a[k];
+            P.j[k * cap + p] = st.j[k];
+        }
+        P.pot[p] = st.pot;
+    } else {
+        P.pot[p] = st.pot;
+    }
+    return next;
+}
+
 __global__ __launch_bounds__(64) void k_ens_bcorrect(Pool P, SysHdr *H,
                                                      const Chunk *__restrict__ chunks,
                                                      const int *__restrict__ nchunks,
                                                      double dt_min)
 {
-#pragma clang fp contract(off)
     const int total = *nchunks;
     for (int c = blockIdx.x; c < total; c += gridDim.x) {
         const Chunk ch = chunks[c];
         SysHdr &h = H[ch.slot];
-        const int phase = h.phase;
-        const bool blk = phase == STEP;
+        const bool blk = h.phase == STEP;
         const int cnt = blk ? h.nact : h.n;
         const ISel sel = select_i(P, h, blk, cnt, ch.i0, threadIdx.x);
-        const int i = sel.i;
         double next = __builtin_inf();
         int fail = 0;
-        if (i >= 0) {
-            const size_t cap = P.cap;
-            const size_t p = (size_t)h.off + i;
-            const size_t q = (size_t)h.off + sel.pos;
-            double a1[3], j1[3], pot;
-            {
-                const double *in = P.part;
-                a1[0] = in[0 * cap + q]; a1[1] = in[1 * cap + q]; a1[2] = in[2 * cap + q];
-                j1[0] = in[3 * cap + q]; j1[1] = in[4 * cap + q]; j1[2] = in[5 * cap + q];
-                pot = in[6 * cap + q];
-            }
-            for (int s = 1; s < h.nsplit; ++s) {
-                const double *in = P.part + (size_t)s * 7 * cap;
-                a1[0] += in[0 * cap + q]; a1[1] += in[1 * cap + q]; a1[2] += in[2 * cap + q];
-                j1[0] += in[3 * cap + q]; j1[1] += in[4 * cap + q]; j1[2] += in[5 * cap + q];
-                pot += in[6 * cap + q];
-            }
-            PState st;
-            double xp[3], vp[3];
-            bpredict(P, h, p, xp, vp);
-            for (int k = 0; k < 3; ++k) {
-                st.x[k] = P.x[k * cap + p]; st.v[k] = P.v[k * cap + p];
-                st.a[k] = P.a[k * cap + p]; st.j[k] = P.j[k * cap + p];
-            }
-            st.pot = 0; st.pot0 = 0;
-            double ti = P.pt[p], di = P.pdt[p];
-            const double cand = ens_correct_one(st, xp, vp, a1, j1, pot, phase, di, h.eta, h.prec);
-            if (phase != FINAL) {
-                next = block_advance(ti, di, cand, phase, h, dt_min, &fail);
-                P.pt[p] = ti;
-                P.pdt[p] = di;
-            }
-            if (phase == INIT) {
-                for (int k = 0; k < 3; ++k) { P.a[k * cap + p] = st.a[k]; P.j[k * cap + p] = st.j[k]; }
-                P.pot[p] = st.pot;
-                P.pot0[p] = st.pot0;
-            } else if (phase == STEP) {
-                for (int k = 0; k < 3; ++k) {
-                    P.x[k * cap + p] = st.x[k];
-                    P.v[k * cap + p] = st.v[k];
-                    P.a[k * cap + p] = st.a[k];
-                    P.j[k * cap + p] = st.j[k];
-                }
-                P.pot[p] = st.pot;
-            } else {
-                P.pot[p] = st.pot;
-            }
-        }
+        if (sel.i >= 0) next = bcorrect_one(P, h, sel.i, sel.pos, dt_min, &fail);
         wave_atomic_min(&h.tmin, next);
         if (__ballot(fail) && threadIdx.x == 0) atomicOr(&h.fail, 1);
     }
@@ -1315,6 +1324,212 @@ __global__ __launch_bounds__(NT) void k_ens_wg(Pool P, SysHdr *__restrict__ H,
 }
 
 // ---------------------------------------------------------------------------
+// Workgroup kernel for block-step systems (option wgb_steps > 0): one
+// workgroup of NT threads (256 up to wg_small_max particles, else 512) owns
+// one system of light_max < n <= wg_max_n and
+// runs up to max_iters block steps per launch:
+//   select   the block (t + dt = the block time) into an LDS list, and the
+//            smallest next time of the other particles;
+//   predict  every j to the block time into LDS (FP64 records, or the ten
+//            FP32 fields as separate arrays);
+//   force    the waves take the fused block path's work items (64-i chunk,
+//            j-split) in turn and compute each as k_ens_bforce does (same
+//            lane packing, pair order, tile folds and butterfly), writing the
+//            split partials to HBM;
+//   correct  bcorrect_one per block particle, a workgroup minimum of the next
+//            times, and the header update by thread 0.
+// Each step performs the fused block path's operations on the same values,
+// so a trajectory is bitwise that of the step graphs.  The step costs no
+// kernel launch and no pass over other systems: a block of a few particles
+// costs a prediction of the system's j and a few thousand pairs.  Dynamic
+// LDS: the records of nmax j, then nmax ints for the block list.
+// ---------------------------------------------------------------------------
+// threads per system of the large block-step lane: 512 leave 256 VGPRs per
+// thread (1024 threads allow 128, and the kernel spilled there)
+constexpr int kWgbLargeThreads = 512;
+
+size_t ens_wgb_lds(int nmax, int prec)
+{
+    const size_t rec = prec == 64 ? sizeof(JPred64) : 10 * sizeof(float);
+    return (size_t)nmax * (rec + sizeof(int));
+}
+
+template <int NT, int PREC>
+__global__ __launch_bounds__(NT) void k_ens_wgb(Pool P, SysHdr *__restrict__ H,
+                                                const Chunk *__restrict__ list, int nlist,
+                                                int *__restrict__ phase_out, long long max_steps,
+                                                double dt_min, int max_iters, int nmax)
+{
+#pragma clang fp contract(off)
+    constexpr int NW = NT / 64;
+    extern __shared__ double wgb_lds[];
+    __shared__ SysHdr sh;
+    __shared__ int s_cnt, s_fail;
+    __shared__ double s_min[NW];
+    const int w = blockIdx.x;
+    if (w >= nlist) return;                        // uniform per workgroup
+    const int slot = list[w].slot;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) sh = H[slot];
+    __syncthreads();
+    if (sh.mode != 1 || sh.phase < INIT || sh.phase > FINAL || sh.n > nmax || sh.prec != PREC)
+        return;                                    // uniform
+    const int n = sh.n;
+    const size_t off = (size_t)sh.off;
+    JPred64 *r64 = reinterpret_cast<JPred64 *>(wgb_lds);
+    float *f32 = reinterpret_cast<float *>(wgb_lds);   // [10][nmax]
+    int *blist = reinterpret_cast<int *>(reinterpret_cast<char *>(wgb_lds) +
+                                         (size_t)nmax * (PREC == 64 ? sizeof(JPred64) : 40));
+    auto rec32 = [&](int j) {
+        JPred r;
+        r.xh = f32[0 * nmax + j]; r.yh = f32[1 * nmax + j]; r.zh = f32[2 * nmax + j];
+        r.xl = f32[3 * nmax + j]; r.yl = f32[4 * nmax + j]; r.zl = f32[5 * nmax + j];
+        r.vx = f32[6 * nmax + j]; r.vy = f32[7 * nmax + j]; r.vz = f32[8 * nmax + j];
+        r.m = f32[9 * nmax + j];
+        return r;
+    };
+    for (int it = 0; it < max_iters; ++it) {
+        const int phase = sh.phase;
+        const bool blk = phase == STEP;
+        if (tid == 0) { s_cnt = 0; s_fail = 0; }
+        __syncthreads();
+        // ---- the block, and the others' next times ----
+        double nmin = __builtin_inf();
+        if (blk) {
+            for (int i0 = 0; i0 < n; i0 += NT) {
+                const int i = i0 + tid;
+                const bool valid = i < n;
+                const size_t p = off + (valid ? i : 0);
+                const double ti = P.pt[p], di = P.pdt[p];
+                const bool in = valid && ti < sh.t_end && ti + di == sh.tnext;
+                const unsigned long long bal = __ballot(in);
+                int base = 0;
+                if (lane == 0 && bal) base = atomicAdd(&s_cnt, __popcll(bal));
+                base = __shfl(base, 0);
+                if (in) blist[base + __popcll(bal & ((1ull << lane) - 1))] = i;
+                if (valid && !in && ti < sh.t_end) nmin = fmin(nmin, ti + di);
+            }
+        }
+        // ---- every j predicted to the evaluation time ----
+        for (int j = tid; j < n; j += NT) {
+            double xp[3], vp[3];
+            bpredict(P, sh, off + j, xp, vp);
+            if (PREC == 64) {
+                r64[j] = make_jpred64(xp, vp, P.m[off + j]);
+            } else {
+                const JPred r = make_jpred(xp, vp, P.m[off + j]);
+                f32[0 * nmax + j] = r.xh; f32[1 * nmax + j] = r.yh; f32[2 * nmax + j] = r.zh;
+                f32[3 * nmax + j] = r.xl; f32[4 * nmax + j] = r.yl; f32[5 * nmax + j] = r.zl;
+                f32[6 * nmax + j] = r.vx; f32[7 * nmax + j] = r.vy; f32[8 * nmax + j] = r.vz;
+                f32[9 * nmax + j] = r.m;
+            }
+        }
+        __syncthreads();
+        const int cnt = blk ? s_cnt : n;
+        // ---- force: the fused block path's items, one wave each ----
+        {
+            const int nc = (cnt + 63) / 64, ns = sh.nsplit;
+            int nip = 64;
+            if (PREC != 64 && cnt <= 32) {
+                nip = 1;
+                while (nip < cnt) nip *= 2;
+            }
+            const int Q = 64 / nip;
+            const int islot = lane / Q, q = lane - islot * Q;
+            for (int item = wave; item < nc * ns; item += NW) {
+                const int c = item / ns, sp = item - c * ns;
+                const int pos = 64 * c + islot;
+                const int self = pos < cnt ? (blk ? blist[pos] : pos) : -1;
+                const int ic = self >= 0 ? self : 0;
+                const int jbeg = sp * sh.jchunk;
+                const int jend = min(jbeg + sh.jchunk, n);
+                double A[7] = {0, 0, 0, 0, 0, 0, 0};
+                if (PREC == 64) {
+                    const JPred64 I = r64[ic];
+                    DAcc a;
+                    a.ax = a.ay = a.az = a.jx = a.jy = a.jz = a.p = 0.0;
+                    for (int t = jbeg; t < jend; t += 64) {
+                        const int tn = min(64, jend - t);
+                        const unsigned rel = (unsigned)(self - t);
+                        if (__ballot(rel < (unsigned)tn) == 0) {
+                            for (int jj = 0; jj < tn; ++jj)
+                                ens_pair64<false>(r64[t + jj], t + jj, self, I, sh.eps2, a);
+                        } else {
+                            for (int jj = 0; jj < tn; ++jj)
+                                ens_pair64<true>(r64[t + jj], t + jj, self, I, sh.eps2, a);
+                        }
+                    }
+                    A[0] = a.ax; A[1] = a.ay; A[2] = a.az;
+                    A[3] = a.jx; A[4] = a.jy; A[5] = a.jz; A[6] = a.p;
+                } else {
+                    const JPred I = rec32(ic);
+                    const float eps2 = (float)sh.eps2;
+                    for (int t = jbeg; t < jend; t += kTile) {
+                        const int tn = min(kTile, jend - t);
+                        EAcc e;
+                        e.ax = e.ay = e.az = e.jx = e.jy = e.jz = e.kx = e.ky = e.kz = e.p = 0.f;
+                        for (int s0 = 0; s0 < tn; s0 += 64) {
+                            const int sn = min(64, tn - s0);
+                            const int b0 = t + s0;
+                            const unsigned rel = (unsigned)(self - b0);
+                            if (__ballot(rel < (unsigned)sn) == 0) {
+                                for (int jj = q; jj < sn; jj += Q)
+                                    ens_pair<false>(rec32(b0 + jj), jj, rel, I, eps2, e);
+                            } else {
+                                for (int jj = q; jj < sn; jj += Q)
+                                    ens_pair<true>(rec32(b0 + jj), jj, rel, I, eps2, e);
+                            }
+                        }
+                        A[0] += (double)e.ax;
+                        A[1] += (double)e.ay;
+                        A[2] += (double)e.az;
+                        A[3] += (double)e.jx - 3.0 * (double)e.kx;
+                        A[4] += (double)e.jy - 3.0 * (double)e.ky;
+                        A[5] += (double)e.jz - 3.0 * (double)e.kz;
+                        A[6] += (double)e.p;
+                    }
+                    for (int o = 1; o < Q; o <<= 1) {
+#pragma unroll
+                        for (int k = 0; k < 7; ++k) A[k] += __shfl_xor(A[k], o);
+                    }
+                }
+                if (q == 0 && self >= 0) {
+                    const size_t cap = P.cap;
+                    double *out = P.part + (size_t)sp * 7 * cap;
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) out[k * cap + off + pos] = A[k];
+                }
+            }
+        }
+        __syncthreads();                           // the partials are written
+        // ---- correct the block, next block time ----
+        int fail = 0;
+        for (int pos = tid; pos < cnt; pos += NT) {
+            const int i = blk ? blist[pos] : pos;
+            nmin = fmin(nmin, bcorrect_one(P, sh, i, pos, dt_min, &fail));
+        }
+        for (int o = 32; o > 0; o >>= 1) nmin = fmin(nmin, __shfl_xor(nmin, o));
+        if (lane == 0) s_min[wave] = nmin;
+        if (__ballot(fail) && lane == 0) atomicOr(&s_fail, 1);
+        __syncthreads();
+        if (tid == 0) {
+            double m = s_min[0];
+            for (int k = 1; k < NW; ++k) m = fmin(m, s_min[k]);
+            ens_finalize_block(sh, m, blk ? cnt : 0, s_fail, max_steps);
+        }
+        __syncthreads();                           // header visible; LDS free for the next step
+        if (sh.phase == DONE || sh.phase == FAILED) break;
+    }
+    if (tid == 0) {
+        sh.pmin = sh.tmin = kInfBits;
+        sh.nact = 0;
+        sh.fail = 0;
+        H[slot] = sh;
+        phase_out[slot] = sh.phase;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // admission and retirement
 // staged particle record: m, x[3], v[3], dest index (as double)
 __global__ void k_ens_unpack(Pool P, const double *__restrict__ rec, int n)
@@ -1429,7 +1644,9 @@ struct Lane {
     std::vector<GraphExec> graphs;
     long long steps = 0;
     int nmax = 0;                 // largest system of a workgroup-fused lane
-    int n32 = 0;                  // its systems of precision 32 (listed first)
+    // its systems by group, listed in this order: shared steps FP32, shared
+    // FP64, block steps FP32, block FP64 (one launch per group)
+    int ngrp[4] = {0, 0, 0, 0};
     // block-step systems on the fused block path: every 64-particle chunk (for
     // the select kernel); the step's chunk and item lists are built on the
     // device (d_bchunks, d_bitems, counters d_bcount)
@@ -1482,6 +1699,8 @@ struct Engine {
                                       // measured slower than the graphs on the 4096-cluster batch)
     int wg_small_max = 512;           // lane 2: light_max < n <= this, 256 threads per system
     int wg_max_n = 2048;              // lane 3: wg_small_max < n <= this, 1024 threads per system
+    int wgb_steps = 0;                // block-step systems in the workgroup lanes: block steps
+                                      // per launch of k_ens_wgb (0: the fused block path)
     double dt_max = 0.125;
     long long max_steps = 50000000;
     // stats
@@ -1533,12 +1752,15 @@ size_t bytes_per_particle(int max_split)
            sizeof(double) * 7 * (size_t)max_split;
 }
 
-// The workgroup-fused lanes step shared-step systems only.
+// Medium systems go to the workgroup lanes when their step mode's kernel is
+// on: k_ens_wg (shared steps, wg_steps > 0) or k_ens_wgb (block steps,
+// wgb_steps > 0).
 int lane_for(int n, int mode)
 {
     if (n <= X.light_max) return 1;
-    if (mode == 0 && X.wg_steps > 0 && n <= X.wg_small_max) return 2;
-    if (mode == 0 && X.wg_steps > 0 && n <= X.wg_max_n) return 3;
+    const bool wg = mode == 0 ? X.wg_steps > 0 : X.wgb_steps > 0;
+    if (wg && n <= X.wg_small_max) return 2;
+    if (wg && n <= X.wg_max_n) return 3;
     return 0;
 }
 
@@ -1850,19 +2072,20 @@ int rebuild_items()
         std::vector<int> sys;
         for (int s = 0; s < X.max_slots; ++s)
             if (X.slots[s].busy && X.slots[s].lane == ln) sys.push_back(s);
-        // precision 32 first (one kernel per precision), largest first within each
-        std::stable_sort(sys.begin(), sys.end(), [](int a, int b) {
-            const SlotInfo &p = X.slots[a], &q = X.slots[b];
-            return p.prec != q.prec ? p.prec < q.prec : p.n > q.n;
+        // by group (step mode, then precision: one kernel per group), largest
+        // first within each
+        auto grp = [](int s) { return 2 * X.slots[s].mode + (X.slots[s].prec == 64 ? 1 : 0); };
+        std::stable_sort(sys.begin(), sys.end(), [&](int a, int b) {
+            return grp(a) != grp(b) ? grp(a) < grp(b) : X.slots[a].n > X.slots[b].n;
         });
         const int nc = (int)sys.size();
         if (ensure_items(L, std::max(nc, 1), 1)) return -1;
         L.nmax = 0;
-        L.n32 = 0;
+        for (int &g : L.ngrp) g = 0;
         for (int q = 0; q < nc; ++q) {
             L.h_chunks[q] = Chunk{sys[q], 0};
             L.nmax = std::max(L.nmax, X.slots[sys[q]].n);
-            if (X.slots[sys[q]].prec == 32) ++L.n32;
+            ++L.ngrp[grp(sys[q])];
         }
         L.n_chunks = nc;
         L.n_items = 0;
@@ -2150,6 +2373,19 @@ int open_engine(long long hbm_budget_bytes, int max_systems)
                                  hipFuncAttributeMaxDynamicSharedMemorySize, small));
         ECHK(hipFuncSetAttribute((const void *)k_ens_wg<256, 64>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, small));
+        const int nbig = std::max(X.wg_max_n, 1), nsmall = std::max(X.wg_small_max, 1);
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wgb<kWgbLargeThreads, 32>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ens_wgb_lds(nbig, 32)));
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wgb<kWgbLargeThreads, 64>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ens_wgb_lds(nbig, 64)));
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wgb<256, 32>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ens_wgb_lds(nsmall, 32)));
+        ECHK(hipFuncSetAttribute((const void *)k_ens_wgb<256, 64>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ens_wgb_lds(nsmall, 64)));
     }
     X.P.max_split = X.max_split;
     const size_t bpp = bytes_per_particle(X.max_split);
@@ -2300,28 +2536,32 @@ int g6e_run(int steps)
             if (!med[ln]) continue;
             Lane &M = X.lanes[ln];
             ECHK(hipEventRecord(M.ev0, M.stream));
-            const int n32 = M.n32, n64 = M.n_chunks - M.n32;
-            const Chunk *l32 = M.d_chunks, *l64 = M.d_chunks + n32;
-            const int nt = ln == 2 ? 256 : kWgLargeThreads;
-            const size_t lds = ens_wg_lds(M.nmax, nt);
-            if (ln == 2) {
-                if (n32) hipLaunchKernelGGL((k_ens_wg<256, 32>), dim3(n32), dim3(256), lds, M.stream,
-                                            X.P, X.d_hdr, l32, n32, X.d_phase, X.max_steps, 1e-30,
-                                            X.wg_steps, M.nmax);
-                if (n64) hipLaunchKernelGGL((k_ens_wg<256, 64>), dim3(n64), dim3(256), lds, M.stream,
-                                            X.P, X.d_hdr, l64, n64, X.d_phase, X.max_steps, 1e-30,
-                                            X.wg_steps, M.nmax);
-            } else {
-                if (n32) hipLaunchKernelGGL((k_ens_wg<kWgLargeThreads, 32>), dim3(n32),
-                                            dim3(kWgLargeThreads), lds, M.stream, X.P, X.d_hdr,
-                                            l32, n32, X.d_phase, X.max_steps, 1e-30, X.wg_steps,
-                                            M.nmax);
-                if (n64) hipLaunchKernelGGL((k_ens_wg<kWgLargeThreads, 64>), dim3(n64),
-                                            dim3(kWgLargeThreads), lds, M.stream, X.P, X.d_hdr,
-                                            l64, n64, X.d_phase, X.max_steps, 1e-30, X.wg_steps,
-                                            M.nmax);
+            const size_t lds_wg = ens_wg_lds(M.nmax, ln == 2 ? 256 : kWgLargeThreads);
+            const Chunk *lst = M.d_chunks;
+            for (int g = 0; g < 4; ++g) {
+                const int cnt = M.ngrp[g];
+                if (cnt == 0) continue;
+                const bool block = g >= 2, f64 = g & 1;
+                const size_t lds = block ? ens_wgb_lds(M.nmax, f64 ? 64 : 32) : lds_wg;
+                const int iters = block ? X.wgb_steps : X.wg_steps;
+#define G6_WG_LAUNCH(KERNEL, NT)                                                               \
+    hipLaunchKernelGGL(KERNEL, dim3(cnt), dim3(NT), lds, M.stream, X.P, X.d_hdr, lst, cnt,      \
+                       X.d_phase, X.max_steps, 1e-30, iters, M.nmax)
+                if (ln == 2) {
+                    if (!block && !f64) G6_WG_LAUNCH((k_ens_wg<256, 32>), 256);
+                    if (!block && f64) G6_WG_LAUNCH((k_ens_wg<256, 64>), 256);
+                    if (block && !f64) G6_WG_LAUNCH((k_ens_wgb<256, 32>), 256);
+                    if (block && f64) G6_WG_LAUNCH((k_ens_wgb<256, 64>), 256);
+                } else {
+                    if (!block && !f64) G6_WG_LAUNCH((k_ens_wg<kWgLargeThreads, 32>), kWgLargeThreads);
+                    if (!block && f64) G6_WG_LAUNCH((k_ens_wg<kWgLargeThreads, 64>), kWgLargeThreads);
+                    if (block && !f64) G6_WG_LAUNCH((k_ens_wgb<kWgbLargeThreads, 32>), kWgbLargeThreads);
+                    if (block && f64) G6_WG_LAUNCH((k_ens_wgb<kWgbLargeThreads, 64>), kWgbLargeThreads);
+                }
+#undef G6_WG_LAUNCH
+                ECHK(hipGetLastError());
+                lst += cnt;
             }
-            ECHK(hipGetLastError());
             ECHK(hipEventRecord(M.ev1, M.stream));
         }
         auto others_running = [&]() {
@@ -2372,7 +2612,7 @@ int g6e_run(int steps)
             ECHK(hipStreamSynchronize(M.stream));
             float ms = 0;
             if (hipEventElapsedTime(&ms, M.ev0, M.ev1) == hipSuccess) ms_other = std::max(ms_other, (double)ms);
-            M.steps += X.wg_steps;
+            M.steps += std::max(X.wg_steps, X.wgb_steps);
         }
         ms_total += ms_other;
         X.host_s[4] += since(t0);
@@ -2384,7 +2624,7 @@ int g6e_run(int steps)
         X.dev_seconds += ms_total * 1e-3;
         // every round counts at least one step, so g6e_run(steps) ends
         const int round_steps = std::max(1, heavy ? X.steps_per_graph
-                                            : (med[2] || med[3]) ? X.wg_steps
+                                            : (med[2] || med[3]) ? std::max(X.wg_steps, X.wgb_steps)
                                                                  : X.steps_per_graph);
         X.engine_steps += round_steps;
         done += round_steps;
@@ -2446,14 +2686,15 @@ int g6e_set_option(const char *key, long long value)
         if (X.open) return efail(-1, "%s is fixed while open", k.c_str());
         X.block_fusion = value ? 1 : 0;
     }
-    else if (k == "light_max" || k == "fused_steps" || k == "wg_steps") {
+    else if (k == "light_max" || k == "fused_steps" || k == "wg_steps" || k == "wgb_steps") {
         // they decide which lane and kernel each resident system steps in:
         // changed under resident systems, a lane could stop advancing
         if (X.open) return efail(-1, "%s is fixed while open", k.c_str());
         const int v = (int)std::max(0LL, value);
         if (k == "light_max") X.light_max = v;
         else if (k == "fused_steps") X.fused_steps = v;
-        else X.wg_steps = v;
+        else if (k == "wg_steps") X.wg_steps = v;
+        else X.wgb_steps = v;
     }
     else if (k == "wg_small_max" || k == "wg_max_n") {
         // records of 64 B per particle must fit the 160 KB of LDS
@@ -2492,6 +2733,7 @@ long long g6e_get_option(const char *key)
     if (k == "light_max") return X.light_max;
     if (k == "fused_steps") return X.fused_steps;
     if (k == "wg_steps") return X.wg_steps;
+    if (k == "wgb_steps") return X.wgb_steps;
     if (k == "wg_small_max") return X.wg_small_max;
     if (k == "wg_max_n") return X.wg_max_n;
     if (k == "medium_steps") return X.lanes[2].steps + X.lanes[3].steps;

@@ -125,10 +125,11 @@ class Ensemble:
 
 # Engine options the tuner chooses among: j per split within a system (the
 # split-j shape of the step kernels), engine steps per graph replay, and
-# whether medium systems run in the workgroup-fused kernel (shared-step
-# systems only: block-step systems always take the step graphs).
+# whether medium systems run in a workgroup kernel, one workgroup per system
+# (k_ens_wg for shared-step jobs, k_ens_wgb for block-step jobs).
 TUNE_SPACE = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32), wg_steps=(0, 32))
-TUNE_SPACE_BLOCK = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32))
+TUNE_SPACE_BLOCK = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32),
+                        wgb_steps=(0, 64))
 
 
 def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192,

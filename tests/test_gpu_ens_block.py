@@ -35,7 +35,8 @@ pytestmark = pytest.mark.gpu
 
 GB = 1 << 30
 DEFAULTS = dict(jsplit_target=512, precision=64, max_steps=50_000_000, steps_per_graph=8,
-                fused_steps=256, light_max=64, wg_steps=0, block_steps=1, block_fusion=1)
+                fused_steps=256, light_max=64, wg_steps=0, block_steps=1, block_fusion=1,
+                wgb_steps=0)
 PATHS = pytest.mark.parametrize("fused", [256, 0], ids=["fused", "graphs"])
 
 
@@ -267,3 +268,35 @@ def test_kernel_timing_runs_the_same_steps():
         np.testing.assert_array_equal(res[i]["x"], r0["x"])
         assert res[i]["steps"] == r0["steps"]
     assert all(v > 0 for v in us.values())
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("jsplit", [0, 256])
+def test_workgroup_block_kernel_equals_block_path_bitwise(prec, jsplit):
+    """k_ens_wgb (one workgroup per system of 64 < n <= 2048, block steps
+    inside the launch) performs the fused block path's operations on the same
+    values: trajectories, block counts and energies bit for bit, in both
+    thread counts (256 up to 512 bodies, 512 above), beside a system too
+    large for it (block path) and light ones (wave-fused kernel)."""
+    jobs = _jobs(17, [65, 100, 300, 512, 513, 900, 2048, 2100, 3, 40], t_end=1.0 / 16)
+    wg, sw = _run(jobs, precision=prec, jsplit_target=jsplit, wgb_steps=32)
+    bp, sb = _run(jobs, precision=prec, jsplit_target=jsplit, wgb_steps=0)
+    for j, f, g in zip(jobs, wg, bp):
+        assert not f["failed"] and f["t"] == g["t"] == 1.0 / 16, len(j["m"])
+        assert f["steps"] == g["steps"], len(j["m"])
+        np.testing.assert_array_equal(f["x"], g["x"])
+        np.testing.assert_array_equal(f["v"], g["v"])
+        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
+    assert sw["pairs"] == sb["pairs"]
+
+
+def test_workgroup_block_kernel_short_launches_resume_bitwise():
+    """One block step per launch, resumed from the header and the pool,
+    gives the trajectory of 64 steps per launch."""
+    jobs = _jobs(18, [130, 700, 1500], t_end=1.0 / 32)
+    a, _ = _run(jobs, precision=32, wgb_steps=1)
+    b, _ = _run(jobs, precision=32, wgb_steps=64)
+    for f, g in zip(a, b):
+        assert f["steps"] == g["steps"]
+        np.testing.assert_array_equal(f["x"], g["x"])
+        assert f["e1"] == g["e1"]
