@@ -300,3 +300,30 @@ def test_workgroup_block_kernel_short_launches_resume_bitwise():
         assert f["steps"] == g["steps"]
         np.testing.assert_array_equal(f["x"], g["x"])
         assert f["e1"] == g["e1"]
+
+
+@PATHS
+def test_block_edge_cases_bitwise_equal_oracle(fused):
+    """Block steps at the edges, FP64 against the block-step oracle bit for
+    bit: an end time that is no multiple of the steps (each particle's last
+    step is cut to end there), one body alone, two bodies, and a light
+    system beside a medium one."""
+    rng = np.random.default_rng(19)
+    jobs = []
+    m, x, v = plummer_cluster(90, rng)
+    jobs.append(dict(m=m, x=x, v=v, t_end=0.3, eta=0.05, eps2=(4.0 / 90) ** 2))
+    jobs.append(dict(m=np.array([1.0]), x=np.array([[0.1, 0.2, 0.3]]), v=np.array([[0.0, 0.1, 0.0]]),
+                     t_end=0.25, eta=0.05, eps2=0.0))
+    jobs.append(dict(m=np.array([0.7, 0.3]), x=np.array([[-0.3, 0, 0], [0.7, 0, 0]]),
+                     v=np.array([[0, -0.25, 0], [0, 0.6, 0]]), t_end=1.7, eta=0.02, eps2=0.0))
+    m, x, v = plummer_cluster(20, rng)
+    jobs.append(dict(m=m, x=x, v=v, t_end=0.3, eta=0.05, eps2=(4.0 / 20) ** 2))
+    res, _ = _run(jobs, jsplit_target=0, fused_steps=fused)
+    for j, r in zip(jobs, res):
+        xf, vf, t, nb, ps, e0, e1, rc = oracle.hermite_block(j["m"], j["x"], j["v"], j["t_end"],
+                                                            j["eta"], j["eps2"])
+        assert rc == 0 and not r["failed"], len(j["m"])
+        assert r["t"] == t == j["t_end"] and r["steps"] == nb, len(j["m"])
+        np.testing.assert_array_equal(r["x"], xf)
+        np.testing.assert_array_equal(r["v"], vf)
+        assert r["e1"] == e1
