@@ -45,6 +45,7 @@ struct Options {
     int wpg = 0;              // waves per force workgroup (j-splits folded in LDS): 0 auto, 1..16
     int fused_small = 1;      // ABI blocks of one i-group predict j inside the force kernel
     int small_fp64 = 1;       // ABI blocks of at most kSmallMaxI i: one FP64 launch (k_small)
+    int small_flag = 1;       // lasthalf of such a block polls k_small's completion word
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every
                               // size so far, so off unless asked for)
@@ -85,6 +86,14 @@ struct Engine {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev_done = nullptr;
+    // completion word of the one-launch small-block kernel: its last
+    // workgroup stores the block's sequence number here (pinned, coherent)
+    // after the results, so lasthalf can poll it instead of waiting for
+    // ev_done's completion signal
+    unsigned long long *h_flag = nullptr;
+    unsigned long long *h_flag_dev = nullptr;
+    unsigned long long flag_seq = 0;
+    bool flag_pending = false;
     std::string last_error;
 
     // ---- j-memory ---------------------------------------------------------

@@ -13,6 +13,7 @@
 #include "g6_core.h"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -74,6 +75,7 @@ void read_env()
     o.wpg = (int)env_ll("WPG", o.wpg);
     o.fused_small = (int)env_ll("FUSED_SMALL", o.fused_small);
     o.small_fp64 = (int)env_ll("SMALL_FP64", o.small_fp64);
+    o.small_flag = (int)env_ll("SMALL_FLAG", o.small_flag);
     o.fail_lasthalf = (int)env_ll("FAIL_LASTHALF", o.fail_lasthalf);
     o.fail_read_nl = (int)env_ll("FAIL_READNL", o.fail_read_nl);
     o.fail_nn = (int)env_ll("FAIL_NN", o.fail_nn);
@@ -596,6 +598,11 @@ int do_open(int dev)
         if (*tf && load_tuning(tf) < 0) E.last_error.clear();   // a missing file is not an error
     }
     HIPCHK(hipEventCreateWithFlags(&E.ev_done, hipEventDisableTiming));
+    HIPCHK(hipHostMalloc((void **)&E.h_flag, 64, hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void **)&E.h_flag_dev, E.h_flag, 0));
+    __atomic_store_n(E.h_flag, 0ull, __ATOMIC_RELEASE);
+    E.flag_seq = 0;
+    E.flag_pending = false;
     E.device = dev;
     E.open = true;
     E.pred_valid = false;
@@ -634,6 +641,9 @@ int do_close()
     E.ev_pool.clear();
     if (E.ev_done) (void)hipEventDestroy(E.ev_done);
     E.ev_done = nullptr;
+    hfree(E.h_flag);
+    E.h_flag_dev = nullptr;
+    E.flag_pending = false;
     if (E.stream) (void)hipStreamDestroy(E.stream);
     E.stream = nullptr;
     E.open = false;
@@ -821,10 +831,17 @@ int calc_small(int nj, int ni, const double (*xi)[3], const double (*vi)[3], con
     }
     const Results r = results_at(E.h_res_dev, ni);
     KTimer t(K_FORCE);
+    // with kernel timing on, lasthalf waits for the event (it reads the timers)
+    const bool flag = E.h_flag_dev && E.opt.small_flag && !E.opt.timing;
     const hipError_t e = launch_small(E.stream, B, E.d_jraw, E.d_jindex, E.d_xp64, E.d_vp64,
                                       E.dcap, E.d_small,
-                                      reinterpret_cast<unsigned *>(E.d_small + npart), r);
+                                      reinterpret_cast<unsigned *>(E.d_small + npart), r,
+                                      flag ? E.h_flag_dev : nullptr, E.flag_seq + 1);
     t.stop();
+    if (e == hipSuccess && flag) {
+        E.flag_seq += 1;
+        E.flag_pending = true;
+    }
     if (e != hipSuccess) {
         // updates carried in the arguments never reached HBM: send them again
         for (int k = 0; k < B.nd; ++k) {
@@ -862,6 +879,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     E.calc_failed = true;
     E.sweep_results = false;
     E.nb_valid = false;
+    E.flag_pending = false;
     if (ensure_open(who)) return -1;
     if (ni <= 0) return fail(-1, "%s: ni = %d", who, ni);
     if (nj <= 0) return fail(-1, "%s: nj = %d", who, nj);
@@ -923,7 +941,9 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         }
     }
     E.res_seq = mark_queued();
-    HIPCHK(hipEventRecord(E.ev_done, E.stream));
+    // a block whose completion word lasthalf polls needs no event: if the
+    // poll gives up, lasthalf waits for the stream instead
+    if (!E.flag_pending) HIPCHK(hipEventRecord(E.ev_done, E.stream));
     E.done_seq = E.op_seq;
     E.last_ni = ni;
     E.last_nj = nj;
@@ -931,6 +951,19 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     E.calc_pending = true;
     E.calc_failed = false;
     return 0;
+}
+
+// Poll the small-block kernel's completion word for seq, at most ~2 ms
+// (a slower block, or a failed one, then waits for the event instead).
+bool poll_done(unsigned long long seq)
+{
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    for (unsigned k = 0;; ++k) {
+        if (__atomic_load_n(E.h_flag, __ATOMIC_ACQUIRE) == seq) return true;
+        if ((k & 255) == 255 && clk::now() - t0 > std::chrono::milliseconds(2)) return false;
+        __builtin_ia32_pause();
+    }
 }
 
 const Results *calc_wait(const char *who)
@@ -942,6 +975,29 @@ const Results *calc_wait(const char *who)
         return nullptr;
     }
     E.calc_pending = false;
+    if (E.flag_pending) {
+        E.flag_pending = false;
+        if (poll_done(E.flag_seq)) {
+            // the results are in host memory, and every host buffer the block
+            // used (it reads its j updates and i-block from its arguments) is
+            // free; the kernel's own end is ordered before the next launch
+            E.synced_seq = std::max(E.synced_seq, E.done_seq);
+            h = results_at(E.h_res, E.last_ni);
+            E.last_refinements = count_refined(h, E.last_ni);
+            return &h;
+        }
+        // no event was recorded for this block: the stream holds nothing after it
+        const hipError_t e = hipStreamSynchronize(E.stream);
+        if (e != hipSuccess) {
+            E.calc_failed = true;
+            fail(-1, "%s: %s", who, hipGetErrorString(e));
+            return nullptr;
+        }
+        E.synced_seq = std::max(E.synced_seq, E.done_seq);
+        h = results_at(E.h_res, E.last_ni);
+        E.last_refinements = count_refined(h, E.last_ni);
+        return &h;
+    }
     hipError_t e = hipEventSynchronize(E.ev_done);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
@@ -981,7 +1037,9 @@ int read_nl()
     if (ensure_open("read_neighbour_list")) return -1;
     if (E.last_ni <= 0 || E.calc_failed) return fail(-1, "read_neighbour_list: no calculation");
     if (E.calc_pending) {
-        HIPCHK(hipEventSynchronize(E.ev_done));
+        // a block with a completion word recorded no event
+        if (E.flag_pending) HIPCHK(hipStreamSynchronize(E.stream));
+        else HIPCHK(hipEventSynchronize(E.ev_done));
     }
     const int ni = E.last_ni;
     int nslots = 0;

@@ -168,9 +168,12 @@ static_assert(sizeof(SmallBlock) <= 3712, "kernel argument block");
 // workgroups and 64-j chunks per wave for nj j-particles
 void small_grid(int nj, int &wgs, int &chunks);
 size_t small_part_doubles();   // partial-sum buffer of one launch
+// done_flag (pinned host word, may be null): the last workgroup stores seq
+// there after the results, with system-scope fences in between.
 hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jindex,
                         double *xp64, double *vp64, int cap, double *part, unsigned *ticket,
-                        const Results &r);
+                        const Results &r, unsigned long long *done_flag = nullptr,
+                        unsigned long long seq = 0);
 
 // Potential only: pot[i] = -sum_j m_j / r_ij for the i-block (self excluded
 // by address as in the force kernel); part holds nsplit * ib.ld doubles.

@@ -1158,7 +1158,8 @@ template <int NIP>
 __global__ __launch_bounds__(kSmallThreads) void k_small(
     const SmallBlock B, JRaw *__restrict__ jraw, int *__restrict__ jindex,
     double *__restrict__ xp64, double *__restrict__ vp64, int cap, int chunks,
-    double *__restrict__ part, unsigned *__restrict__ ticket, Results R)
+    double *__restrict__ part, unsigned *__restrict__ ticket, Results R,
+    unsigned long long *done_flag, unsigned long long seq)
 {
 #pragma clang fp contract(off)
     constexpr int NW = kSmallThreads / 64;
@@ -1270,12 +1271,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_small(
                 col[G] = (double)a;   // component 8
             }
         }
-        __threadfence();
+        // release: the partials are visible device-wide before the ticket
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (lane == 0) is_last = atomicAdd(ticket, 1u) == (unsigned)(G - 1);
     }
     __syncthreads();
     if (!is_last) return;
-    __threadfence();
+    // acquire: the last workgroup sees every partial released before its ticket
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     // The last workgroup: NIP groups of QF threads, one group per i; thread
     // fq of a group adds workgroups fq, fq + QF, ... in order, then the group
     // combines its lanes (DPP) and, when it spans several waves, their sums
@@ -1348,6 +1351,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_small(
         R.nn[i] = nn;
         R.flags[i] = 0;
     }
+    if (done_flag) {
+        // release at system scope: each writer's results reach host memory
+        // before the completion word
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(done_flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     if (threadIdx.x == 0) atomicExch(ticket, 0u);   // ready for the next launch
 }
 
@@ -1362,7 +1373,7 @@ size_t small_part_doubles() { return (size_t)kSmallMaxI * kSmallComp * kSmallMax
 
 hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jindex,
                         double *xp64, double *vp64, int cap, double *part, unsigned *ticket,
-                        const Results &r)
+                        const Results &r, unsigned long long *done_flag, unsigned long long seq)
 {
     if (b.ni < 1 || b.ni > kSmallMaxI || b.nj < 1 || b.nj > cap || b.nd < 0 ||
         b.nd > kSmallMaxDirty)
@@ -1376,7 +1387,7 @@ hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jin
     while (nip < b.ni) nip *= 2;
 #define G6_SMALL_LAUNCH(N)                                                                  \
     hipLaunchKernelGGL(k_small<N>, grid, block, 0, s, b, jraw, jindex, xp64, vp64, cap,     \
-                       chunks, part, ticket, r)
+                       chunks, part, ticket, r, done_flag, seq)
     switch (nip) {
     case 1: G6_SMALL_LAUNCH(1); break;
     case 2: G6_SMALL_LAUNCH(2); break;
