@@ -1660,6 +1660,12 @@ struct Lane {
     int n_schunks = 0, n_bsys = 0, n_bitems_max = 0;
 };
 
+// kernel_ms slots, named kernel_us_<name> by g6e_get_option
+enum KernelSlot { KS_PREDICT, KS_FORCE, KS_CORRECT, KS_BSELECT, KS_BITEMS, KS_BFORCE, KS_BCORRECT,
+                  KS_FINALIZE, KS_FUSED, KS_N };
+const char *const kKernelNames[KS_N] = {"predict", "force", "correct", "bselect", "bitems",
+                                        "bforce", "bcorrect", "finalize", "fused"};
+
 struct Engine {
     bool open = false;
     hipStream_t stream = nullptr;
@@ -1712,15 +1718,9 @@ struct Engine {
     // option kernel_timing: lanes 0 and 1 launch their step kernels one by one
     // (no graphs) between HIP events; device milliseconds per kernel
     int kernel_timing = 0;
-    double kernel_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double kernel_ms[KS_N] = {};
     hipEvent_t kev[2] = {nullptr, nullptr};
 };
-
-// kernel_ms slots, named kernel_us_<name> by g6e_get_option
-enum KernelSlot { KS_PREDICT, KS_FORCE, KS_CORRECT, KS_BSELECT, KS_BITEMS, KS_BFORCE, KS_BCORRECT,
-                  KS_FINALIZE, KS_FUSED, KS_N };
-const char *const kKernelNames[KS_N] = {"predict", "force", "correct", "bselect", "bitems",
-                                        "bforce", "bcorrect", "finalize", "fused"};
 
 Engine X;
 
@@ -2175,6 +2175,33 @@ int bucket(int n)
 constexpr int kBForceWaves = 8192;
 constexpr int kBCorrectWaves = 8192;
 
+// The block path's launches, shared by the step graphs and the event-timed
+// steps.  sgrid: select chunks (one wave each); ygrid: block-mode systems.
+void launch_bselect(const Lane &L, int sgrid)
+{
+    hipLaunchKernelGGL(k_ens_bselect, dim3(sgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
+                       L.d_schunks, L.d_nsel);
+}
+
+void launch_bitems(const Lane &L, int ygrid)
+{
+    const BLists B{L.d_bchunks, L.d_bitems, L.d_bcount};
+    hipLaunchKernelGGL(k_ens_bitems, dim3(ygrid / 64), dim3(64), 0, L.stream, X.d_hdr, L.d_bsys,
+                       L.d_nsel + 1, B);
+}
+
+void launch_bforce(const Lane &L)
+{
+    hipLaunchKernelGGL(k_ens_bforce, dim3(std::min(kBForceWaves, L.n_bitems_max)), dim3(64), 0,
+                       L.stream, X.P, X.d_hdr, L.d_bitems, L.d_bcount + 1);
+}
+
+void launch_bcorrect(const Lane &L)
+{
+    hipLaunchKernelGGL(k_ens_bcorrect, dim3(std::min(kBCorrectWaves, L.n_schunks)), dim3(64), 0,
+                       L.stream, X.P, X.d_hdr, L.d_bchunks, L.d_bcount, 1e-30);
+}
+
 void enqueue_step(int ln, int cgrid, int igrid, int sgrid, int ygrid)
 {
     Lane &L = X.lanes[ln];
@@ -2187,15 +2214,10 @@ void enqueue_step(int ln, int cgrid, int igrid, int sgrid, int ygrid)
                            L.d_chunks, L.d_counts, 1e-30);
     }
     if (sgrid) {
-        const BLists B{L.d_bchunks, L.d_bitems, L.d_bcount};
-        hipLaunchKernelGGL(k_ens_bselect, dim3(sgrid), dim3(64), 0, L.stream, X.P, X.d_hdr,
-                           L.d_schunks, L.d_nsel);
-        hipLaunchKernelGGL(k_ens_bitems, dim3(ygrid / 64), dim3(64), 0, L.stream, X.d_hdr,
-                           L.d_bsys, L.d_nsel + 1, B);
-        hipLaunchKernelGGL(k_ens_bforce, dim3(std::min(kBForceWaves, L.n_bitems_max)), dim3(64),
-                           0, L.stream, X.P, X.d_hdr, L.d_bitems, L.d_bcount + 1);
-        hipLaunchKernelGGL(k_ens_bcorrect, dim3(std::min(kBCorrectWaves, L.n_schunks)), dim3(64),
-                           0, L.stream, X.P, X.d_hdr, L.d_bchunks, L.d_bcount, 1e-30);
+        launch_bselect(L, sgrid);
+        launch_bitems(L, ygrid);
+        launch_bforce(L);
+        launch_bcorrect(L);
     }
     hipLaunchKernelGGL(k_ens_finalize, dim3((X.max_slots + 255) / 256), dim3(256), 0, L.stream,
                        X.d_hdr, X.max_slots, X.d_phase, X.max_steps, 1e-30, ln, L.d_bcount);
@@ -2235,24 +2257,10 @@ int timed_step(int ln)
             return -1;
     }
     if (sgrid) {
-        const BLists B{L.d_bchunks, L.d_bitems, L.d_bcount};
-        if (timed(KS_BSELECT, [&] {
-                hipLaunchKernelGGL(k_ens_bselect, dim3(sgrid), dim3(64), 0, L.stream, X.P,
-                                   X.d_hdr, L.d_schunks, L.d_nsel);
-            }) ||
-            timed(KS_BITEMS, [&] {
-                hipLaunchKernelGGL(k_ens_bitems, dim3(ygrid / 64), dim3(64), 0, L.stream, X.d_hdr,
-                                   L.d_bsys, L.d_nsel + 1, B);
-            }) ||
-            timed(KS_BFORCE, [&] {
-                hipLaunchKernelGGL(k_ens_bforce, dim3(std::min(kBForceWaves, L.n_bitems_max)),
-                                   dim3(64), 0, L.stream, X.P, X.d_hdr, L.d_bitems, L.d_bcount + 1);
-            }) ||
-            timed(KS_BCORRECT, [&] {
-                hipLaunchKernelGGL(k_ens_bcorrect, dim3(std::min(kBCorrectWaves, L.n_schunks)),
-                                   dim3(64), 0, L.stream, X.P, X.d_hdr, L.d_bchunks, L.d_bcount,
-                                   1e-30);
-            }))
+        if (timed(KS_BSELECT, [&] { launch_bselect(L, sgrid); }) ||
+            timed(KS_BITEMS, [&] { launch_bitems(L, ygrid); }) ||
+            timed(KS_BFORCE, [&] { launch_bforce(L); }) ||
+            timed(KS_BCORRECT, [&] { launch_bcorrect(L); }))
             return -1;
     }
     return timed(KS_FINALIZE, [&] {
