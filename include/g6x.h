@@ -120,11 +120,14 @@ int g6x_sync(int cluster_id);
  *                    arguments; results within ~1e-15 of the FP64 direct sum,
  *                    nearest neighbours exact.  0 = the FP32 path
  *                    G6_SMALL_FP64
- *   "small_flag"     1 (default) = the last workgroup of such a launch
- *                    stores a sequence number in a pinned word after the
- *                    results, and lasthalf polls it (at most ~2 ms, then
- *                    the stream event) instead of waiting for the event's
- *                    completion signal; 0 = the event   G6_SMALL_FLAG
+ *   "done_word"      1 (default) = the workgroup that finishes an ABI
+ *                    block's last launch (the small-block kernel, or the
+ *                    split reduction of a block of at most 512 i) stores a
+ *                    sequence number in a pinned word after the results,
+ *                    and lasthalf polls it (at most ~2 ms, then a stream
+ *                    wait) instead of waiting for an event's completion
+ *                    signal; 0 = the event (also while "timing" is on)
+ *                    G6_DONE_WORD
  *   "fail_lasthalf", "fail_read_nl", "fail_nn"   host-side failure injection
  *                    for a caller's recovery path (kira resets, reloads and
  *                    retries, hdyn_grape6.C:405-438, 853-900, 2110-2166):

@@ -45,7 +45,7 @@ struct Options {
     int wpg = 0;              // waves per force workgroup (j-splits folded in LDS): 0 auto, 1..16
     int fused_small = 1;      // ABI blocks of one i-group predict j inside the force kernel
     int small_fp64 = 1;       // ABI blocks of at most kSmallMaxI i: one FP64 launch (k_small)
-    int small_flag = 1;       // lasthalf of such a block polls k_small's completion word
+    int done_word = 1;        // lasthalf polls the block's completion word, not an event
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every
                               // size so far, so off unless asked for)
@@ -86,10 +86,10 @@ struct Engine {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev_done = nullptr;
-    // completion word of the one-launch small-block kernel: its last
-    // workgroup stores the block's sequence number here (pinned, coherent)
-    // after the results, so lasthalf can poll it instead of waiting for
-    // ev_done's completion signal
+    // completion word of an ABI block: the workgroup that finishes the
+    // block's last launch (k_small, or the last k_reduce) stores the block's
+    // sequence number here (pinned, coherent) after the results, so lasthalf
+    // can poll it instead of waiting for ev_done's completion signal
     unsigned long long *h_flag = nullptr;
     unsigned long long *h_flag_dev = nullptr;
     unsigned long long flag_seq = 0;
@@ -141,6 +141,7 @@ struct Engine {
     // k_small: its workgroup partials (small_part_doubles()), then its
     // ticket counter (zero between launches)
     double *d_small = nullptr;
+    unsigned *d_done_ticket = nullptr;   // k_reduce's completion ticket (zero between launches)
 
     // ---- results ----------------------------------------------------------
     int rcap = 0;
@@ -235,7 +236,8 @@ int count_refined(const Results &h, int ni);
 // partials_bytes(ib.ld, partial splits of the geometry); nullptr = the
 // library's own (grown) buffer.
 int enqueue_force(int ni, int nj, double eps2, const IBlock &ib, const Results &r,
-                  bool zskip = false, const FusedPredict *fused = nullptr, char *part = nullptr);
+                  bool zskip = false, const FusedPredict *fused = nullptr, char *part = nullptr,
+                  const DoneWord *done = nullptr);
 
 // One i-block through the force path.  Slot k has position x[k], velocity
 // v[k], excluded j address excl[k] (-1: none; nullptr: none for all), and

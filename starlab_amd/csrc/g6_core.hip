@@ -75,7 +75,7 @@ void read_env()
     o.wpg = (int)env_ll("WPG", o.wpg);
     o.fused_small = (int)env_ll("FUSED_SMALL", o.fused_small);
     o.small_fp64 = (int)env_ll("SMALL_FP64", o.small_fp64);
-    o.small_flag = (int)env_ll("SMALL_FLAG", o.small_flag);
+    o.done_word = (int)env_ll("DONE_WORD", o.done_word);
     o.fail_lasthalf = (int)env_ll("FAIL_LASTHALF", o.fail_lasthalf);
     o.fail_read_nl = (int)env_ll("FAIL_READNL", o.fail_read_nl);
     o.fail_nn = (int)env_ll("FAIL_NN", o.fail_nn);
@@ -538,7 +538,7 @@ Geometry choose_geometry(int ni, int nj)
 // Enqueue force + reduction (with the nearest-neighbour refinement) for an
 // i-block already in d_ibuf.
 int enqueue_force(int ni, int nj, double eps2d, const IBlock &ib, const Results &r, bool zskip,
-                  const FusedPredict *fused, char *part)
+                  const FusedPredict *fused, char *part, const DoneWord *done)
 {
     float eps2 = (float)eps2d;
     if (zskip && !(eps2 >= __FLT_MIN__)) {   // as launch_force treats it
@@ -567,7 +567,7 @@ int enqueue_force(int ni, int nj, double eps2d, const IBlock &ib, const Results 
         const NearTile nt{eps2d, g.jchunk, g.variant == 1 ? 1 : g.wpg, zskip ? 1 : 0,
                           E.d_vp64, E.d_vp64 + 3 * (size_t)E.dcap};
         HIPCHK(launch_reduce(E.stream, p, g.npart, ni, nj, ib, eps2, g.tile,
-                             E.d_xp64, E.dcap, E.d_jindex, r, g.lanes, nt));
+                             E.d_xp64, E.dcap, E.d_jindex, r, g.lanes, nt, done));
         t.stop();
     }
     return 0;
@@ -601,6 +601,8 @@ int do_open(int dev)
     HIPCHK(hipHostMalloc((void **)&E.h_flag, 64, hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer((void **)&E.h_flag_dev, E.h_flag, 0));
     __atomic_store_n(E.h_flag, 0ull, __ATOMIC_RELEASE);
+    HIPCHK(hipMalloc((void **)&E.d_done_ticket, 256));
+    HIPCHK(hipMemset(E.d_done_ticket, 0, 256));
     E.flag_seq = 0;
     E.flag_pending = false;
     E.device = dev;
@@ -633,6 +635,7 @@ int do_close()
     E.icap = 0;
     dfree(E.d_part); E.pbytes = 0;
     dfree(E.d_small);
+    dfree(E.d_done_ticket);
     dfree(E.d_nblist); hfree(E.h_nbcount); hfree(E.h_nblist);
     dfree(E.d_nlscratch); E.nlscratch_cap = 0;
     hfree(E.h_nlinfo); E.nlinfo_cap = 0;
@@ -832,7 +835,7 @@ int calc_small(int nj, int ni, const double (*xi)[3], const double (*vi)[3], con
     const Results r = results_at(E.h_res_dev, ni);
     KTimer t(K_FORCE);
     // with kernel timing on, lasthalf waits for the event (it reads the timers)
-    const bool flag = E.h_flag_dev && E.opt.small_flag && !E.opt.timing;
+    const bool flag = E.h_flag_dev && E.opt.done_word && !E.opt.timing;
     const hipError_t e = launch_small(E.stream, B, E.d_jraw, E.d_jindex, E.d_xp64, E.d_vp64,
                                       E.dcap, E.d_small,
                                       reinterpret_cast<unsigned *>(E.d_small + npart), r,
@@ -870,6 +873,9 @@ int calc_small(int nj, int ni, const double (*xi)[3], const double (*vi)[3], con
     E.geom_lanes = 64;
     return 0;
 }
+
+// Largest FP32-path block whose reduction stores the completion word.
+constexpr int kDoneWordMaxI = 512;
 
 int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const double (*vi)[3],
                const int *excl, const double *h2, const double *e2, const IGroup *groups,
@@ -912,8 +918,14 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
         if (scatter_with_iblock(ld)) return -1;
         if (!fused && predict(nj, false)) return -1;
         const IBlock ib = iblock_at(E.d_ibuf, ld);
-        // the reduction writes the results straight into pinned host memory
+        // the reduction writes the results straight into pinned host memory;
+        // for blocks of at most kDoneWordMaxI i the last group's reduction
+        // also stores the completion word (larger blocks gained nothing: each
+        // of their many reduction workgroups writes back its L2 first; with
+        // kernel timing on, lasthalf waits for the event: it reads the timers)
         const Results r = results_at(E.h_res_dev, ni);
+        const bool word = E.h_flag_dev && E.opt.done_word && !E.opt.timing && ni <= kDoneWordMaxI;
+        const DoneWord dw{E.d_done_ticket, E.h_flag_dev, E.flag_seq + 1};
         for (int g = 0; g < ngroups; ++g) {
             // a group is the block shifted by its first slot (same stride); the
             // partial-sum buffer is reused in stream order
@@ -936,8 +948,13 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
                 E.xp_nj = nj;
             }
             if (enqueue_force(groups[g].n, nj, groups[g].eps2, gb, gr, groups[g].zskip,
-                              fused ? &fp : nullptr))
+                              fused ? &fp : nullptr, nullptr,
+                              word && g == ngroups - 1 ? &dw : nullptr))
                 return -1;
+        }
+        if (word) {
+            E.flag_seq += 1;
+            E.flag_pending = true;
         }
     }
     E.res_seq = mark_queued();
@@ -953,8 +970,8 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     return 0;
 }
 
-// Poll the small-block kernel's completion word for seq, at most ~2 ms
-// (a slower block, or a failed one, then waits for the event instead).
+// Poll the block's completion word for seq, at most ~2 ms (a slower block,
+// or a failed one, then waits for the stream instead).
 bool poll_done(unsigned long long seq)
 {
     using clk = std::chrono::steady_clock;

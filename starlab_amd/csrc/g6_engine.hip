@@ -342,7 +342,7 @@ int g6x_set_option(const char *key, long long value)
     else if (k == "tile") o.tile = (int)value;
     else if (k == "fused_small") o.fused_small = value != 0 ? 1 : 0;
     else if (k == "small_fp64") o.small_fp64 = value != 0 ? 1 : 0;
-    else if (k == "small_flag") o.small_flag = value != 0 ? 1 : 0;
+    else if (k == "done_word") o.done_word = value != 0 ? 1 : 0;
     else if (k == "fail_lasthalf") o.fail_lasthalf = (int)std::max(0LL, value);
     else if (k == "fail_read_nl") o.fail_read_nl = (int)std::max(0LL, value);
     else if (k == "fail_nn") o.fail_nn = (int)std::max(0LL, value);
@@ -381,7 +381,7 @@ long long g6x_get_option(const char *key)
     if (k == "fail_nn") return o.fail_nn;
     if (k == "fused_small") return o.fused_small;
     if (k == "small_fp64") return o.small_fp64;
-    if (k == "small_flag") return o.small_flag;
+    if (k == "done_word") return o.done_word;
     if (k == "last_tile") return E.geom_tile;
     if (k == "reduce_lanes") return o.reduce_lanes;
     if (k == "wpg") return o.wpg;

@@ -136,10 +136,20 @@ struct NearTile {
     const double *vp64;
     const double *m64;     // FP64 masses (vp64 + 3 * cap)
 };
+// A block's completion word: the launch that ends the block stores seq into
+// word (pinned host memory) once its results are in host memory; ticket is a
+// device counter, zero between launches.
+struct DoneWord {
+    unsigned *ticket;
+    unsigned long long *word;
+    unsigned long long seq;
+};
+
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
-                         const Results &r, int lanes, const NearTile &nt);
+                         const Results &r, int lanes, const NearTile &nt,
+                         const DoneWord *done = nullptr);
 // Small ABI blocks in one launch (k_small): at most kSmallMaxI i with one
 // softening, FP64 pair arithmetic; each wave predicts 64 j at a time (one
 // per lane) into LDS and splits its lanes into one group per i.

@@ -850,14 +850,18 @@ template <int LOG2L>
 __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
     Partials P, int nsplit, int ni, int nj, IBlock ib,
     float eps2, int tile_len, const double *__restrict__ xp64, int cap,
-    const int *__restrict__ jindex, Results R, NearTile NT)
+    const int *__restrict__ jindex, Results R, NearTile NT, unsigned *ticket,
+    unsigned long long *done_word, unsigned long long seq)
 {
 #pragma clang fp contract(off)
     constexpr int L = 1 << LOG2L;
     const int lane = threadIdx.x & 63;
     const int l = lane & (L - 1);
-    const int i = (blockIdx.x * kReduceWaves + (threadIdx.x >> 6)) * (64 / L) + (lane >> LOG2L);
-    if (i >= ni) return;   // whole group
+    const int ig = (blockIdx.x * kReduceWaves + (threadIdx.x >> 6)) * (64 / L) + (lane >> LOG2L);
+    // groups past the block's end redo its last i and write nothing, so that
+    // every thread reaches the completion ticket below
+    const bool live = ig < ni;
+    const int i = live ? ig : ni - 1;
 
     double A[7] = {0, 0, 0, 0, 0, 0, 0};
     float best = __builtin_inff(), second = __builtin_inff();
@@ -975,7 +979,7 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         else nn = jindex[found];
         if (!(second > best * (1.0f + kNNBand))) ambiguous = true;
     }
-    if (l == 0) {
+    if (l == 0 && live) {
         R.acc[3 * i + 0] = A[0];
         R.acc[3 * i + 1] = A[1];
         R.acc[3 * i + 2] = A[2];
@@ -1005,9 +1009,22 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         }
         if (ba != INT_MAX) nn = jindex[ba];
     }
-    if (l == 0) {
+    if (l == 0 && live) {
         R.nn[i] = nn;
         R.flags[i] = refine ? 1 : 0;
+    }
+    if (done_word) {
+        // Completion word: each workgroup releases its results to host memory
+        // (system scope) before its ticket; the one that draws the last
+        // ticket stores seq, which the host polls instead of an event.
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            if (atomicAdd(ticket, 1u) == gridDim.x - 1) {
+                __hip_atomic_store(done_word, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                atomicExch(ticket, 0u);   // ready for the next launch
+            }
+        }
     }
 }
 
@@ -1022,33 +1039,35 @@ int reduce_lanes(int nsplit)
 hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
                          int nj, const IBlock &ib, float eps2,
                          int tile, const double *xp64, int cap, const int *jindex,
-                         const Results &r, int lanes, const NearTile &nt)
+                         const Results &r, int lanes, const NearTile &nt,
+                         const DoneWord *done)
 {
     if (ni <= 0) return hipSuccess;
     if (nt.jchunk <= 0 || nt.jchunk % tile || nt.wpg < 1) return hipErrorInvalidValue;
     if (lanes <= 0) lanes = reduce_lanes(nsplit);
     const int per_wg = kReduceWaves * (64 / lanes);
     const dim3 grid((ni + per_wg - 1) / per_wg), block(64 * kReduceWaves);
+    const DoneWord dw = done ? *done : DoneWord{nullptr, nullptr, 0};
     switch (lanes) {
     case 4:
         hipLaunchKernelGGL(k_reduce<2>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r, nt);
+                           tile, xp64, cap, jindex, r, nt, dw.ticket, dw.word, dw.seq);
         break;
     case 8:
         hipLaunchKernelGGL(k_reduce<3>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r, nt);
+                           tile, xp64, cap, jindex, r, nt, dw.ticket, dw.word, dw.seq);
         break;
     case 16:
         hipLaunchKernelGGL(k_reduce<4>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r, nt);
+                           tile, xp64, cap, jindex, r, nt, dw.ticket, dw.word, dw.seq);
         break;
     case 32:
         hipLaunchKernelGGL(k_reduce<5>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r, nt);
+                           tile, xp64, cap, jindex, r, nt, dw.ticket, dw.word, dw.seq);
         break;
     case 64:
         hipLaunchKernelGGL(k_reduce<6>, grid, block, 0, s, p, nsplit, ni, nj, ib, eps2,
-                           tile, xp64, cap, jindex, r, nt);
+                           tile, xp64, cap, jindex, r, nt, dw.ticket, dw.word, dw.seq);
         break;
     default:
         return hipErrorInvalidValue;

@@ -212,3 +212,71 @@ def test_small_block_repeats_bitwise(g):
         again = g.force(nj, s["idx"][isel], xp[isel], vp[isel], 1e-8)
         for u, w in zip(first, again):
             np.testing.assert_array_equal(u, w)
+
+
+def _abi_block_run(g, s, nj, seed):
+    """A kira-like sequence of ABI blocks: j updates carried in the launch
+    arguments (at most 16) and by the scatter launch (17), blocks of 1 to 8
+    i (one launch) and of 9 to 2048 i (the FP32 path, with j predicted in
+    the force kernel up to 256 i, by the predictor above), blocks with
+    neighbour spheres and their lists, and a j update made between the two
+    halves of a block.  Returns every result in order."""
+    rng = np.random.default_rng(seed)
+    out = []
+    ti = 0.0
+    steps = [(1, 0), (2, 3), (4, 16), (8, 17), (3, 1), (5, 2), (9, 2), (64, 20), (300, 5),
+             (2048, 0)]
+    for step, (ni, nup) in enumerate(steps):
+        ti += 2.0 ** -12
+        for a in np.sort(rng.choice(nj, nup, replace=False)):
+            s["x"][a] += rng.normal(scale=1e-3, size=3)
+            s["tj"][a] = ti - 2.0 ** -13
+            assert g.set_j_particle(int(a), int(s["idx"][a]), s["tj"][a], s["dtj"][a],
+                                    s["m"][a], s["k18"][a], s["j6"][a], s["a2"][a], s["v"][a],
+                                    s["x"][a]) == 0
+        g.set_ti(ti)
+        xp, vp = _predict(s, ti)
+        isel = np.sort(rng.choice(nj, ni, replace=False))
+        h2 = np.full(ni, 0.02) if step in (4, 7) else np.zeros(ni)
+        g.calc_firsthalf(nj, s["idx"][isel], xp[isel], vp[isel], 1e-8, h2)
+        if step in (5, 8):
+            # the block in flight keeps the j-memory it started with
+            a = int(rng.integers(nj))
+            s["x"][a] += 1e-3
+            assert g.set_j_particle(a, int(s["idx"][a]), s["tj"][a], s["dtj"][a], s["m"][a],
+                                    s["k18"][a], s["j6"][a], s["a2"][a], s["v"][a],
+                                    s["x"][a]) == 0
+        rc, acc, jerk, pot, nn = g.calc_lasthalf2(nj, ni)
+        assert rc == 0, g.last_error()
+        out.append((acc, jerk, pot, nn))
+        if step in (4, 7):
+            assert g.read_neighbour_list() == 0
+            for k in range(ni):
+                rc, lst = g.get_neighbour_list(k, 4096)
+                assert rc == 0
+                out.append((lst,))
+    return out
+
+
+def test_completion_word_equals_event_wait(g):
+    """lasthalf polls the completion word that a block's last launch (k_small,
+    or the split reduction of the FP32 path) writes to pinned memory (option
+    done_word = 1, the default) instead of waiting on a HIP event; every
+    result of the same call sequence is the same bit for bit either way,
+    lists and mid-block j updates included."""
+    nj = 65536
+    base = sphere_state(nj, seed=98)
+    runs = {}
+    try:
+        for flag in (1, 0, 1):
+            assert g.option("done_word", flag) == flag
+            s = {k: v.copy() for k, v in base.items()}
+            load_j(g, s)
+            runs.setdefault(flag, []).append(_abi_block_run(g, s, nj, seed=6))
+    finally:
+        g.option("done_word", 1)
+    for ref in runs[1][1:] + runs[0]:
+        assert len(ref) == len(runs[1][0])
+        for u, w in zip(runs[1][0], ref):
+            for a, b in zip(u, w):
+                np.testing.assert_array_equal(a, b)
