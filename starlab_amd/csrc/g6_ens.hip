@@ -519,6 +519,39 @@ __device__ __forceinline__ double block_advance(double &ti, double &di, double c
     return ti < h.t_end ? ti + di : __builtin_inf();
 }
 
+// The split partials of list position q, added in split order: split 0 as
+// is (no "0 +" rounding), then splits 1, 2, ...  The loads of four splits
+// are issued before their additions, so a system of many splits waits for
+// memory once per four splits instead of once per split.
+__device__ __forceinline__ void sum_splits(const Pool &P, int nsplit, size_t q, double a1[3],
+                                           double j1[3], double &pot)
+{
+#pragma clang fp contract(off)
+    constexpr int U = 4;
+    const size_t cap = P.cap;
+    double A[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) A[c] = P.part[c * cap + q];
+    for (int s0 = 1; s0 < nsplit; s0 += U) {
+        double v[U][7];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double *in = P.part + (size_t)min(s0 + u, nsplit - 1) * 7 * cap;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) v[u][c] = in[c * cap + q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (s0 + u >= nsplit) break;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) A[c] += v[u][c];
+        }
+    }
+    a1[0] = A[0]; a1[1] = A[1]; a1[2] = A[2];
+    j1[0] = A[3]; j1[1] = A[4]; j1[2] = A[5];
+    pot = A[6];
+}
+
 __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__ H,
                                                     const Chunk *__restrict__ chunks,
                                                     const int *__restrict__ nchunks,
@@ -542,20 +575,8 @@ __global__ __launch_bounds__(64) void k_ens_correct(Pool P, SysHdr *__restrict__
         const size_t cap = P.cap;
         const size_t p = (size_t)h.off + i;
         const size_t q = (size_t)h.off + sel.pos;   // partials by list position
-        // split 0 is taken as is (no "0 +" rounding), later splits added in order
         double a1[3], j1[3], pot;
-        {
-            const double *in = P.part;
-            a1[0] = in[0 * cap + q]; a1[1] = in[1 * cap + q]; a1[2] = in[2 * cap + q];
-            j1[0] = in[3 * cap + q]; j1[1] = in[4 * cap + q]; j1[2] = in[5 * cap + q];
-            pot = in[6 * cap + q];
-        }
-        for (int s = 1; s < h.nsplit; ++s) {
-            const double *in = P.part + (size_t)s * 7 * cap;
-            a1[0] += in[0 * cap + q]; a1[1] += in[1 * cap + q]; a1[2] += in[2 * cap + q];
-            j1[0] += in[3 * cap + q]; j1[1] += in[4 * cap + q]; j1[2] += in[5 * cap + q];
-            pot += in[6 * cap + q];
-        }
+        sum_splits(P, h.nsplit, q, a1, j1, pot);
         PState st;
         double xp[3], vp[3];
         for (int k = 0; k < 3; ++k) {
@@ -952,18 +973,7 @@ __device__ __forceinline__ double bcorrect_one(const Pool &P, const SysHdr &h, i
     const size_t p = (size_t)h.off + i;
     const size_t q = (size_t)h.off + pos;
     double a1[3], j1[3], pot;
-    {
-        const double *in = P.part;
-        a1[0] = in[0 * cap + q]; a1[1] = in[1 * cap + q]; a1[2] = in[2 * cap + q];
-        j1[0] = in[3 * cap + q]; j1[1] = in[4 * cap + q]; j1[2] = in[5 * cap + q];
-        pot = in[6 * cap + q];
-    }
-    for (int s = 1; s < h.nsplit; ++s) {
-        const double *in = P.part + (size_t)s * 7 * cap;
-        a1[0] += in[0 * cap + q]; a1[1] += in[1 * cap + q]; a1[2] += in[2 * cap + q];
-        j1[0] += in[3 * cap + q]; j1[1] += in[4 * cap + q]; j1[2] += in[5 * cap + q];
-        pot += in[6 * cap + q];
-    }
+    sum_splits(P, h.nsplit, q, a1, j1, pot);
     PState st;
     double xp[3], vp[3];
     bpredict(P, h, p, xp, vp);
