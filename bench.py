@@ -277,12 +277,13 @@ def aux_legs(g, inp, reps=3):
     return out
 
 
-def grape4_leg(inp, xp, vp, npipe=2048):
-    """One full sweep through the GRAPE-4 ABI (libg4x.so) the way kira drives it
+def grape4_leg(inp, xp, vp, npipe=2048, sweeps=7):
+    """Full sweeps through the GRAPE-4 ABI (libg4x.so) the way kira drives it
     with USE_HALF_CHIP: every particle in two adjacent pipes, 1024 particles per
     h3calc call, host buffers in and out (PCIe and host packing included).
     xp, vp: the j-particles predicted to t = 0 (read back from libg6), which
-    kira would have predicted on the host."""
+    kira would have predicted on the host.  A sweep takes ~10 ms, so one host
+    stall can dominate it: the median of `sweeps` sweeps is reported."""
     from starlab_amd.g4 import GRAPE4
     n = len(inp["m"])
     g4 = GRAPE4()
@@ -292,14 +293,19 @@ def grape4_leg(inp, xp, vp, npipe=2048):
     per = npipe // 2
     blocks = [np.repeat(np.arange(s, min(n, s + per)), 2) for s in range(0, n, per)]
     g4.calc(n, xp[blocks[0]], vp[blocks[0]], EPS2 / 2, 0.0)      # warm
-    t0 = time.perf_counter()
-    for b in blocks:
-        g4.calc(n, xp[b], vp[b], EPS2 / 2, 0.0)
-    wall = time.perf_counter() - t0
+    walls = []
+    for _ in range(sweeps):
+        t0 = time.perf_counter()
+        for b in blocks:
+            g4.calc(n, xp[b], vp[b], EPS2 / 2, 0.0)
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
     slots = g4.option("last_slots")
     g4.close()
     return {"pairs_per_s": n * (n - 1) / wall, "wall_ms": round(wall * 1e3, 3),
-            "calls": len(blocks), "pipes_per_call": npipe, "distinct_i_per_call": slots}
+            "wall_ms_min": round(min(walls) * 1e3, 3), "wall_ms_max": round(max(walls) * 1e3, 3),
+            "sweeps": sweeps, "calls": len(blocks), "pipes_per_call": npipe,
+            "distinct_i_per_call": slots}
 
 
 def abi_leg(nj=N, blocks=200, sizes=(1, 16, 256, 2048), h2=0.0):
