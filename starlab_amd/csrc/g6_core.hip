@@ -996,8 +996,10 @@ const Results *calc_wait(const char *who)
         E.flag_pending = false;
         if (poll_done(E.flag_seq)) {
             // the results are in host memory, and every host buffer the block
-            // used (it reads its j updates and i-block from its arguments) is
-            // free; the kernel's own end is ordered before the next launch
+            // used is free: k_small reads its j updates and i-block from its
+            // arguments, and on the FP32 path the scatter that read the pinned
+            // staging ran before the reduction that stored the word; the
+            // kernel's own end is ordered before the next launch
             E.synced_seq = std::max(E.synced_seq, E.done_seq);
             h = results_at(E.h_res, E.last_ni);
             E.last_refinements = count_refined(h, E.last_ni);
