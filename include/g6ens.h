@@ -72,7 +72,7 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
               double *device_seconds);
 
 /* Options: "steps_per_graph" (default 8), "jsplit_target" (j per split
- * within a system, default 512; 0 = no split), "max_split" (default 16),
+ * within a system, default 256; 0 = no split), "max_split" (default 16),
  * "lookahead" (queued jobs examined per admission pass, default 64),
  * "precision" (pair arithmetic of jobs submitted afterwards: 64 = FP64 in the
  * CPU oracle's operation order, the default; 32 = FP32 pairs on

@@ -1702,7 +1702,7 @@ struct Engine {
     std::vector<long long> finished;
     // options
     int steps_per_graph = 8;
-    int jsplit_target = 512;
+    int jsplit_target = 256;
     int max_split = 16;
     int lookahead = 64;
     int precision = 64;
