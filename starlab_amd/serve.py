@@ -110,12 +110,20 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep
     return out
 
 
-def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=8, keep=False,
+def run_poisson(jobs, rate, seed=0, budget=64 * GB, max_systems=4096, chunk=16, keep=False,
                 **opt):
     """Serve `jobs` arriving as a Poisson stream of `rate` per second;
-    keep=True adds each job's result in job order."""
+    keep=True adds each job's result in job order.
+
+    Each engine call runs `chunk` steps of the heavy lane (one graph) while
+    the light lane's fused kernel takes up to `fused_steps` steps per launch
+    (default here 64).  Arrivals wait for the current call, so longer calls
+    raise the small jobs' latency and shorten the clusters'; on the serving
+    mix at 2000 req/s, 16 and 64 measured p99 37 ms with p50 6.1 ms, against
+    p99 57 ms at 8 and 256, and p50 8.3 ms at 64 and 64."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
+    opt.setdefault("fused_steps", 64)
     eng = Ensemble(budget, max_systems, steps_per_graph=chunk, **opt)
     try:
         t0 = time.perf_counter()
