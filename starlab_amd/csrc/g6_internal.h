@@ -163,7 +163,7 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
 // fast (its cost hardly grows with ni at this size)
 constexpr int kSmallMaxI = 8;
 constexpr int kSmallMaxDirty = 16;
-constexpr int kSmallMaxWG = 512;      // workgroups per launch at most
+constexpr int kSmallMaxWG = 256;      // workgroups per launch at most
 constexpr int kSmallThreads = 256;    // threads per workgroup
 constexpr int kSmallComp = 9;         // ax ay az jx jy jz pot, nearest d2, its address
 struct SmallBlock {

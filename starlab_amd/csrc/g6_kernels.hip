@@ -1381,10 +1381,15 @@ __global__ __launch_bounds__(kSmallThreads) void k_small(
     if (threadIdx.x == 0) atomicExch(ticket, 0u);   // ready for the next launch
 }
 
-// workgroups and 64-j chunks per wave for nj j-particles
+// Workgroups and 64-j chunks per wave for nj j-particles: about two chunks per
+// wave, at most kSmallMaxWG workgroups.  Fewer workgroups mean fewer partials
+// for the last one to read and fewer tickets; measured per block (ni = 1 / 4 /
+// 8): nj = 65536 19.2 / 22.4 / 27.7 us with 128 workgroups against 20.6 /
+// 23.5 / 29.1 with 256; nj = 262144 26.9 / 32.8 / 38.7 us with 256 against
+// 28.7 / 34.6 / 44.0 with 512 and 29.4 / 36.3 / 43.8 with 128.
 void small_grid(int nj, int &wgs, int &chunks)
 {
-    wgs = std::min(kSmallMaxWG, std::max(1, (nj + kSmallThreads - 1) / kSmallThreads));
+    wgs = std::min(kSmallMaxWG, std::max(1, (nj + 2 * kSmallThreads - 1) / (2 * kSmallThreads)));
     chunks = (nj + wgs * kSmallThreads - 1) / (wgs * kSmallThreads);
 }
 

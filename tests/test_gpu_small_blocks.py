@@ -62,11 +62,11 @@ def _check64(acc, jerk, pot, nn, s, xp, vp, xi, vi, excl, eps2, label):
 
 @pytest.mark.parametrize("nj", [1000, 65536, 262144, 300000, 600000])
 def test_small_blocks_match_fp64_oracle(g, nj):
-    """At most 512 workgroups of 256 threads, 64 j per wave and chunk: 1, 1,
-    2, 3 and 5 chunks per wave (the next chunk's records load while one is
-    summed); i counts 1, 2, 3, 5, 8 cover every lane grouping, and 9 takes
-    the FP32 path."""
-    wgs = min(512, -(-nj // 256))
+    """About nj/512 workgroups of 256 threads, at most 256, 64 j per wave and
+    chunk: 2, 2, 4, 5 and 10 chunks per wave (the next chunk's records load
+    while one is summed); i counts 1, 2, 3, 5, 8 cover every lane grouping,
+    and 9 takes the FP32 path."""
+    wgs = min(256, -(-nj // 512))
     chunks = -(-nj // (wgs * 256))
     s = plummer_state(nj, seed=91) if nj <= 65536 else sphere_state(nj, 91)
     load_j(g, s)
