@@ -469,7 +469,7 @@ def main():
                          "reference for scaling HBM traffic counters, and exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "libg6 drives one GPU per process; this build measures "
+        print(json.dumps({"error": "libg6 drives one GPU per process; this benchmark measures "
                                    "N=1 only (multi-GPU sharding is a design note)"}))
         sys.exit(2)
 

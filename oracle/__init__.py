@@ -1,7 +1,7 @@
 """CPU oracle for the GRAPE-6 force path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
-``bench.py`` import this package, and only as the checker.  The product
+``bench.py`` import this package, and only to check results.  The product
 (``starlab_amd`` / ``libg6.so``) never imports, links or calls it.
 
 ``liboracle_g6.so`` is built from ``g6_oracle.c`` (FP64, no FMA contraction)

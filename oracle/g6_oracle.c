@@ -2,7 +2,7 @@
  * g6_oracle.c -- CPU oracle for the GRAPE-6 force path (TEST INFRASTRUCTURE).
  *
  * Only the test suite, the smoke check and the CPU-baseline leg of the
- * benchmark load this code, and only as the checker.  The product library
+ * benchmark load this code, and only to check results.  The product library
  * never links or calls it.
  *
  * Build: see oracle/Makefile (always -O2 -ffp-contract=off: every FP64

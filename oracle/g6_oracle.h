@@ -1,7 +1,7 @@
 /*
  * g6_oracle.h -- CPU oracle for the GRAPE-6 force path (TEST INFRASTRUCTURE).
  *
- * This is the checker, never the product: only tests/, the smoke check and
+ * It checks results and is never the product: only tests/, the smoke check and
  * the CPU-baseline leg of bench.py may load it.  Everything is plain FP64 C,
  * compiled with -ffp-contract=off so every expression rounds exactly as
  * written (no fused multiply-adds), which is what makes bitwise comparison
