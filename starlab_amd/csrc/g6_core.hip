@@ -801,7 +801,7 @@ int calc_small(int nj, int ni, const double (*xi)[3], const double (*vi)[3], con
         HIPCHK(hipMalloc((void **)&E.d_small, sizeof(double) * npart + 256));
         HIPCHK(hipMemsetAsync(E.d_small + npart, 0, 256, E.stream));
     }
-    static SmallBlock B;   // 3.6 KB, copied into the launch's arguments
+    static SmallBlock B;   // the full form; launch_small passes the smallest form that holds it
     B.ti = E.ti;
     B.eps2 = eps2;
     B.ni = ni;

@@ -1173,9 +1173,9 @@ __device__ __forceinline__ JRawLite load_lite(const JRaw &r, int order)
     return o;
 }
 
-template <int NIP>
+template <int NIP, int MD>
 __global__ __launch_bounds__(kSmallThreads) void k_small(
-    const SmallBlock B, JRaw *__restrict__ jraw, int *__restrict__ jindex,
+    const SmallArgs<NIP, MD> B, JRaw *__restrict__ jraw, int *__restrict__ jindex,
     double *__restrict__ xp64, double *__restrict__ vp64, int cap, int chunks,
     double *__restrict__ part, unsigned *__restrict__ ticket, Results R,
     unsigned long long *done_flag, unsigned long long seq)
@@ -1395,6 +1395,41 @@ void small_grid(int nj, int &wgs, int &chunks)
 
 size_t small_part_doubles() { return (size_t)kSmallMaxI * kSmallComp * kSmallMaxWG; }
 
+// One launch of k_small<N, MD>: the block copied into its argument form (N
+// i-slots, room for MD >= b.nd pending records).
+template <int N, int MD>
+hipError_t launch_small_t(hipStream_t s, const SmallBlock &b, dim3 grid, JRaw *jraw,
+                          int *jindex, double *xp64, double *vp64, int cap, int chunks,
+                          double *part, unsigned *ticket, const Results &r,
+                          unsigned long long *done_flag, unsigned long long seq)
+{
+    static_assert(N <= kSmallMaxI && MD <= kSmallMaxDirty, "argument form larger than the block");
+    SmallArgs<N, MD> a;
+    a.ti = b.ti;
+    a.eps2 = b.eps2;
+    a.ni = b.ni;
+    a.nj = b.nj;
+    a.nd = b.nd;
+    a.order = b.order;
+    a.write_pred = b.write_pred;
+    a.pad = 0;
+    for (int k = 0; k < N; ++k) {
+        for (int c = 0; c < 3; ++c) {
+            a.xi[k][c] = b.xi[k][c];
+            a.vi[k][c] = b.vi[k][c];
+        }
+        a.excl[k] = b.excl[k];
+    }
+    for (int k = 0; k < b.nd; ++k) {
+        a.daddr[k] = b.daddr[k];
+        a.dindex[k] = b.dindex[k];
+        a.drec[k] = b.drec[k];
+    }
+    hipLaunchKernelGGL((k_small<N, MD>), grid, dim3(kSmallThreads), 0, s, a, jraw, jindex, xp64,
+                       vp64, cap, chunks, part, ticket, r, done_flag, seq);
+    return hipGetLastError();
+}
+
 hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jindex,
                         double *xp64, double *vp64, int cap, double *part, unsigned *ticket,
                         const Results &r, unsigned long long *done_flag, unsigned long long seq)
@@ -1406,21 +1441,23 @@ hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jin
         if (b.daddr[k] < 0 || b.daddr[k] >= b.nj) return hipErrorInvalidValue;
     int wgs, chunks;
     small_grid(b.nj, wgs, chunks);
-    const dim3 grid(wgs), block(kSmallThreads);
+    const dim3 grid(wgs);
     int nip = 1;
     while (nip < b.ni) nip *= 2;
+    const bool few = b.nd <= kSmallFewDirty;
 #define G6_SMALL_LAUNCH(N)                                                                  \
-    hipLaunchKernelGGL(k_small<N>, grid, block, 0, s, b, jraw, jindex, xp64, vp64, cap,     \
-                       chunks, part, ticket, r, done_flag, seq)
+    (few ? launch_small_t<N, kSmallFewDirty>(s, b, grid, jraw, jindex, xp64, vp64, cap,     \
+                                             chunks, part, ticket, r, done_flag, seq)       \
+         : launch_small_t<N, kSmallMaxDirty>(s, b, grid, jraw, jindex, xp64, vp64, cap,     \
+                                             chunks, part, ticket, r, done_flag, seq))
     switch (nip) {
-    case 1: G6_SMALL_LAUNCH(1); break;
-    case 2: G6_SMALL_LAUNCH(2); break;
-    case 4: G6_SMALL_LAUNCH(4); break;
-    case 8: G6_SMALL_LAUNCH(8); break;
+    case 1: return G6_SMALL_LAUNCH(1);
+    case 2: return G6_SMALL_LAUNCH(2);
+    case 4: return G6_SMALL_LAUNCH(4);
+    case 8: return G6_SMALL_LAUNCH(8);
     default: return hipErrorInvalidValue;
     }
 #undef G6_SMALL_LAUNCH
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
