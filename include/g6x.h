@@ -113,7 +113,7 @@ int g6x_sync(int cluster_id);
  *                    softening predict j inside the force kernel (no
  *                    predictor launch, no JPred round trip; bitwise the
  *                    same results); 0 = predictor first   G6_FUSED_SMALL
- *   "small_fp64"     1 (default) = ABI blocks of at most 8 i with one
+ *   "small_fp64"     1 (default) = ABI blocks of at most 12 i with one
  *                    softening run as one launch: FP64 pair arithmetic with
  *                    the j-particles across lanes, the reduction in the same
  *                    kernel, up to 16 pending j updates carried in the launch

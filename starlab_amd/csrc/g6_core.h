@@ -44,7 +44,7 @@ struct Options {
     int reduce_lanes = 0;     // lanes per i in the split reduction: 0 auto, 4..64
     int wpg = 0;              // waves per force workgroup (j-splits folded in LDS): 0 auto, 1..16
     int fused_small = 1;      // ABI blocks of one i-group predict j inside the force kernel
-    int small_fp64 = 1;       // ABI blocks of at most kSmallMaxI i: one FP64 launch (k_small)
+    int small_fp64 = 1;       // ABI blocks of at most kSmallUseI i: one FP64 launch (k_small)
     int done_word = 1;        // lasthalf polls the block's completion word, not an event
     int lds_min_ni = 1 << 30; // auto: LDS kernel from this many i on (measured
                               // slower than the scalar-load kernel at every

@@ -784,7 +784,7 @@ int scatter_with_iblock(int ld)
     return 0;
 }
 
-// A block of at most kSmallMaxI i with one softening and no zero-distance
+// A block of at most kSmallUseI i with one softening and no zero-distance
 // rule: one k_small launch (FP64 pairs, the reduction inside).  Pending j
 // updates travel in its arguments when they fit and lie below nj; otherwise
 // the scatter launch runs first, as for other blocks.  With neighbour
@@ -903,7 +903,7 @@ int calc_begin(const char *who, int nj, int ni, const double (*xi)[3], const dou
     else E.last_h2.assign(ni, 0.0);
     bool lists = false;
     for (int k = 0; k < ni && h2; ++k) lists = lists || h2[k] > 0;
-    if (E.opt.small_fp64 && ngroups == 1 && !groups[0].zskip && ni <= kSmallMaxI &&
+    if (E.opt.small_fp64 && ngroups == 1 && !groups[0].zskip && ni <= kSmallUseI &&
         nj <= E.dcap) {
         if (calc_small(nj, ni, xi, vi, excl, h2, e2, groups[0].eps2, lists)) return -1;
     } else {

@@ -159,9 +159,12 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
 // partial sums; the last one to finish (a ticket counter) sums them in
 // workgroup order and writes the results.  write_pred: also write the FP64
 // prediction (xp64, vp64, m64) as the predictor kernel does.
-// blocks of up to 8 i: from about 12 i on, the i-per-lane FP32 path is as
-// fast (its cost hardly grows with ni at this size)
-constexpr int kSmallMaxI = 8;
+// i-slots of the largest instantiation; blocks of at most kSmallUseI i take
+// it: per block at 9 / 12 / 16 i it measured 31.2 / 33.3 / 36.8 us against
+// 35.9 / 35.8 / 36.0 us on the i-per-lane FP32 path at nj = 65536, and
+// 47.2 / 48.4 / 53.0 against 52.5 / 52.3 / 52.4 us at nj = 262144
+constexpr int kSmallMaxI = 16;
+constexpr int kSmallUseI = 12;
 constexpr int kSmallMaxDirty = 16;
 constexpr int kSmallMaxWG = 256;      // workgroups per launch at most
 constexpr int kSmallThreads = 256;    // threads per workgroup

@@ -1076,7 +1076,7 @@ hipError_t launch_reduce(hipStream_t s, const Partials &p, int nsplit, int ni,
 }
 
 // ---------------------------------------------------------------------------
-// Small ABI blocks in one launch (at most 8 i; kira's commonest blocks hold
+// Small ABI blocks in one launch (at most 12 i; kira's commonest blocks hold
 // 1-7 nodes).  A block this small leaves most lanes idle in the
 // i-per-lane force kernel and pays three launches (scatter, force,
 // reduction).  Here each wave takes 64 j at a time, one per lane, predicts
@@ -1455,6 +1455,7 @@ hipError_t launch_small(hipStream_t s, const SmallBlock &b, JRaw *jraw, int *jin
     case 2: return G6_SMALL_LAUNCH(2);
     case 4: return G6_SMALL_LAUNCH(4);
     case 8: return G6_SMALL_LAUNCH(8);
+    case 16: return G6_SMALL_LAUNCH(16);
     default: return hipErrorInvalidValue;
     }
 #undef G6_SMALL_LAUNCH

@@ -1,4 +1,4 @@
-"""Small ABI blocks -- at most 8 i with one softening, kira's commonest
+"""Small ABI blocks -- at most 12 i with one softening, kira's commonest
 calls -- run as one launch (k_small): FP64 pair arithmetic with the
 j-particles spread over the lanes, the split reduction inside the kernel,
 and pending j updates carried in the launch's arguments.
@@ -64,8 +64,8 @@ def _check64(acc, jerk, pot, nn, s, xp, vp, xi, vi, excl, eps2, label):
 def test_small_blocks_match_fp64_oracle(g, nj):
     """About nj/512 workgroups of 256 threads, at most 256, 64 j per wave and
     chunk: 2, 2, 4, 5 and 10 chunks per wave (the next chunk's records load
-    while one is summed); i counts 1, 2, 3, 5, 8 cover every lane grouping,
-    and 9 takes the FP32 path."""
+    while one is summed); i counts 1, 2, 3, 5, 8, 9, 12 cover every lane
+    grouping (1 to 16 i-slots), and 13 takes the FP32 path."""
     wgs = min(256, -(-nj // 512))
     chunks = -(-nj // (wgs * 256))
     s = plummer_state(nj, seed=91) if nj <= 65536 else sphere_state(nj, 91)
@@ -74,7 +74,7 @@ def test_small_blocks_match_fp64_oracle(g, nj):
     ti = 2.0 ** -12
     g.set_ti(ti)
     xp, vp = _predict(s, ti)
-    for ni in (1, 2, 3, 5, 8):
+    for ni in (1, 2, 3, 5, 8, 9, 12):
         isel = np.sort(rng.choice(nj, ni, replace=False))
         for eps2 in (1e-8, 0.0):
             acc, jerk, pot, nn = g.force(nj, s["idx"][isel], xp[isel], vp[isel], eps2)
@@ -88,9 +88,9 @@ def test_small_blocks_match_fp64_oracle(g, nj):
     excl = np.array([10 ** 7], np.int32)
     acc, jerk, pot, nn = g.force(nj, excl, xi, vi, 1e-8)
     _check64(acc, jerk, pot, nn, s, xp, vp, xi, vi, excl, 1e-8, f"nj={nj} outside i")
-    isel = np.sort(rng.choice(nj, 9, replace=False))
+    isel = np.sort(rng.choice(nj, 13, replace=False))
     g.force(nj, s["idx"][isel], xp[isel], vp[isel], 1e-8)
-    assert g.L.g6x_last_force_geometry(None, None, None) != 2   # 9 i: the FP32 path
+    assert g.L.g6x_last_force_geometry(None, None, None) != 2   # 13 i: the FP32 path
 
 
 def test_small_block_nn_ties_exact(g):
@@ -216,16 +216,16 @@ def test_small_block_repeats_bitwise(g):
 
 def _abi_block_run(g, s, nj, seed):
     """A kira-like sequence of ABI blocks: j updates carried in the launch
-    arguments (at most 16) and by the scatter launch (17), blocks of 1 to 8
-    i (one launch) and of 9 to 2048 i (the FP32 path, with j predicted in
+    arguments (at most 16) and by the scatter launch (17), blocks of 1 to 12
+    i (one launch) and of 13 to 2048 i (the FP32 path, with j predicted in
     the force kernel up to 256 i, by the predictor above), blocks with
     neighbour spheres and their lists, and a j update made between the two
     halves of a block.  Returns every result in order."""
     rng = np.random.default_rng(seed)
     out = []
     ti = 0.0
-    steps = [(1, 0), (2, 3), (4, 16), (8, 17), (3, 1), (5, 2), (9, 2), (64, 20), (300, 5),
-             (2048, 0)]
+    steps = [(1, 0), (2, 3), (4, 16), (8, 17), (3, 1), (5, 2), (12, 2), (64, 20), (300, 5),
+             (2048, 0), (13, 5)]
     for step, (ni, nup) in enumerate(steps):
         ti += 2.0 ** -12
         for a in np.sort(rng.choice(nj, nup, replace=False)):
