@@ -87,6 +87,11 @@ int g6e_stats(long long *engine_steps, double *pairs, long long *finished,
  * "kernel_us_force", "kernel_us_correct", "kernel_us_bselect",
  * "kernel_us_bitems", "kernel_us_bforce", "kernel_us_bcorrect",
  * "kernel_us_finalize", "kernel_us_fused"),
+ * "heavy_lanes" (1, the default, or 2: heavy systems alternate between two
+ * lanes with their own streams and graphs, so one lane's force kernel can
+ * overlap the other's select and correct kernels; faster on large batches
+ * when the runtime gives the two streams separate hardware queues, slower
+ * when they share one; fixed while open),
  * "max_steps" (per system; beyond it the job fails), "dt_max_bits" (largest
  * block step 2^bits, default -3),
  * "light_max" (systems with n <= light_max run in the light lane, which

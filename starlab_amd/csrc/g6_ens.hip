@@ -67,7 +67,7 @@ struct SysHdr {
     unsigned long long pmin;      // bits of the smallest candidate step (positive double)
     long long steps;              // engine steps (shared mode) or blocks (block mode)
     int off, n, phase, nsplit, jchunk, prec;   // prec: 32 or 64
-    int lane;                                  // 0 heavy, 1 light
+    int lane;                                  // 0, 4 heavy; 1 light; 2, 3 workgroup kernels
     int mode;                                  // 0 shared step, 1 block steps
     // block mode: the next block time, the bits of the smallest t + dt among
     // the particles (atomic min, +inf: none left), the block's size, the
@@ -1684,7 +1684,9 @@ struct Engine {
     int max_slots = 0;
     SysHdr *d_hdr = nullptr;
     int *d_phase = nullptr, *h_phase = nullptr;
-    Lane lanes[4];
+    // 0: heavy systems (step graphs); 1: light (n <= light_max); 2, 3:
+    // workgroup-kernel systems; 4: a second heavy lane (option heavy_lanes)
+    Lane lanes[5];
     bool items_dirty = true;
     // staging
     double *h_stage = nullptr, *d_stage = nullptr;
@@ -1715,6 +1717,7 @@ struct Engine {
                                       // measured slower than the graphs on the 4096-cluster batch)
     int wg_small_max = 512;           // lane 2: light_max < n <= this, 256 threads per system
     int wg_max_n = 2048;              // lane 3: wg_small_max < n <= this, 1024 threads per system
+    int heavy_lanes = 1;              // heavy systems split over lanes 0 and 4 when 2
     int wgb_steps = 0;                // block-step systems in the workgroup lanes: block steps
                                       // per launch of k_ens_wgb (0: the fused block path)
     double dt_max = 0.125;
@@ -2009,6 +2012,7 @@ int admit()
         si.n = it->n;
         si.nsplit = nsplit_for(it->n);
         si.lane = lane_for(it->n, it->mode);
+        if (si.lane == 0 && X.heavy_lanes == 2 && (slot & 1)) si.lane = 4;
         si.prec = it->prec;
         si.mode = it->mode;
         si.m = it->m;
@@ -2107,7 +2111,7 @@ int rebuild_items()
                             X.stream));
     }
     const bool fused_light = X.fused_steps > 0 && X.light_max <= 64;
-    for (int ln = 0; ln < 2; ++ln) {
+    for (int ln : {0, 1, 4}) {
         Lane &L = X.lanes[ln];
         // block-step systems take the fused block path, except in the fused
         // light-system kernel, which steps both modes itself
@@ -2367,7 +2371,11 @@ int open_engine(long long hbm_budget_bytes, int max_systems)
     ECHK(hipEventCreate(&X.kev[0]));
     ECHK(hipEventCreate(&X.kev[1]));
     for (double &k : X.kernel_ms) k = 0;
-    for (Lane &L : X.lanes) {
+    // streams are created heavy lanes first: the runtime deals hardware
+    // queues to streams in creation order, and the two heavy lanes must not
+    // share one to run side by side
+    for (int ln : {0, 4, 1, 2, 3}) {
+        Lane &L = X.lanes[ln];
         ECHK(hipStreamCreate(&L.stream));
         ECHK(hipEventCreate(&L.ev0));
         ECHK(hipEventCreate(&L.ev1));
@@ -2528,25 +2536,35 @@ int g6e_run(int steps)
         X.host_s[2] += since(t0);
         if (busy_count() == 0 || done >= steps) break;
         t0 = clk::now();
-        Lane &H = X.lanes[0], &Lt = X.lanes[1];
-        const bool heavy = H.n_chunks > 0 || H.n_schunks > 0;
+        Lane &Lt = X.lanes[1];
+        // the heavy lanes: 0, and 4 when heavy systems are split over two
+        // streams (one lane's force kernel then overlaps the other's
+        // latency-bound select and correct kernels)
+        const int hl[2] = {0, 4};
+        bool hv[2];
+        for (int h = 0; h < 2; ++h)
+            hv[h] = X.lanes[hl[h]].n_chunks > 0 || X.lanes[hl[h]].n_schunks > 0;
+        const bool heavy = hv[0] || hv[1];
         const bool light = Lt.n_chunks > 0 || Lt.n_schunks > 0;
         const bool med[4] = {false, false, X.lanes[2].n_chunks > 0, X.lanes[3].n_chunks > 0};
-        hipGraphExec_t eh = nullptr, el = nullptr;
+        hipGraphExec_t eh[2] = {nullptr, nullptr}, el = nullptr;
         const bool timing = X.kernel_timing != 0;
-        if (heavy && !timing && get_graph(0, eh)) return -1;
+        for (int h = 0; h < 2; ++h)
+            if (hv[h] && !timing && get_graph(hl[h], eh[h])) return -1;
         if (light && !timing && !(X.fused_steps > 0 && X.light_max <= 64) && get_graph(1, el))
             return -1;
         X.host_s[3] += since(t0);
         t0 = clk::now();
         double ms_total = 0;
-        if (heavy) {
+        for (int h = 0; h < 2; ++h) {
+            if (!hv[h]) continue;
+            Lane &H = X.lanes[hl[h]];
             ECHK(hipEventRecord(H.ev0, H.stream));
             if (timing) {
                 for (int s = 0; s < X.steps_per_graph; ++s)
-                    if (timed_step(0)) return -1;
+                    if (timed_step(hl[h])) return -1;
             } else {
-                ECHK(hipGraphLaunch(eh, H.stream));
+                ECHK(hipGraphLaunch(eh[h], H.stream));
             }
             ECHK(hipEventRecord(H.ev1, H.stream));
         }
@@ -2583,7 +2601,8 @@ int g6e_run(int steps)
             ECHK(hipEventRecord(M.ev1, M.stream));
         }
         auto others_running = [&]() {
-            if (heavy && hipStreamQuery(H.stream) == hipErrorNotReady) return true;
+            for (int h = 0; h < 2; ++h)
+                if (hv[h] && hipStreamQuery(X.lanes[hl[h]].stream) == hipErrorNotReady) return true;
             for (int ln = 2; ln < 4; ++ln)
                 if (med[ln] && hipStreamQuery(X.lanes[ln].stream) == hipErrorNotReady) return true;
             return false;
@@ -2618,7 +2637,9 @@ int g6e_run(int steps)
             } while (any_other && others_running() && replays < 4096);
         }
         double ms_other = 0;
-        if (heavy) {
+        for (int h = 0; h < 2; ++h) {
+            if (!hv[h]) continue;
+            Lane &H = X.lanes[hl[h]];
             ECHK(hipStreamSynchronize(H.stream));
             float ms = 0;
             if (hipEventElapsedTime(&ms, H.ev0, H.ev1) == hipSuccess) ms_other = std::max(ms_other, (double)ms);
@@ -2704,7 +2725,8 @@ int g6e_set_option(const char *key, long long value)
         if (X.open) return efail(-1, "%s is fixed while open", k.c_str());
         X.block_fusion = value ? 1 : 0;
     }
-    else if (k == "light_max" || k == "fused_steps" || k == "wg_steps" || k == "wgb_steps") {
+    else if (k == "light_max" || k == "fused_steps" || k == "wg_steps" || k == "wgb_steps" ||
+             k == "heavy_lanes") {
         // they decide which lane and kernel each resident system steps in:
         // changed under resident systems, a lane could stop advancing
         if (X.open) return efail(-1, "%s is fixed while open", k.c_str());
@@ -2712,6 +2734,7 @@ int g6e_set_option(const char *key, long long value)
         if (k == "light_max") X.light_max = v;
         else if (k == "fused_steps") X.fused_steps = v;
         else if (k == "wg_steps") X.wg_steps = v;
+        else if (k == "heavy_lanes") X.heavy_lanes = v == 2 ? 2 : 1;
         else X.wgb_steps = v;
     }
     else if (k == "wg_small_max" || k == "wg_max_n") {
@@ -2752,6 +2775,7 @@ long long g6e_get_option(const char *key)
     if (k == "fused_steps") return X.fused_steps;
     if (k == "wg_steps") return X.wg_steps;
     if (k == "wgb_steps") return X.wgb_steps;
+    if (k == "heavy_lanes") return X.heavy_lanes;
     if (k == "wg_small_max") return X.wg_small_max;
     if (k == "wg_max_n") return X.wg_max_n;
     if (k == "medium_steps") return X.lanes[2].steps + X.lanes[3].steps;
@@ -2760,7 +2784,7 @@ long long g6e_get_option(const char *key)
     for (int q = 0; q < 6; ++q)
         if (k == phases[q]) return (long long)(X.host_s[q] * 1e6);
     if (k == "light_steps") return X.lanes[1].steps;
-    if (k == "heavy_steps") return X.lanes[0].steps;
+    if (k == "heavy_steps") return X.lanes[0].steps + X.lanes[4].steps;
     if (k == "pool_capacity") return X.P.cap;
     if (k == "resident") return busy_count();
     if (k == "queued") return (long long)X.queue.size();

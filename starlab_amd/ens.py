@@ -124,12 +124,16 @@ class Ensemble:
 # ---------------------------------------------------------------------------
 
 # Engine options the tuner chooses among: j per split within a system (the
-# split-j shape of the step kernels), engine steps per graph replay, and
-# whether medium systems run in a workgroup kernel, one workgroup per system
-# (k_ens_wg for shared-step jobs, k_ens_wgb for block-step jobs).
-TUNE_SPACE = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32), wg_steps=(0, 32))
-TUNE_SPACE_BLOCK = dict(jsplit_target=(256, 512, 1024), steps_per_graph=(8, 32),
-                        wgb_steps=(0, 64))
+# split-j shape of the step kernels), engine steps per graph replay, whether
+# medium systems run in a workgroup kernel, one workgroup per system
+# (k_ens_wg for shared-step jobs, k_ens_wgb for block-step jobs), and whether
+# heavy systems are split over two streams.  The last pays only when the two
+# streams get hardware queues of their own, which depends on the streams the
+# process already holds, so it is measured in the process that runs the jobs.
+TUNE_SPACE = dict(jsplit_target=(256, 512), steps_per_graph=(8, 32), wg_steps=(0, 32),
+                  heavy_lanes=(1, 2))
+TUNE_SPACE_BLOCK = dict(jsplit_target=(256, 512), steps_per_graph=(8, 32), wgb_steps=(0, 64),
+                        heavy_lanes=(1, 2))
 
 
 def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192,

@@ -39,7 +39,7 @@ def _jobs(seed, sizes, t_end, eta=0.05):
 
 
 DEFAULTS = dict(jsplit_target=256, precision=64, max_steps=50_000_000, steps_per_graph=8,
-                fused_steps=256, light_max=64, wg_steps=0, block_steps=1)
+                fused_steps=256, light_max=64, wg_steps=0, block_steps=1, heavy_lanes=1)
 PATHS = pytest.mark.parametrize("fused", [256, 0], ids=["fused", "graphs"])
 
 
@@ -244,7 +244,7 @@ def test_failed_open_releases_and_bad_arguments_are_errors():
 
 
 def test_lane_options_fixed_while_open():
-    """light_max, fused_steps and wg_steps decide which lane and kernel a
+    """light_max, fused_steps, wg_steps and heavy_lanes decide which lane and kernel a
     resident system steps in: refused while an engine is open (a change
     under resident systems could leave a lane that never advances), taken
     before g6e_open, and a run with them set up front finishes."""
@@ -252,7 +252,7 @@ def test_lane_options_fixed_while_open():
     L = load()
     e = Ensemble(GB, 16, **DEFAULTS)
     try:
-        for k in ("light_max", "fused_steps", "wg_steps"):
+        for k in ("light_max", "fused_steps", "wg_steps", "heavy_lanes"):
             before = L.g6e_get_option(k.encode())
             assert L.g6e_set_option(k.encode(), 0) == -1
             assert b"fixed while open" in L.g6e_last_error()

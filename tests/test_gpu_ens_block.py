@@ -36,7 +36,7 @@ pytestmark = pytest.mark.gpu
 GB = 1 << 30
 DEFAULTS = dict(jsplit_target=256, precision=64, max_steps=50_000_000, steps_per_graph=8,
                 fused_steps=256, light_max=64, wg_steps=0, block_steps=1, block_fusion=1,
-                wgb_steps=0)
+                wgb_steps=0, heavy_lanes=1)
 PATHS = pytest.mark.parametrize("fused", [256, 0], ids=["fused", "graphs"])
 
 
@@ -192,6 +192,40 @@ def test_alone_equals_mixed_bitwise(prec, fused):
         np.testing.assert_array_equal(got["v"], ref["v"])
         assert got["steps"] == ref["steps"] and got["e1"] == ref["e1"]
     assert len(res) == len(mix) + 2 and not any(r["failed"] for r in res.values())
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_two_heavy_lanes_equal_one_bitwise(prec):
+    """Option heavy_lanes = 2 deals the heavy systems (n > light_max) to two
+    lanes, each with its own stream and graphs.  A system's arithmetic does
+    not depend on the lane that steps it: results are bitwise those of one
+    heavy lane, for block-step and shared-step systems, with j-splits, beside
+    light systems, and the engine counts the same pairs."""
+    jobs = _jobs(17, [3, 40, 65, 100, 300, 700, 1300, 2100, 150, 520], t_end=1.0 / 16)
+    out = []
+    for lanes in (1, 2):
+        e = Ensemble(2 * GB, 64, precision=prec, heavy_lanes=lanes, block_steps=1)
+        try:
+            ids = []
+            for k, j in enumerate(jobs):
+                e.option("block_steps", 0 if k % 3 == 2 else 1)
+                ids.append(e.submit(j["m"], j["x"], j["v"], j["t_end"], j["eta"], j["eps2"]))
+            res = e.run_until_idle()
+            stats = e.stats()
+            assert e.option("heavy_lanes") == lanes
+        finally:
+            e.close()
+            for k, v in DEFAULTS.items():
+                e.option(k, v)
+        out.append(([res[i] for i in ids], stats))
+    (one, s1), (two, s2) = out
+    for j, f, g in zip(jobs, one, two):
+        assert not f["failed"] and f["t"] == g["t"] == 1.0 / 16, len(j["m"])
+        assert f["steps"] == g["steps"], len(j["m"])
+        np.testing.assert_array_equal(f["x"], g["x"])
+        np.testing.assert_array_equal(f["v"], g["v"])
+        assert f["e0"] == g["e0"] and f["e1"] == g["e1"]
+    assert s1["pairs"] == s2["pairs"]
 
 
 def test_step_limit_marks_failure():
