@@ -124,16 +124,17 @@ class Ensemble:
 # ---------------------------------------------------------------------------
 
 # Engine options the tuner chooses among: j per split within a system (the
-# split-j shape of the step kernels), engine steps per graph replay, whether
-# medium systems run in a workgroup kernel, one workgroup per system
-# (k_ens_wg for shared-step jobs, k_ens_wgb for block-step jobs), and whether
-# heavy systems are split over two streams.  The last pays only when the two
-# streams get hardware queues of their own, which depends on the streams the
-# process already holds, so it is measured in the process that runs the jobs.
-TUNE_SPACE = dict(jsplit_target=(256, 512), steps_per_graph=(8, 32), wg_steps=(0, 32),
-                  heavy_lanes=(1, 2))
-TUNE_SPACE_BLOCK = dict(jsplit_target=(256, 512), steps_per_graph=(8, 32), wgb_steps=(0, 64),
-                        heavy_lanes=(1, 2))
+# split-j shape of the step kernels), engine steps per graph replay, and
+# whether medium systems run in a workgroup kernel, one workgroup per system
+# (k_ens_wg for shared-step jobs, k_ens_wgb for block-step jobs).  The option
+# heavy_lanes is not among them: on config 4's clusters two heavy lanes ran
+# the whole batch to t = 1 13 % faster, but probes of 1024 to 4096 of the
+# systems to t = 1/128 .. 1/8 scored them 0-3 % apart, within the probe's
+# noise.  The gain comes once the block schedules have spread and small
+# blocks (latency-bound select and correct kernels) dominate, which a short
+# probe does not reach.
+TUNE_SPACE = dict(jsplit_target=(256, 512), steps_per_graph=(8, 32), wg_steps=(0, 32))
+TUNE_SPACE_BLOCK = dict(jsplit_target=(256, 512), steps_per_graph=(8, 32), wgb_steps=(0, 64))
 
 
 def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192,
@@ -141,13 +142,14 @@ def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192
     """Choose the ensemble engine's shape options for a workload.
 
     Takes a size-stratified sample of `jobs` (dicts with m, x, v, eta, eps2
-    and optionally prec), runs copies of it to t = t_probe once per candidate
-    option set (the product of `space`; default TUNE_SPACE, or
-    TUNE_SPACE_BLOCK when the jobs run on block steps) on an idle engine,
-    and returns (best options, {options tuple: seconds per pair interaction},
-    seconds spent).  The jobs themselves are not touched.  `fixed` options
-    are applied to every candidate (block_steps among them: the jobs' step
-    mode, else the engine's current one)."""
+    and optionally prec and t_end), runs copies of it to t = t_probe (or the
+    job's own t_end if earlier) once per candidate option set (the product of
+    `space`; default TUNE_SPACE, or TUNE_SPACE_BLOCK when the jobs run on
+    block steps) on an idle engine, and returns (best options, {options
+    tuple: seconds per pair interaction}, seconds spent).  The jobs
+    themselves are not touched.  `fixed` options are applied to every
+    candidate (block_steps among them: the jobs' step mode, else the
+    engine's current one)."""
     import itertools
     import time as _time
     if space is None:
@@ -173,7 +175,8 @@ def autotune(jobs, sample=1024, t_probe=1.0 / 128, budget=None, max_systems=8192
         try:
             for j in probe:
                 eng.option("precision", j.get("prec", 64))
-                eng.submit(j["m"], j["x"], j["v"], t_probe, j["eta"], j["eps2"])
+                eng.submit(j["m"], j["x"], j["v"], min(t_probe, j.get("t_end", t_probe)),
+                           j["eta"], j["eps2"])
             t0 = _time.perf_counter()
             eng.run_until_idle(chunk=64)
             wall = _time.perf_counter() - t0
