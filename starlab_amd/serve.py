@@ -69,9 +69,8 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep
     result (x, v, t, steps, e0, e1, failed) in job order.
 
     Block-step jobs default to two heavy lanes (results are bitwise the same
-    as with one): config 4's 4096 clusters to t = 1 took 2.70-2.76 s against
-    3.10 s with libg6 open in the same process, and 2.82 s against
-    3.09-3.15 s alone."""
+    as with one): config 4's 4096 clusters to t = 1 took 2.52-2.73 s against
+    2.97-3.03 s on one."""
     block = opt.get("block_steps")
     if block is None:
         block = load().g6e_get_option(b"block_steps")
@@ -79,7 +78,8 @@ def run_batch(jobs, budget=64 * GB, max_systems=8192, chunk=32, tune=False, keep
         opt.setdefault("heavy_lanes", 2)
     tuned = None
     if tune:
-        mode = {k: v for k, v in opt.items() if k in ("block_steps", "block_fusion")}
+        # the tuner's candidates run with the step mode and lanes the batch will use
+        mode = {k: v for k, v in opt.items() if k in ("block_steps", "block_fusion", "heavy_lanes")}
         best, timings, secs = autotune(jobs, max_systems=max_systems, fixed=mode)
         opt = {**opt, **best}
         chunk = best.get("steps_per_graph", chunk)
@@ -190,7 +190,8 @@ def main(argv=None):
 
         python -m starlab_amd.serve batch [--count 4096 --nmin 32 --nmax 2048
                                            --t-end 0.03125 --prec 32 --tune
-                                           --block-steps 1 --kernel-timing]
+                                           --block-steps 1 --heavy-lanes 2
+                                           --kernel-timing]
         python -m starlab_amd.serve poisson [--rate 2000 --seconds 2]
     """
     import argparse
@@ -207,6 +208,8 @@ def main(argv=None):
     b.add_argument("--tune", action="store_true", help="engine shapes from ens.autotune")
     b.add_argument("--block-steps", type=int, choices=(0, 1), default=1,
                    help="1: kira's individual block steps; 0: one shared step per system")
+    b.add_argument("--heavy-lanes", type=int, choices=(1, 2), default=None,
+                   help="heavy systems on one or two streams (default: 2 for block steps)")
     b.add_argument("--kernel-timing", action="store_true",
                    help="step kernels one at a time between HIP events; adds kernel_us")
     p = sub.add_parser("poisson", help="70 %% three-body scatterings, 30 %% clusters")
@@ -219,6 +222,8 @@ def main(argv=None):
         opt = {"block_steps": a.block_steps}
         if a.kernel_timing:
             opt["kernel_timing"] = 1
+        if a.heavy_lanes is not None:
+            opt["heavy_lanes"] = a.heavy_lanes
         r = run_batch(jobs, tune=a.tune, **opt)
     else:
         r = run_poisson(serving_mix(int(a.rate * a.seconds), a.seed), a.rate)
