@@ -785,9 +785,10 @@ hipError_t launch_force(hipStream_t s, const JPred *jpred, int nj,
 // on ties, and the runner-up over everything else.  The address inside the
 // winning tile is found by recomputing FP32 r^2 exactly as the force kernel
 // did (first match in ascending address).  When the runner-up lies within
-// kNNBand of the best, or nothing matched, the group rescans all j in FP64
-// (d^2 evaluated as the FP64 CPU path, lexicographic (d^2, address) minimum)
-// and flags the i as refined.
+// kNNBand of the best, the group rescans in FP64 the j of every split whose
+// own best lies within the band (d^2 evaluated as the FP64 CPU path,
+// lexicographic (d^2, address) minimum) and flags the i as refined; when
+// nothing matched it rescans all j.
 //
 // Nearest tile in FP64: the winning tile holds the closest pair, which
 // usually dominates acc and jerk, and its FP32 sums carry that pair's
@@ -914,6 +915,7 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
     const int excl = ib.excl[i];
     int nn = 0;
     bool ambiguous = !(best < __builtin_inff()) || tile < 0 || tile == INT_MAX;
+    bool narrow = false;   // ambiguous only through the band test
     if (!ambiguous) {
         const float xih = ib.xh[i], yih = ib.yh[i], zih = ib.zh[i];
         const float xil = ib.xl[i], yil = ib.yl[i], zil = ib.zl[i];
@@ -977,7 +979,10 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         second = __builtin_fminf(second, m2);
         if (found == INT_MAX) ambiguous = true;
         else nn = jindex[found];
-        if (!(second > best * (1.0f + kNNBand))) ambiguous = true;
+        if (!(second > best * (1.0f + kNNBand))) {
+            narrow = !ambiguous;
+            ambiguous = true;
+        }
     }
     if (l == 0 && live) {
         R.acc[3 * i + 0] = A[0];
@@ -993,13 +998,29 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         const double xi = ib.x64[i], yi = ib.y64[i], zi = ib.z64[i];
         double bd = __builtin_inf();
         int ba = INT_MAX;
-        for (int j = l; j < nj; j += L) {
-            if (j == excl) continue;
-            const double dx = xp64[j] - xi;
-            const double dy = xp64[(size_t)cap + j] - yi;
-            const double dz = xp64[2 * (size_t)cap + j] - zi;
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 < bd) { bd = d2; ba = j; }
+        // FP32 r^2 is within ~2^-21 of the FP64 d^2 (+ eps2), so the FP64
+        // nearest has FP32 r^2 <= best (1 + kNNBand): when the ambiguity
+        // came only from the band test it lies in a partial row (j range
+        // [s jchunk wpg, (s + 1) jchunk wpg)) whose own best is within that
+        // bound, and only those rows are rescanned, in ascending address as
+        // the whole-range scan would.  Otherwise every j is rescanned.
+        const float thr = best * (1.0f + kNNBand);
+        const int span = NT.jchunk * NT.wpg;
+        for (int s = 0; s < (narrow ? nsplit : 1); ++s) {
+            int jb = 0, je = nj;
+            if (narrow) {
+                if (!(P.best[(size_t)s * P.ld + i] <= thr)) continue;
+                jb = s * span;
+                je = min(nj, jb + span);
+            }
+            for (int j = jb + l; j < je; j += L) {
+                if (j == excl) continue;
+                const double dx = xp64[j] - xi;
+                const double dy = xp64[(size_t)cap + j] - yi;
+                const double dz = xp64[2 * (size_t)cap + j] - zi;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 < bd) { bd = d2; ba = j; }
+            }
         }
 #pragma unroll
         for (int off = L / 2; off > 0; off >>= 1) {

@@ -194,6 +194,29 @@ def test_nn_ties_on_lattice(g, variant):
     assert g.refinements() > 0        # exact ties must go through the FP64 rescan
 
 
+def test_nn_ties_across_splits(g, variant):
+    """A 16^3 lattice in random address order: every point's nearest
+    neighbours (up to six at one exact distance) sit in different j-splits.
+    The FP64 rescan visits only the splits whose own FP32 best lies within
+    the band of the best, and must still return the lowest address among the
+    tied neighbours, as the oracle does."""
+    gr = np.arange(16, dtype=float)
+    x = np.array([[a, b, c] for a in gr for b in gr for c in gr]) * 0.125
+    x = x[np.random.default_rng(3).permutation(len(x))]
+    n = len(x)
+    s = dict(m=np.full(n, 1.0 / n), x=x, v=np.zeros_like(x), a2=np.zeros_like(x),
+             j6=np.zeros_like(x), k18=np.zeros_like(x), tj=np.zeros(n), dtj=np.ones(n),
+             idx=np.arange(n, dtype=np.int32))
+    load_j(g, s)
+    g.set_ti(0.0)
+    _, _, _, nn = g.force(n, s["idx"], x, s["v"], 0.0)
+    _, nsplit, jchunk = g.geometry()       # of the last call
+    print("geometry: nsplit", nsplit, "jchunk", jchunk, "refined", g.refinements(), flush=True)
+    _, _, _, rnn, _ = oracle.force(x, s["v"], s["idx"], x, s["v"], s["m"], s["idx"], 0.0)
+    np.testing.assert_array_equal(nn, rnn)
+    assert nsplit > 1 and g.refinements() > 0
+
+
 def test_neighbour_lists_exact(g):
     s = plummer_state(3000, seed=13)
     load_j(g, s)
