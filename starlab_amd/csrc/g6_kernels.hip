@@ -934,32 +934,51 @@ __global__ __launch_bounds__(64 * kReduceWaves) void k_reduce(
         // (as a multiset) for the tile's own runner-up, and its pairs in FP64
         float m1 = __builtin_inff(), m2 = __builtin_inff();
         double F[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (int j0 = tile; j0 < tend; j0 += L) {
-            const int j = j0 + l;
-            bool hit = false;
-            if (j < tend && j != excl) {
-                // the double-single words the force kernel used, split again
-                // from the FP64 prediction they were made from
-                const double xj = xp64[j], yj = xp64[(size_t)cap + j];
-                const double zj = xp64[2 * (size_t)cap + j];
-                float xjh, xjl, yjh, yjl, zjh, zjl;
-                split_ds(xj, xjh, xjl);
-                split_ds(yj, yjh, yjl);
-                split_ds(zj, zjh, zjl);
-                float dx, dy, dz;
-                const float r2 = ds_r2(xjh, yjh, zjh, xjl, yjl, zjl, xih, yih, zih, xil, yil,
-                                       zil, eps2, dx, dy, dz);
-                hit = __float_as_uint(r2) == bb;
-                m2 = __builtin_amdgcn_fmed3f(m1, m2, r2);
-                m1 = __builtin_fminf(m1, r2);
-                if (!(NT.zskip && r2 == 0.0f))
-                    pair64(xj - xi, yj - yi, zj - zi, NT.vp64[j] - ui,
-                           NT.vp64[(size_t)cap + j] - vi, NT.vp64[2 * (size_t)cap + j] - wi,
-                           NT.m64[j], NT.eps2d, F);
+        // L j per pass; the loads of kScanPasses passes are issued together
+        // (a 256-j tile at L = 32 is 8 passes, each a memory round trip when
+        // taken one at a time), then the passes are evaluated in address
+        // order, so sums and the first match are those of one pass at a time
+        constexpr int kScanPasses = 4;
+        for (int j00 = tile; j00 < tend; j00 += kScanPasses * L) {
+            double q[kScanPasses][7];
+#pragma unroll
+            for (int u = 0; u < kScanPasses; ++u) {
+                const int j = min(j00 + u * L + l, tend - 1);   // in range; unused past tend
+                q[u][0] = xp64[j];
+                q[u][1] = xp64[(size_t)cap + j];
+                q[u][2] = xp64[2 * (size_t)cap + j];
+                q[u][3] = NT.vp64[j];
+                q[u][4] = NT.vp64[(size_t)cap + j];
+                q[u][5] = NT.vp64[2 * (size_t)cap + j];
+                q[u][6] = NT.m64[j];
             }
-            unsigned long long m = __ballot(hit) >> gshift;
-            if (L < 64) m &= (1ull << L) - 1ull;
-            if (m && found == INT_MAX) found = j0 + __ffsll((long long)m) - 1;
+#pragma unroll
+            for (int u = 0; u < kScanPasses; ++u) {
+                const int j0 = j00 + u * L;
+                const int j = j0 + l;
+                bool hit = false;
+                if (j < tend && j != excl) {
+                    // the double-single words the force kernel used, split
+                    // again from the FP64 prediction they were made from
+                    const double xj = q[u][0], yj = q[u][1], zj = q[u][2];
+                    float xjh, xjl, yjh, yjl, zjh, zjl;
+                    split_ds(xj, xjh, xjl);
+                    split_ds(yj, yjh, yjl);
+                    split_ds(zj, zjh, zjl);
+                    float dx, dy, dz;
+                    const float r2 = ds_r2(xjh, yjh, zjh, xjl, yjl, zjl, xih, yih, zih, xil,
+                                           yil, zil, eps2, dx, dy, dz);
+                    hit = __float_as_uint(r2) == bb;
+                    m2 = __builtin_amdgcn_fmed3f(m1, m2, r2);
+                    m1 = __builtin_fminf(m1, r2);
+                    if (!(NT.zskip && r2 == 0.0f))
+                        pair64(xj - xi, yj - yi, zj - zi, q[u][3] - ui, q[u][4] - vi,
+                               q[u][5] - wi, q[u][6], NT.eps2d, F);
+                }
+                unsigned long long m = __ballot(hit) >> gshift;
+                if (L < 64) m &= (1ull << L) - 1ull;
+                if (m && found == INT_MAX) found = j0 + __ffsll((long long)m) - 1;
+            }
         }
 #pragma unroll
         for (int off = L / 2; off > 0; off >>= 1) {
