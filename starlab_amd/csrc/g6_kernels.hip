@@ -1787,14 +1787,57 @@ hipError_t launch_knn(hipStream_t s, const double *xp64, int cap, int nj, int i_
 //   k_nl_scan_slots    one workgroup: packed bases of the clipped lists
 //   k_nl<true>         the same sweep, writing each hit at its place in one
 //                      packed array (slot s's list at base[s], nb_cap at most)
+//
+// FP32 filter.  The exact test costs ~12 FP64 instructions per (i, j); both
+// sweeps first compare an FP32 r^2 against T = h2 - e2 and take the FP64
+// test only inside a band around T that bounds the FP32 error, so the lists
+// are the FP64 test's.  k_nl_prep writes FP32 copies of the j positions and,
+// per 8 j, the largest |coordinate| m8.  With u = 2^-24, each FP32 difference
+// is within delta = u (2 + u) (|x_j| + |x_i|) / (1 - u) <= 2^-22 (m8 + X_i) of
+// the exact one, so |r| moves by at most sqrt(3) delta, and with the FP32
+// rounding of the squares (3 u relative)
+//   r2 >= T  =>  r2_32 >= T - (2 sqrt(3) sqrt(T) delta + 3 u T)
+//   r2 <  T  =>  r2_32 <  T + (2 sqrt(3) sqrt(T) delta + 3 delta^2 + 3 u T)
+// The band B = 4 sqrt(T) delta + 4 delta^2 + 2^-20 T + 2^-48 (h2 + e2)
+// covers both, the FP32 rounding of T and B, and the FP64 test's own
+// rounding (relative 2^-52 of h2 + e2).  r2_32 < T - B is a hit, r2_32 >
+// T + B is not, and between them the FP64 test decides.
 // ---------------------------------------------------------------------------
+constexpr int kNlGroup = 8;   // j per step of the list sweeps, and per m8 entry
+
+__global__ __launch_bounds__(256) void k_nl_prep(const double *__restrict__ xp64, int cap,
+                                                 int nj, float *__restrict__ xf,
+                                                 float *__restrict__ yf,
+                                                 float *__restrict__ zf,
+                                                 float *__restrict__ m8)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    float m = 0.0f;
+    if (j < nj) {
+        const float a = (float)xp64[j];
+        const float b = (float)xp64[(size_t)cap + j];
+        const float c = (float)xp64[2 * (size_t)cap + j];
+        xf[j] = a;
+        yf[j] = b;
+        zf[j] = c;
+        m = fmaxf(fabsf(a), fmaxf(fabsf(b), fabsf(c)));
+    }
+    // groups of kNlGroup consecutive j are consecutive lanes of one wave
+    m = fmaxf(m, __shfl_xor(m, 1));
+    m = fmaxf(m, __shfl_xor(m, 2));
+    m = fmaxf(m, __shfl_xor(m, 4));
+    if ((j & (kNlGroup - 1)) == 0 && j < nj) m8[j / kNlGroup] = m;
+}
+
 template <bool WRITE>
 __global__ __launch_bounds__(64) void k_nl(
     const double *__restrict__ xp64, int cap, const int *__restrict__ jindex, int nj,
     IBlock ib, const double *__restrict__ h2, const double *__restrict__ e2,
     const int *__restrict__ slots, int nslots, int include_self, int jchunk, int nsplit,
     int *__restrict__ cnt, const int *__restrict__ base, int nb_cap, int *__restrict__ packed,
-    const int *__restrict__ total, int host_cap, int *__restrict__ host_packed)
+    const int *__restrict__ total, int host_cap, int *__restrict__ host_packed,
+    const float *__restrict__ xf, const float *__restrict__ yf, const float *__restrict__ zf,
+    const float *__restrict__ m8)
 {
 #pragma clang fp contract(off)
     const int n_sgroups = (nslots + 63) / 64;
@@ -1822,12 +1865,14 @@ __global__ __launch_bounds__(64) void k_nl(
     const double *__restrict__ yp = xp64 + cap;
     const double *__restrict__ zp = xp64 + 2 * (size_t)cap;
     int c = 0;
-    auto test = [&](int j, double xj, double yj, double zj) {
-        const double dx = xj - xi;
-        const double dy = yj - yi;
-        const double dz = zj - zi;
+    auto exact = [&](int j) {
+        const double dx = xp64[j] - xi;
+        const double dy = yp[j] - yi;
+        const double dz = zp[j] - zi;
         const double d2 = ((ee + dx * dx) + dy * dy) + dz * dz;
-        const bool hit = d2 < hh && j != excl;
+        return d2 < hh && j != excl;
+    };
+    auto take = [&](int j, bool hit) {
         if (WRITE) {
             if (hit && s < nslots) {
                 if (pos < lim) dst[out0 + pos] = jindex[j];
@@ -1837,21 +1882,67 @@ __global__ __launch_bounds__(64) void k_nl(
             c += hit ? 1 : 0;
         }
     };
+    // the lane's FP32 position and band terms (see above)
+    const float xif = (float)xi, yif = (float)yi, zif = (float)zi;
+    const float Xi = fmaxf(fabsf(xif), fmaxf(fabsf(yif), fabsf(zif)));
+    const double Td = hh - ee;
+    const float T = (float)Td;
+    const float sT = Td > 0.0 ? sqrtf(T) : 0.0f;
+    const float Bfix = T * 0x1p-20f + (float)(hh + ee) * 0x1p-48f;
+    // d2 >= ee >= hh when T <= 0: no j can be a hit
+    const bool none = !(Td > 0.0);
     int j = jbeg;
-    // 8 j per step: their coordinates arrive as wide scalar loads issued
-    // together, so the load latency is paid once per 8 j, not per j
-    for (; j + 8 <= jend; j += 8) {
-        double X[8], Y[8], Z[8];
+    // kNlGroup j per step (jbeg is a multiple of kNlGroup): their FP32
+    // coordinates arrive as scalar loads issued together, one step ahead of
+    // the step being tested (the last step reloads its own j)
+    float X[kNlGroup], Y[kNlGroup], Z[kNlGroup], M = 0.0f;
+    auto fetch = [&](int jf, float *Xs, float *Ys, float *Zs, float &Ms) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            X[u] = xp64[j + u];
-            Y[u] = yp[j + u];
-            Z[u] = zp[j + u];
+        for (int u = 0; u < kNlGroup; ++u) {
+            Xs[u] = xf[jf + u];
+            Ys[u] = yf[jf + u];
+            Zs[u] = zf[jf + u];
+        }
+        Ms = m8[jf / kNlGroup];
+    };
+    if (j + kNlGroup <= jend) fetch(j, X, Y, Z, M);
+    for (; j + kNlGroup <= jend; j += kNlGroup) {
+        float Xn[kNlGroup], Yn[kNlGroup], Zn[kNlGroup], Mn;
+        fetch(j + 2 * kNlGroup <= jend ? j + kNlGroup : j, Xn, Yn, Zn, Mn);
+        const float d = 0x1p-22f * (M + Xi);
+        const float B = 4.0f * sT * d + 4.0f * d * d + Bfix;
+        const float lo = T - B, hi = T + B;
+        unsigned in = 0, amb = 0;
+#pragma unroll
+        for (int u = 0; u < kNlGroup; ++u) {
+            const float dx = X[u] - xif;
+            const float dy = Y[u] - yif;
+            const float dz = Z[u] - zif;
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const bool a = !none && !(r2 < lo) && !(r2 > hi);
+            in |= (!none && r2 < lo && j + u != excl) ? 1u << u : 0u;
+            amb |= a ? 1u << u : 0u;
+        }
+        if (__ballot(amb != 0)) {
+#pragma unroll
+            for (int u = 0; u < kNlGroup; ++u)
+                if ((amb >> u) & 1u) in |= exact(j + u) ? 1u << u : 0u;
+        }
+        if (WRITE) {
+#pragma unroll
+            for (int u = 0; u < kNlGroup; ++u) take(j + u, (in >> u) & 1u);
+        } else {
+            c += __popc(in);
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) test(j + u, X[u], Y[u], Z[u]);
+        for (int u = 0; u < kNlGroup; ++u) {
+            X[u] = Xn[u];
+            Y[u] = Yn[u];
+            Z[u] = Zn[u];
+        }
+        M = Mn;
     }
-    for (; j < jend; ++j) test(j, xp64[j], yp[j], zp[j]);
+    for (; j < jend; ++j) take(j, !none && exact(j));
     if (!WRITE && s < nslots) cnt[c_at] = c;
 }
 
@@ -1940,20 +2031,30 @@ __global__ __launch_bounds__(kScanThreads) void k_nl_scan_slots(int *__restrict_
     }
 }
 
-// j-split of the list passes: about 4096 waves in all, at least 16 j per wave
+// j-split of the list passes: about 8192 waves in all (8 per SIMD: each wave
+// walks its j one scalar-loaded step at a time), at least 16 j per wave,
+// splits a multiple of kNlGroup j long
 void nl_geometry(int nslots, int nj, int &nsplit, int &jchunk)
 {
     const int n_sgroups = (nslots + 63) / 64;
-    nsplit = std::max(1, std::min(4096 / n_sgroups, (nj + 15) / 16));
+    nsplit = std::max(1, std::min(8192 / n_sgroups, (nj + 15) / 16));
     jchunk = (nj + nsplit - 1) / nsplit;
+    jchunk = (jchunk + kNlGroup - 1) / kNlGroup * kNlGroup;
     nsplit = (nj + jchunk - 1) / jchunk;
+}
+
+// scratch: cnt[nslots][nsplit], base[nslots], total, then (from a 16-B
+// boundary) the FP32 j positions xf, yf, zf [nj] and m8[nj / kNlGroup]
+static size_t nl_counts_ints(int nslots, int nsplit)
+{
+    return ((size_t)nslots * nsplit + nslots + 1 + 3) & ~(size_t)3;
 }
 
 size_t nl_scratch_ints(int nslots, int nj)
 {
     int nsplit, jchunk;
     nl_geometry(nslots, nj, nsplit, jchunk);
-    return (size_t)nslots * nsplit + nslots + 1;
+    return nl_counts_ints(nslots, nsplit) + 3 * (size_t)nj + (nj + kNlGroup - 1) / kNlGroup;
 }
 
 hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap, const int *jindex, int nj,
@@ -1967,17 +2068,21 @@ hipError_t launch_nblist(hipStream_t s, const double *xp64, int cap, const int *
     int *cnt = scratch;
     int *base = cnt + (size_t)nslots * nsplit;
     int *total = base + nslots;
+    float *xf = reinterpret_cast<float *>(scratch + nl_counts_ints(nslots, nsplit));
+    float *yf = xf + nj, *zf = yf + nj, *m8 = zf + nj;
     const dim3 grid(((nslots + 63) / 64) * nsplit);
+    hipLaunchKernelGGL(k_nl_prep, dim3((nj + 255) / 256), dim3(256), 0, s, xp64, cap, nj, xf, yf,
+                       zf, m8);
     hipLaunchKernelGGL(k_nl<false>, grid, dim3(64), 0, s, xp64, cap, jindex, nj, ib, h2, e2,
                        slots, nslots, include_self, jchunk, nsplit, cnt, base, nb_cap, packed,
-                       total, host_cap, host_packed);
+                       total, host_cap, host_packed, xf, yf, zf, m8);
     hipLaunchKernelGGL(k_nl_scan_splits, dim3(nslots), dim3(64), 0, s, cnt, nsplit, slots,
                        nb_cap, counts, base);
     hipLaunchKernelGGL(k_nl_scan_slots, dim3(1), dim3(kScanThreads), 0, s, base, nslots, total,
                        host_info);
     hipLaunchKernelGGL(k_nl<true>, grid, dim3(64), 0, s, xp64, cap, jindex, nj, ib, h2, e2,
                        slots, nslots, include_self, jchunk, nsplit, cnt, base, nb_cap, packed,
-                       total, host_cap, host_packed);
+                       total, host_cap, host_packed, xf, yf, zf, m8);
     return hipGetLastError();
 }
 

@@ -241,6 +241,43 @@ def test_neighbour_lists_exact(g):
             assert s["idx"][isel][k] in lst      # n_pipes > 48: self is listed
 
 
+@pytest.mark.parametrize("base", [0.25, 10.0, 1000.0])
+def test_neighbour_lists_exact_on_the_boundary(g, base):
+    """A 2^-10 lattice (a third of its points jittered by 1e-3 of a step)
+    at coordinates around `base`, where FP32 resolves 3e-8 (0.25) to 6e-5
+    (1000) of a coordinate: half the radii are exactly one step (lattice
+    neighbours at d2 == h2, which the strict test leaves out), half random.
+    Both list sweeps compare FP32 r^2 first and leave a band around h2 to
+    the FP64 test; the lists must be the oracle's exactly, including where
+    FP32 cannot tell any pair apart (base 1000)."""
+    a = 2.0 ** -10
+    gr = np.arange(12) * a
+    lat = np.array([[p, q, r] for p in gr for q in gr for r in gr])
+    rng = np.random.default_rng(7)
+    jit = rng.normal(0.0, a * 1e-3, lat.shape) * (np.arange(len(lat)) % 3 == 0)[:, None]
+    x = base + lat + jit
+    n = len(x)
+    z = np.zeros_like(x)
+    s = dict(m=np.full(n, 1.0 / n), x=x, v=z, a2=z, j6=z, k18=z, tj=np.zeros(n),
+             dtj=np.ones(n), idx=np.arange(n, dtype=np.int32))
+    load_j(g, s)
+    g.set_ti(0.0)
+    isel = np.arange(1, n, 3)[:512]             # not jittered
+    h2 = rng.uniform(0.5, 3.0, len(isel)) * a * a
+    h2[::2] = a * a
+    g.calc_firsthalf(n, s["idx"][isel], x[isel], z[isel], 0.0, h2)
+    rc, *_ = g.calc_lasthalf2(n, len(isel))
+    assert rc == 0
+    assert g.read_neighbour_list() == 0
+    counts, lists = oracle.neighbours(x[isel], h2, s["idx"][isel], 1, x, s["idx"], 4096)
+    assert counts.min() >= 1 and counts.max() > 10
+    for k in range(len(isel)):
+        rc, lst = g.get_neighbour_list(k, 4096)
+        assert rc == 0
+        assert len(lst) == counts[k], (k, len(lst), counts[k])
+        np.testing.assert_array_equal(lst, lists[k, :counts[k]])
+
+
 def test_neighbour_list_overflow(g):
     s = plummer_state(2000, seed=17)
     load_j(g, s)
